@@ -1,0 +1,169 @@
+/*
+ * rt_hip.h — C ABI of the MI355X (gfx950) flip-MCMC library `librt_hip.so`.
+ *
+ * This is the boundary the per-model C++ classes (DominoTiler, LozengeTiler,
+ * RectTriangleTiler, TriangleDimerTiler; randomtilings_amd/src/) drive, and what a
+ * ctypes/cgo/JNI binding would bind.  Signatures use plain pointers and sizes only.
+ * Every entry point returns RT_OK (0) or a negative RT_ERR_* code; rt_last_error()
+ * returns a thread-local message for the last failure on the calling thread.
+ *
+ * Ownership: host arrays passed in are borrowed for the duration of the call.  The
+ * library owns all device memory and one HIP stream per context.  Calls on one context
+ * are serialized by the caller (as with the reference's one in-order queue per tiler);
+ * separate contexts may be used concurrently.
+ *
+ * Tiling layouts are the reference's: an N x N row-major int32 vertex-state array
+ * (domino: DominoTiler.h:17-37, lozenge: LozengeTiler.h:8-28, rect-triangle:
+ * RectTriangleTiler.h:16-27, triangle-dimer: TriangleDimerTiler.h:13-69).
+ */
+#ifndef RT_HIP_H_
+#define RT_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define RT_OK 0
+#define RT_ERR_ARG (-1)     /* invalid argument or shape */
+#define RT_ERR_PARAMS (-2)  /* TinyMT parameter table missing or shorter than the chain needs */
+#define RT_ERR_HIP (-3)     /* HIP runtime error */
+#define RT_ERR_IO (-4)      /* parameter file missing or malformed */
+#define RT_ERR_DEVICE (-5)  /* device-side error flag (e.g. a bounded grid-barrier wait expired) */
+#define RT_ERR_UNTILEABLE (-6) /* domain has no tiling (max/min construction failed) */
+
+/* models */
+#define RT_DOMINO 0
+#define RT_LOZENGE 1
+#define RT_RECTTRI 2
+#define RT_TRIDIMER 3
+
+/* domino rules (the reference selects them by kernel file name) */
+#define RT_DOMINO_LAZY 0     /* dominokernel.cl: toggle 3<->12 iff rd < .8            */
+#define RT_DOMINO_CFTP 1     /* dominokernelCFTP.cl: rd < .5 -> 12, else -> 3           */
+#define RT_DOMINO_WEIGHTED 2 /* heat-bath, P(12) per (class, row parity); rparams = p12[4] */
+
+/* lozenge rules */
+#define RT_LOZENGE_UNIFORM 0 /* lozengekernel.cl: rd < .4 add, .4 <= rd < .8 remove     */
+#define RT_LOZENGE_QREF 1    /* qlozengekernel.cl rule with a runtime q; rparams = {q}   */
+#define RT_LOZENGE_QHEAT 2   /* heat-bath: add with prob q/(1+q), remove otherwise; {q}  */
+
+/* seeding modes for rt_chain_seed */
+#define RT_SEED_WALK 0  /* single-seed Walk: schedule RNG seeded with `seed`; TinyMT seed as the
+                           reference's Walk(t, steps, seed) derives it per model            */
+#define RT_SEED_EPOCH 1 /* epoch of Walk(t, vector steps, vector seeds): DominoTiler.cpp:79-82 */
+
+/* execution paths */
+#define RT_PATH_AUTO 0       /* persistent temporal-blocked kernel where available        */
+#define RT_PATH_PERSISTENT 1 /* persistent LDS-tiled kernel, generators resident in VGPRs */
+#define RT_PATH_STEPWISE 2   /* one Rotate + Update launch pair per step (reference shape) */
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_chain rt_chain;
+
+/* Thread-local description of the last error on this thread. */
+const char* rt_last_error(void);
+
+/* Library version string. */
+const char* rt_version(void);
+
+/* Create a context on HIP device `device` (XTiler(ctx, queue, devices, kernelPath, err),
+ * e.g. DominoTiler.h:61-67, minus the runtime OpenCL compile: kernels are AOT gfx950). */
+int rt_create(int device, rt_ctx** out);
+int rt_destroy(rt_ctx* ctx);
+
+/* TinyMT parameter table: n entries of (mat1, mat2, tmat) interleaved in `mat3`.
+ * Replaces get_params_buffer (src/TinyMT/file_reader.cpp:22-38). */
+int rt_load_params(rt_ctx* ctx, const uint32_t* mat3, int64_t n);
+
+/* Read the first n parameter sets of a tinymtdc-format file (lines starting with '#'
+ * skipped; mat1/mat2/tmat are the hex fields after the 3rd/4th/5th comma).
+ * Replaces XTiler::LoadTinyMT(path, size) (DominoTiler.cpp:109-111,
+ * file_reader.cpp:51-70,139-166). */
+int rt_load_params_file(rt_ctx* ctx, const char* path, int64_t n);
+
+/* Number of parameter sets currently loaded (0 if none). */
+int64_t rt_params_count(rt_ctx* ctx);
+
+/* Default execution path and tile configuration for chains created on this context
+ * afterwards (no reference counterpart: the reference has one kernel shape). */
+int rt_ctx_set_path(rt_ctx* ctx, int path, int32_t tile_config);
+
+/* The integer thresholds on k = u>>9 that replace the reference's float decisions for
+ * (model, rule, rparams): 24 words of domino/lozenge transition intervals
+ * [class][row parity] (A->B lo, A->B hi, B->A lo, B->A hi) followed by 36 words of the
+ * rectangle-triangle rule (gate, 3 band limits, 5x4 cycle and 12 toggle acceptance
+ * limits).  Returns the word count (60), writing them when cap >= 60.  Host-only; used
+ * to check the derivation against the reference predicates without a GPU. */
+int32_t rt_rule_thresholds(int model, int rule, const double* rparams, int32_t nrparams, uint32_t* out,
+                           int32_t cap);
+
+/* ---- whole-walk entry points (what XTiler::Walk calls) ----------------------------- */
+
+/* DominoTiler::Walk(tiling&, long steps, long seed) — DominoTiler.cpp:9-55.
+ * rparams: NULL, or p12[4] for RT_DOMINO_WEIGHTED. */
+int rt_domino_walk(rt_ctx* ctx, int rule, const double* rparams, int32_t* tiling, int32_t N, int64_t steps,
+                   int64_t seed);
+
+/* DominoTiler::Walk(tiling&, vector<long> steps, vector<long> seeds) — DominoTiler.cpp:58-107. */
+int rt_domino_walk_epochs(rt_ctx* ctx, int rule, const double* rparams, int32_t* tiling, int32_t N,
+                          const int64_t* steps, const int64_t* seeds, int32_t K);
+
+/* LozengeTiler::Walk(tiling&, int steps, long seed) — LozengeTiler.cpp:18-88.
+ * q is used by RT_LOZENGE_QREF / RT_LOZENGE_QHEAT (qlozengekernel.cl:3 hard-codes 0.99). */
+int rt_lozenge_walk(rt_ctx* ctx, int rule, double q, int32_t* tiling, int32_t N, int64_t steps, int64_t seed);
+
+/* RectTriangleTiler::Walk(tiling&, int steps, long seed) — RectTriangleTiler.cpp:17-92.
+ * weights16: NULL for the reference's compile-time weights (recttrianglekernel.cl:3-13,
+ * all 1.0), or the 16-entry per-digit weight table. */
+int rt_recttri_walk(rt_ctx* ctx, const float* weights16, int32_t* tiling, int32_t N, int64_t steps, int64_t seed);
+
+/* ---- device-resident chains (benchmarks, CFTP, batching) --------------------------- */
+
+/* Upload a tiling and split it into colour classes on the device.  rparams as in the
+ * walk entry points (domino: p12[4] or NULL; lozenge: {q}; rect-triangle: 16 weights). */
+int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int32_t nrparams,
+                    const int32_t* tiling, int32_t N, rt_chain** out);
+int rt_chain_destroy(rt_chain* ch);
+
+/* Select the execution path and, for RT_PATH_PERSISTENT, a tile configuration index
+ * (-1 = the autotuned/default choice). */
+int rt_chain_set_path(rt_chain* ch, int path, int32_t tile_config);
+
+/* (Re)seed: initialise the TinyMT generators on the device and the host colour-schedule
+ * RNG exactly as the reference's Walk does for `mode` (RT_SEED_WALK / RT_SEED_EPOCH). */
+int rt_chain_seed(rt_chain* ch, int mode, int64_t seed);
+
+/* Enqueue `steps` chain steps on the context stream (asynchronous). */
+int rt_chain_run(rt_chain* ch, int64_t steps);
+
+/* Wait for the chain's work; returns RT_ERR_DEVICE if a kernel raised its error flag. */
+int rt_chain_sync(rt_chain* ch);
+
+/* GPU milliseconds between the first and last kernel of the most recent rt_chain_run,
+ * measured with HIP events on the chain's stream (synchronizes). */
+int rt_chain_elapsed_ms(rt_chain* ch, double* ms);
+
+/* Download and merge the colour classes into an N x N tiling (synchronizes). */
+int rt_chain_read(rt_chain* ch, int32_t* tiling);
+
+/* Flip attempts per step (Rotate work-items, padding included, as in the reference). */
+int64_t rt_chain_attempts_per_step(rt_chain* ch);
+
+/* Bytes of packed tiling state on the device (all colour classes). */
+int64_t rt_chain_state_bytes(rt_chain* ch);
+
+/* Number of persistent tile configurations compiled for this chain's model. */
+int32_t rt_chain_tile_configs(rt_chain* ch);
+
+/* Describe tile configuration `idx`: rows, cols, halo, threads per workgroup. */
+int rt_chain_tile_config_info(rt_chain* ch, int32_t idx, int32_t* rows, int32_t* cols, int32_t* halo,
+                              int32_t* threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RT_HIP_H_ */

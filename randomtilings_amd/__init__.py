@@ -1,0 +1,252 @@
+"""randomtilings_amd — MI355X (gfx950) flip-MCMC for random tilings.
+
+Python face of the C ABI in ``include/rt_hip.h`` (``lib/librt_hip.so``).  The HIP library
+is the only compute path: if it is missing, importing the bindings raises, there is no
+CPU fallback.  The reference-compatible C++ classes live in ``src/`` and are built into
+``lib/librandomtilings.so``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_DIR = PKG / "lib"
+
+# status codes (rt_hip.h)
+RT_OK, RT_ERR_ARG, RT_ERR_PARAMS, RT_ERR_HIP, RT_ERR_IO, RT_ERR_DEVICE, RT_ERR_UNTILEABLE = 0, -1, -2, -3, -4, -5, -6
+# models
+RT_DOMINO, RT_LOZENGE, RT_RECTTRI, RT_TRIDIMER = 0, 1, 2, 3
+# rules
+RT_DOMINO_LAZY, RT_DOMINO_CFTP, RT_DOMINO_WEIGHTED = 0, 1, 2
+RT_LOZENGE_UNIFORM, RT_LOZENGE_QREF, RT_LOZENGE_QHEAT = 0, 1, 2
+RT_SEED_WALK, RT_SEED_EPOCH = 0, 1
+RT_PATH_AUTO, RT_PATH_PERSISTENT, RT_PATH_STEPWISE = 0, 1, 2
+
+# Every symbol include/rt_hip.h declares, with (restype, argtypes).
+_P = C.c_void_p
+_I32P = C.POINTER(C.c_int32)
+_U32P = C.POINTER(C.c_uint32)
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+_FP = C.POINTER(C.c_float)
+SIGNATURES = {
+    "rt_last_error": (C.c_char_p, []),
+    "rt_version": (C.c_char_p, []),
+    "rt_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "rt_destroy": (C.c_int, [_P]),
+    "rt_load_params": (C.c_int, [_P, _U32P, C.c_int64]),
+    "rt_load_params_file": (C.c_int, [_P, C.c_char_p, C.c_int64]),
+    "rt_params_count": (C.c_int64, [_P]),
+    "rt_ctx_set_path": (C.c_int, [_P, C.c_int, C.c_int32]),
+    "rt_rule_thresholds": (C.c_int32, [C.c_int, C.c_int, _DP, C.c_int32, _U32P, C.c_int32]),
+    "rt_domino_walk": (C.c_int, [_P, C.c_int, _DP, _I32P, C.c_int32, C.c_int64, C.c_int64]),
+    "rt_domino_walk_epochs": (C.c_int, [_P, C.c_int, _DP, _I32P, C.c_int32, _I64P, _I64P, C.c_int32]),
+    "rt_lozenge_walk": (C.c_int, [_P, C.c_int, C.c_double, _I32P, C.c_int32, C.c_int64, C.c_int64]),
+    "rt_recttri_walk": (C.c_int, [_P, _FP, _I32P, C.c_int32, C.c_int64, C.c_int64]),
+    "rt_chain_create": (C.c_int, [_P, C.c_int, C.c_int, _DP, C.c_int32, _I32P, C.c_int32, C.POINTER(_P)]),
+    "rt_chain_destroy": (C.c_int, [_P]),
+    "rt_chain_set_path": (C.c_int, [_P, C.c_int, C.c_int32]),
+    "rt_chain_seed": (C.c_int, [_P, C.c_int, C.c_int64]),
+    "rt_chain_run": (C.c_int, [_P, C.c_int64]),
+    "rt_chain_sync": (C.c_int, [_P]),
+    "rt_chain_elapsed_ms": (C.c_int, [_P, _DP]),
+    "rt_chain_read": (C.c_int, [_P, _I32P]),
+    "rt_chain_attempts_per_step": (C.c_int64, [_P]),
+    "rt_chain_state_bytes": (C.c_int64, [_P]),
+    "rt_chain_tile_configs": (C.c_int32, [_P]),
+    "rt_chain_tile_config_info": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
+}
+
+_lib = None
+
+
+class RTError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib_path() -> Path:
+    return LIB_DIR / "librt_hip.so"
+
+
+def load() -> C.CDLL:
+    """Load librt_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = lib_path()
+    if not p.exists():
+        raise RuntimeError(f"{p} not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != RT_OK:
+        raise RTError(rc, load().rt_last_error().decode())
+
+
+def _i32(a: np.ndarray):
+    return a.ctypes.data_as(_I32P)
+
+
+def rule_thresholds(model: int, rule: int, rparams=None) -> np.ndarray:
+    lib = load()
+    rp = None if rparams is None else np.ascontiguousarray(rparams, dtype=np.float64)
+    out = np.zeros(60, dtype=np.uint32)
+    n = lib.rt_rule_thresholds(model, rule, None if rp is None else rp.ctypes.data_as(_DP),
+                               0 if rp is None else len(rp), out.ctypes.data_as(_U32P), 60)
+    if n < 0:
+        _check(n)
+    return out
+
+
+class Context:
+    """One HIP device + stream + TinyMT parameter table (the reference's XTiler state)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        h = _P()
+        _check(self._lib.rt_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            self._lib.rt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def load_params(self, mat3: np.ndarray) -> None:
+        m = np.ascontiguousarray(mat3, dtype=np.uint32).reshape(-1, 3)
+        _check(self._lib.rt_load_params(self.h, m.ctypes.data_as(_U32P), m.shape[0]))
+
+    def load_params_file(self, path: str, n: int) -> None:
+        _check(self._lib.rt_load_params_file(self.h, str(path).encode(), n))
+
+    def params_count(self) -> int:
+        return int(self._lib.rt_params_count(self.h))
+
+    def set_path(self, path: int, tile_config: int = -1) -> None:
+        _check(self._lib.rt_ctx_set_path(self.h, path, tile_config))
+
+    # -- reference Walk entry points ------------------------------------------------
+    def domino_walk(self, t: np.ndarray, steps: int, seed: int, rule: int = RT_DOMINO_LAZY, p12=None) -> np.ndarray:
+        t = np.ascontiguousarray(t, dtype=np.int32).copy()
+        N = int(round(np.sqrt(t.size)))
+        rp = None if p12 is None else np.ascontiguousarray(p12, dtype=np.float64)
+        _check(self._lib.rt_domino_walk(self.h, rule, None if rp is None else rp.ctypes.data_as(_DP), _i32(t), N,
+                                        steps, seed))
+        return t
+
+    def domino_walk_epochs(self, t: np.ndarray, steps, seeds, rule: int = RT_DOMINO_CFTP, p12=None) -> np.ndarray:
+        t = np.ascontiguousarray(t, dtype=np.int32).copy()
+        N = int(round(np.sqrt(t.size)))
+        st = np.ascontiguousarray(steps, dtype=np.int64)
+        sd = np.ascontiguousarray(seeds, dtype=np.int64)
+        rp = None if p12 is None else np.ascontiguousarray(p12, dtype=np.float64)
+        _check(self._lib.rt_domino_walk_epochs(self.h, rule, None if rp is None else rp.ctypes.data_as(_DP), _i32(t),
+                                               N, st.ctypes.data_as(_I64P), sd.ctypes.data_as(_I64P), len(st)))
+        return t
+
+    def lozenge_walk(self, t: np.ndarray, steps: int, seed: int, rule: int = RT_LOZENGE_UNIFORM,
+                     q: float = 0.99) -> np.ndarray:
+        t = np.ascontiguousarray(t, dtype=np.int32).copy()
+        N = int(round(np.sqrt(t.size)))
+        _check(self._lib.rt_lozenge_walk(self.h, rule, q, _i32(t), N, steps, seed))
+        return t
+
+    def recttri_walk(self, t: np.ndarray, steps: int, seed: int, weights16=None) -> np.ndarray:
+        t = np.ascontiguousarray(t, dtype=np.int32).copy()
+        N = int(round(np.sqrt(t.size)))
+        w = None if weights16 is None else np.ascontiguousarray(weights16, dtype=np.float32)
+        _check(self._lib.rt_recttri_walk(self.h, None if w is None else w.ctypes.data_as(_FP), _i32(t), N, steps,
+                                         seed))
+        return t
+
+    def chain(self, model: int, rule: int, tiling: np.ndarray, rparams=None) -> "Chain":
+        return Chain(self, model, rule, tiling, rparams)
+
+
+class Chain:
+    """A device-resident chain (rt_chain_*)."""
+
+    def __init__(self, ctx: Context, model: int, rule: int, tiling: np.ndarray, rparams=None):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        t = np.ascontiguousarray(tiling, dtype=np.int32)
+        self.N = int(round(np.sqrt(t.size)))
+        rp = None if rparams is None else np.ascontiguousarray(rparams, dtype=np.float64)
+        h = _P()
+        _check(self._lib.rt_chain_create(ctx.h, model, rule, None if rp is None else rp.ctypes.data_as(_DP),
+                                         0 if rp is None else len(rp), _i32(t), self.N, C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            self._lib.rt_chain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_path(self, path: int, tile_config: int = -1) -> None:
+        _check(self._lib.rt_chain_set_path(self.h, path, tile_config))
+
+    def seed(self, seed: int, mode: int = RT_SEED_WALK) -> None:
+        _check(self._lib.rt_chain_seed(self.h, mode, seed))
+
+    def run(self, steps: int) -> None:
+        _check(self._lib.rt_chain_run(self.h, steps))
+
+    def sync(self) -> None:
+        _check(self._lib.rt_chain_sync(self.h))
+
+    def elapsed_ms(self) -> float:
+        v = C.c_double()
+        _check(self._lib.rt_chain_elapsed_ms(self.h, C.byref(v)))
+        return v.value
+
+    def read(self) -> np.ndarray:
+        t = np.zeros(self.N * self.N, dtype=np.int32)
+        _check(self._lib.rt_chain_read(self.h, _i32(t)))
+        return t
+
+    @property
+    def attempts_per_step(self) -> int:
+        return int(self._lib.rt_chain_attempts_per_step(self.h))
+
+    @property
+    def state_bytes(self) -> int:
+        return int(self._lib.rt_chain_state_bytes(self.h))
+
+    def tile_configs(self):
+        out = []
+        for i in range(self._lib.rt_chain_tile_configs(self.h)):
+            v = [C.c_int32() for _ in range(4)]
+            _check(self._lib.rt_chain_tile_config_info(self.h, i, *[C.byref(x) for x in v]))
+            out.append(tuple(x.value for x in v))
+        return out

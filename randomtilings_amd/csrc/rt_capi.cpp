@@ -1,0 +1,628 @@
+// rt_capi.cpp — the C ABI (include/rt_hip.h): contexts, TinyMT parameter tables, chains,
+// the host colour schedule and the reference-compatible Walk entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "rt_internal.hpp"
+
+namespace rt {
+bool domino_rule(int rule, const double* p12, FlipRule& fr);
+bool lozenge_rule(int rule, double q, FlipRule& fr);
+bool recttri_rule(const float* weights16, RectTriRule& r);
+}  // namespace rt
+
+using namespace rt;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail(RT_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint4* params = nullptr;  // (mat1, mat2, tmat, 0) per generator id
+    int64_t nparams = 0;
+    bool uniform = false;  // every entry identical -> pass as kernel arguments
+    uint32_t um1 = 0, um2 = 0, utm = 0;
+    int default_path = RT_PATH_AUTO;
+    int default_cfg = -1;
+};
+
+struct rt_chain {
+    rt_ctx* ctx = nullptr;
+    int model = 0, rule = 0;
+    int N = 0, W = 0, NC = 0, esize = 0;
+    int64_t G = 0;  // generators used by Rotate
+    RuleParams rp{};
+    void* cls[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint4* gens[2] = {nullptr, nullptr};
+    int gcur = 0;
+    uint8_t* dsched = nullptr;
+    int64_t dsched_cap = 0;
+    uint32_t* misc = nullptr;  // [0] barrier counter, [1] error flag
+    bool seeded = false;
+    std::mt19937 mt;                      // domino schedule (DominoTiler.cpp:31,81)
+    std::minstd_rand0 lcg;                // lozenge / rect-tri schedule (LozengeTiler.cpp:50-52)
+    std::uniform_int_distribution<int> d3{0, 2};
+    int path = RT_PATH_AUTO;
+    int cfg = -1;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_sched = nullptr;  // the last schedule upload has read hsched
+    bool sched_pending = false;
+    bool timed = false;
+    std::vector<uint8_t> hsched;
+};
+
+extern "C" {
+
+const char* rt_last_error(void) { return g_err.c_str(); }
+
+const char* rt_version(void) { return "randomtilings_amd 0.1 (gfx950)"; }
+
+int rt_create(int device, rt_ctx** out) {
+    if (!out) return fail(RT_ERR_ARG, "rt_create: null out");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(RT_ERR_ARG, "rt_create: device %d of %d", device, n);
+    HIPCHK(hipSetDevice(device));
+    rt_ctx* c = new rt_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(RT_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return RT_OK;
+}
+
+int rt_destroy(rt_ctx* c) {
+    if (!c) return RT_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->params) hipFree(c->params);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return RT_OK;
+}
+
+int rt_ctx_set_path(rt_ctx* c, int path, int32_t cfg) {
+    if (!c || path < RT_PATH_AUTO || path > RT_PATH_STEPWISE) return fail(RT_ERR_ARG, "rt_ctx_set_path: bad args");
+    c->default_path = path;
+    c->default_cfg = cfg;
+    return RT_OK;
+}
+
+int rt_load_params(rt_ctx* c, const uint32_t* mat3, int64_t n) {
+    if (!c || !mat3 || n <= 0) return fail(RT_ERR_ARG, "rt_load_params: bad args");
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<uint4> h((size_t)n);
+    bool uni = true;
+    for (int64_t i = 0; i < n; i++) {
+        h[i] = make_uint4(mat3[3 * i], mat3[3 * i + 1], mat3[3 * i + 2], 0u);
+        if (h[i].x != h[0].x || h[i].y != h[0].y || h[i].z != h[0].z) uni = false;
+    }
+    if (c->params) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->params));
+        c->params = nullptr;
+        c->nparams = 0;
+    }
+    HIPCHK(hipMalloc(&c->params, sizeof(uint4) * (size_t)n));
+    HIPCHK(hipMemcpy(c->params, h.data(), sizeof(uint4) * (size_t)n, hipMemcpyHostToDevice));
+    c->nparams = n;
+    c->uniform = uni;
+    c->um1 = h[0].x;
+    c->um2 = h[0].y;
+    c->utm = h[0].z;
+    return RT_OK;
+}
+
+// file_reader.cpp:114-166: the hex fields after the 3rd, 4th and 5th comma of each
+// non-'#' line are mat1, mat2, tmat.  Read sequentially (the reference reopens the file
+// and seeks for every entry).
+int rt_load_params_file(rt_ctx* c, const char* path, int64_t n) {
+    if (!c || !path || n <= 0) return fail(RT_ERR_ARG, "rt_load_params_file: bad args");
+    std::ifstream f(path);
+    if (!f) return fail(RT_ERR_IO, "parameter file not found: %s", path);
+    std::vector<uint32_t> m;
+    m.reserve((size_t)n * 3);
+    std::string line;
+    int64_t got = 0;
+    while (got < n && std::getline(f, line)) {
+        if (!line.empty() && line[0] == '#') continue;
+        if (line.empty()) return fail(RT_ERR_IO, "file format error at entry %lld", (long long)got);
+        uint32_t v[3];
+        for (int k = 0; k < 3; k++) {
+            int commas = 0;
+            size_t pos = 0;
+            for (; pos < line.size(); pos++)
+                if (line[pos] == ',' && ++commas == 3 + k) break;
+            if (pos >= line.size()) return fail(RT_ERR_IO, "comma not found in line: %s", line.c_str());
+            errno = 0;
+            v[k] = (uint32_t)strtoul(line.c_str() + pos + 1, nullptr, 16);
+        }
+        m.push_back(v[0]);
+        m.push_back(v[1]);
+        m.push_back(v[2]);
+        got++;
+    }
+    if (got < n)
+        return fail(RT_ERR_IO, "parameter file %s holds %lld entries, %lld requested", path, (long long)got,
+                    (long long)n);
+    return rt_load_params(c, m.data(), n);
+}
+
+int64_t rt_params_count(rt_ctx* c) { return c ? c->nparams : 0; }
+
+// Rule thresholds for inspection by tests: FlipRule (24 words) then RectTriRule (36).
+int32_t rt_rule_thresholds(int model, int rule, const double* rparams, int32_t nrparams, uint32_t* out, int32_t cap) {
+    RuleParams rp{};
+    if (model == RT_DOMINO) {
+        if (rule == RT_DOMINO_WEIGHTED && (!rparams || nrparams < 4)) return fail(RT_ERR_ARG, "p12[4] required");
+        if (!domino_rule(rule, rparams, rp.fr)) return fail(RT_ERR_ARG, "bad domino rule");
+    } else if (model == RT_LOZENGE) {
+        const double q = (rparams && nrparams > 0) ? rparams[0] : 0.99;
+        if (!lozenge_rule(rule, q, rp.fr)) return fail(RT_ERR_ARG, "bad lozenge rule");
+    } else if (model == RT_RECTTRI) {
+        float w[16];
+        const float* wp = nullptr;
+        if (rparams && nrparams >= 16) {
+            for (int k = 0; k < 16; k++) w[k] = (float)rparams[k];
+            wp = w;
+        }
+        recttri_rule(wp, rp.rt);
+    } else {
+        return fail(RT_ERR_ARG, "bad model");
+    }
+    const int32_t nw = (int32_t)((sizeof(FlipRule) + sizeof(RectTriRule)) / 4);
+    if (out && cap >= nw) {
+        std::memcpy(out, &rp.fr, sizeof(FlipRule));
+        std::memcpy(out + sizeof(FlipRule) / 4, &rp.rt, sizeof(RectTriRule));
+    }
+    return nw;
+}
+
+// ---- chains --------------------------------------------------------------------------
+
+static int mod3(int x) { return ((x % 3) + 3) % 3; }
+
+static void split(const rt_chain* ch, const int32_t* t, std::vector<uint8_t>& out) {
+    const int N = ch->N, W = ch->W;
+    const size_t CS = (size_t)N * W;
+    out.assign(CS * ch->NC * ch->esize, 0);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            int c, col;
+            if (ch->model == RT_DOMINO) {
+                c = (i + j) % 2;  // DominoTiler.cpp:20-21
+                col = j / 2;
+            } else if (ch->model == RT_LOZENGE) {
+                c = mod3(j - i);  // LozengeTiler.cpp:31-37
+                col = j / 3;
+            } else {
+                c = mod3(i - j);  // RectTriangleTiler.cpp:26-32
+                col = j / 3;
+            }
+            const size_t at = c * CS + (size_t)i * W + col;
+            if (ch->esize == 1) out[at] = (uint8_t)t[(size_t)i * N + j];
+            else std::memcpy(&out[at * 4], &t[(size_t)i * N + j], 4);
+        }
+}
+
+static void merge(const rt_chain* ch, const std::vector<uint8_t>& in, int32_t* t) {
+    const int N = ch->N, W = ch->W;
+    const size_t CS = (size_t)N * W;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            int c, col;
+            if (ch->model == RT_DOMINO) {
+                c = (i + j) % 2;
+                col = j / 2;
+            } else if (ch->model == RT_LOZENGE) {
+                c = mod3(j - i);
+                col = j / 3;
+            } else {
+                c = mod3(i - j);
+                col = j / 3;
+            }
+            const size_t at = c * CS + (size_t)i * W + col;
+            if (ch->esize == 1) t[(size_t)i * N + j] = in[at];
+            else std::memcpy(&t[(size_t)i * N + j], &in[at * 4], 4);
+        }
+}
+
+int rt_chain_destroy(rt_chain* ch) {
+    if (!ch) return RT_OK;
+    hipSetDevice(ch->ctx->device);
+    hipStreamSynchronize(ch->ctx->stream);
+    for (int b = 0; b < 2; b++) {
+        if (ch->cls[b]) hipFree(ch->cls[b]);
+        if (ch->gens[b]) hipFree(ch->gens[b]);
+    }
+    if (ch->dsched) hipFree(ch->dsched);
+    if (ch->misc) hipFree(ch->misc);
+    if (ch->ev0) hipEventDestroy(ch->ev0);
+    if (ch->ev1) hipEventDestroy(ch->ev1);
+    if (ch->ev_sched) hipEventDestroy(ch->ev_sched);
+    delete ch;
+    return RT_OK;
+}
+
+int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int32_t nrparams, const int32_t* tiling,
+                    int32_t N, rt_chain** out) {
+    if (!ctx || !tiling || !out) return fail(RT_ERR_ARG, "rt_chain_create: null argument");
+    rt_chain* ch = new rt_chain();
+    ch->ctx = ctx;
+    ch->model = model;
+    ch->rule = rule;
+    ch->N = N;
+    ch->path = ctx->default_path;
+    ch->cfg = ctx->default_cfg;
+    if (model == RT_DOMINO) {
+        // DominoTiler.h:34-37: 2M x 2M, twice zero-padded.  The reference only warns on odd N
+        // (DominoTiler.cpp:12) and then indexes past its arrays; odd N is rejected here.
+        if (N < 6 || N % 2) {
+            delete ch;
+            return fail(RT_ERR_ARG, "domino tiling side %d must be even and >= 6", N);
+        }
+        ch->NC = 2;
+        ch->W = N / 2;
+        ch->esize = 1;
+        ch->G = (int64_t)(N - 2) * (ch->W - 2);
+        if (rule == RT_DOMINO_WEIGHTED && (!rparams || nrparams < 4)) {
+            delete ch;
+            return fail(RT_ERR_ARG, "weighted domino rule needs p12[4]");
+        }
+        if (!domino_rule(rule, rparams, ch->rp.fr)) {
+            delete ch;
+            return fail(RT_ERR_ARG, "invalid domino rule %d or probabilities", rule);
+        }
+    } else if (model == RT_LOZENGE || model == RT_RECTTRI) {
+        // LozengeTiler.h:26-28, RectTriangleTiler.h:26-27: N divisible by 3, triple padding
+        if (N < 6 || N % 3) {
+            delete ch;
+            return fail(RT_ERR_ARG, "tiling side %d must be a multiple of 3 and >= 6", N);
+        }
+        ch->NC = 3;
+        ch->W = N / 3;
+        ch->G = (int64_t)N * ch->W;
+        if (model == RT_LOZENGE) {
+            ch->esize = 1;
+            const double q = (rparams && nrparams > 0) ? rparams[0] : 0.99;
+            if (!lozenge_rule(rule, q, ch->rp.fr)) {
+                delete ch;
+                return fail(RT_ERR_ARG, "invalid lozenge rule %d or q", rule);
+            }
+        } else {
+            ch->esize = 4;
+            float w[16];
+            const float* wp = nullptr;
+            if (rparams && nrparams >= 16) {
+                for (int k = 0; k < 16; k++) w[k] = (float)rparams[k];
+                wp = w;
+            }
+            recttri_rule(wp, ch->rp.rt);
+        }
+    } else {
+        delete ch;
+        return fail(RT_ERR_ARG, "rt_chain_create: model %d has no planar chain", model);
+    }
+    if (model == RT_LOZENGE) {
+        for (int i = 0; i < N * N; i++)
+            if (tiling[i] < 0 || tiling[i] > 255) {
+                delete ch;
+                return fail(RT_ERR_ARG, "lozenge vertex state %d out of the 8-bit range", tiling[i]);
+            }
+    } else if (model == RT_DOMINO) {
+        for (int i = 0; i < N * N; i++)
+            if (tiling[i] < 0 || tiling[i] > 255) {
+                delete ch;
+                return fail(RT_ERR_ARG, "domino vertex state %d out of the 8-bit range", tiling[i]);
+            }
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    const size_t bytes = (size_t)N * ch->W * ch->NC * ch->esize;
+    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&ch->cls[b], bytes);
+    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&ch->gens[b], sizeof(uint4) * (size_t)ch->G);
+    if (e == hipSuccess) e = hipMalloc(&ch->misc, 64);
+    if (e == hipSuccess) e = hipEventCreate(&ch->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ch->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_sched, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        rt_chain_destroy(ch);
+        return fail(RT_ERR_HIP, "chain allocation: %s", hipGetErrorString(e));
+    }
+    std::vector<uint8_t> h;
+    split(ch, tiling, h);
+    e = hipMemcpy(ch->cls[0], h.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(ch->misc, 0, 64);
+    if (e != hipSuccess) {
+        rt_chain_destroy(ch);
+        return fail(RT_ERR_HIP, "chain upload: %s", hipGetErrorString(e));
+    }
+    *out = ch;
+    return RT_OK;
+}
+
+int rt_chain_set_path(rt_chain* ch, int path, int32_t cfg) {
+    if (!ch || path < RT_PATH_AUTO || path > RT_PATH_STEPWISE) return fail(RT_ERR_ARG, "rt_chain_set_path: bad args");
+    if (cfg >= persist_config_count(ch->model)) return fail(RT_ERR_ARG, "tile config %d out of range", cfg);
+    ch->path = path;
+    ch->cfg = cfg;
+    return RT_OK;
+}
+
+int rt_chain_seed(rt_chain* ch, int mode, int64_t seed) {
+    if (!ch) return fail(RT_ERR_ARG, "rt_chain_seed: null chain");
+    rt_ctx* c = ch->ctx;
+    if (!c->params) return fail(RT_ERR_PARAMS, "no TinyMT parameter table loaded (LoadTinyMT)");
+    if (c->nparams < ch->G)
+        return fail(RT_ERR_PARAMS, "TinyMT parameter table holds %lld sets; this chain needs %lld",
+                    (long long)c->nparams, (long long)ch->G);
+    uint32_t tmseed;
+    const long lseed = (long)seed;
+    if (ch->model == RT_DOMINO) {
+        ch->mt.seed(lseed);
+        // Walk: TinyMT seeded with the first mt19937 output (DominoTiler.cpp:31-32);
+        // epoch: with the epoch seed itself (:81-82).  Both pass through a 32-bit kernel arg.
+        tmseed = (mode == RT_SEED_EPOCH) ? (uint32_t)(uint64_t)seed : (uint32_t)ch->mt();
+    } else {
+        // LozengeTiler.cpp:47-52, RectTriangleTiler.cpp:41-45
+        ch->lcg.seed(lseed);
+        ch->d3.reset();
+        tmseed = (uint32_t)(uint64_t)seed;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(launch_init_gens(ch->G, tmseed, c->uniform ? nullptr : c->params, c->um1, c->um2, c->utm,
+                            ch->gens[ch->gcur], c->stream));
+    ch->seeded = true;
+    return RT_OK;
+}
+
+static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c) {
+    const int n = persist_config_count(ch->model);
+    const bool uni = ch->ctx->uniform;
+    auto fits = [&](int idx) {
+        int tr, tc, h, nt;
+        if (!persist_config(ch->model, idx, &tr, &tc, &h, &nt)) return false;
+        const int trn = (ch->N + tr - 1) / tr, tcn = (ch->W + tc - 1) / tc;
+        const int cap = persist_capacity(ch->model, idx, uni, ch->ctx->device);
+        if (trn * tcn > cap) return false;
+        *nblocks = trn * tcn;
+        *tiles_c = tcn;
+        return true;
+    };
+    if (ch->cfg >= 0) return fits(ch->cfg) ? ch->cfg : -1;
+    // default: the configuration with the fewest redundant halo positions per owned one
+    // among those whose tile grid fits in one wave of resident workgroups
+    int best = -1;
+    double best_cost = 1e30;
+    for (int i = 0; i < n; i++) {
+        int nb, tcn;
+        if (!fits(i)) continue;
+        int tr, tc, h, nt;
+        persist_config(ch->model, i, &tr, &tc, &h, &nt);
+        (void)nb;
+        const double redund = (double)(tr + 2 * h) * (tc + 2 * h) / ((double)tr * tc);
+        const double cost = redund + 2.0 / h;  // halo compute + barrier cost per step
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = i;
+        }
+    }
+    if (best >= 0) fits(best);
+    return best;
+}
+
+int rt_chain_run(rt_chain* ch, int64_t steps) {
+    if (!ch) return fail(RT_ERR_ARG, "rt_chain_run: null chain");
+    if (!ch->seeded) return fail(RT_ERR_ARG, "rt_chain_run: chain not seeded");
+    if (steps < 0) return fail(RT_ERR_ARG, "rt_chain_run: negative steps");
+    rt_ctx* c = ch->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    if (ch->sched_pending) {  // the previous upload may still be reading hsched
+        HIPCHK(hipEventSynchronize(ch->ev_sched));
+        ch->sched_pending = false;
+    }
+    // host colour schedule, drawn exactly as the reference's Walk loop draws it
+    ch->hsched.resize((size_t)std::max<int64_t>(steps, 1));
+    for (int64_t s = 0; s < steps; s++) {
+        if (ch->model == RT_DOMINO) {
+            const int r = ch->mt() % 2;  // DominoTiler.cpp:35-36: r==1 rotates B (class 0)
+            ch->hsched[s] = (r == 1) ? 0 : 1;
+        } else {
+            ch->hsched[s] = (uint8_t)ch->d3(ch->lcg);
+        }
+    }
+    ch->timed = false;
+    if (steps == 0) return RT_OK;
+
+    int nblocks = 0, tiles_c = 0, cfg = -1;
+    if (ch->path != RT_PATH_STEPWISE) cfg = choose_persist_cfg(ch, &nblocks, &tiles_c);
+    if (ch->path == RT_PATH_PERSISTENT && cfg < 0)
+        return fail(RT_ERR_ARG, "no persistent tile configuration fits this chain on the device");
+
+    HIPCHK(hipEventRecord(ch->ev0, c->stream));
+    if (cfg < 0) {
+        StepwiseArgs a{};
+        a.rp = ch->rp;
+        a.cls = ch->cls[ch->cur];
+        a.N = ch->N;
+        a.W = ch->W;
+        a.gens = ch->gens[ch->gcur];
+        a.params = c->uniform ? nullptr : c->params;
+        a.m1 = c->um1;
+        a.m2 = c->um2;
+        a.tm = c->utm;
+        a.sched_host = ch->hsched.data();
+        a.nsteps = steps;
+        HIPCHK(launch_stepwise(ch->model, a, c->stream));
+    } else {
+        if (ch->dsched_cap < steps) {
+            if (ch->dsched) {
+                HIPCHK(hipStreamSynchronize(c->stream));
+                HIPCHK(hipFree(ch->dsched));
+                ch->dsched = nullptr;
+            }
+            HIPCHK(hipMalloc(&ch->dsched, (size_t)steps));
+            ch->dsched_cap = steps;
+        }
+        HIPCHK(hipMemcpyAsync(ch->dsched, ch->hsched.data(), (size_t)steps, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipEventRecord(ch->ev_sched, c->stream));
+        ch->sched_pending = true;
+        int tr, tc, h, nt;
+        persist_config(ch->model, cfg, &tr, &tc, &h, &nt);
+        // one launch per chunk keeps single kernels short; chunk is a multiple of h
+        const int64_t chunk = (int64_t)h * 2048;
+        for (int64_t s0 = 0; s0 < steps; s0 += chunk) {
+            const int64_t ns = std::min<int64_t>(chunk, steps - s0);
+            HIPCHK(launch_fill_u32(ch->misc, 1, 0u, c->stream));
+            PersistArgs a{};
+            a.rp = ch->rp;
+            a.cls[0] = ch->cls[0];
+            a.cls[1] = ch->cls[1];
+            a.gen_in = ch->gens[ch->gcur];
+            a.gen_out = ch->gens[ch->gcur ^ 1];
+            a.params = c->uniform ? nullptr : c->params;
+            a.m1 = c->um1;
+            a.m2 = c->um2;
+            a.tm = c->utm;
+            a.sched = ch->dsched + s0;
+            a.nsteps = ns;
+            a.N = ch->N;
+            a.W = ch->W;
+            a.tiles_c = tiles_c;
+            a.buf = ch->cur;
+            a.bar = ch->misc;
+            a.err = (int*)(ch->misc + 1);
+            a.spin_limit = 1u << 22;
+            HIPCHK(launch_persist(ch->model, cfg, c->uniform, a, nblocks, c->stream));
+            ch->gcur ^= 1;
+            const int64_t nexch = (ns + h - 1) / h;  // each block of h steps publishes once
+            if (nexch & 1) ch->cur ^= 1;
+        }
+    }
+    HIPCHK(hipEventRecord(ch->ev1, c->stream));
+    ch->timed = true;
+    return RT_OK;
+}
+
+int rt_chain_sync(rt_chain* ch) {
+    if (!ch) return fail(RT_ERR_ARG, "rt_chain_sync: null chain");
+    HIPCHK(hipSetDevice(ch->ctx->device));
+    HIPCHK(hipStreamSynchronize(ch->ctx->stream));
+    uint32_t m[2];
+    HIPCHK(hipMemcpy(m, ch->misc, sizeof(m), hipMemcpyDeviceToHost));
+    if (m[1] != 0) return fail(RT_ERR_DEVICE, "device error flag raised (grid barrier wait expired)");
+    return RT_OK;
+}
+
+int rt_chain_elapsed_ms(rt_chain* ch, double* ms) {
+    if (!ch || !ms) return fail(RT_ERR_ARG, "rt_chain_elapsed_ms: null argument");
+    if (!ch->timed) {
+        *ms = 0.0;
+        return RT_OK;
+    }
+    HIPCHK(hipEventSynchronize(ch->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, ch->ev0, ch->ev1));
+    *ms = f;
+    return RT_OK;
+}
+
+int rt_chain_read(rt_chain* ch, int32_t* tiling) {
+    if (!ch || !tiling) return fail(RT_ERR_ARG, "rt_chain_read: null argument");
+    int rc = rt_chain_sync(ch);
+    if (rc) return rc;
+    std::vector<uint8_t> h((size_t)ch->N * ch->W * ch->NC * ch->esize);
+    HIPCHK(hipMemcpy(h.data(), ch->cls[ch->cur], h.size(), hipMemcpyDeviceToHost));
+    merge(ch, h, tiling);
+    return RT_OK;
+}
+
+int64_t rt_chain_attempts_per_step(rt_chain* ch) { return ch ? ch->G : 0; }
+
+int64_t rt_chain_state_bytes(rt_chain* ch) {
+    return ch ? (int64_t)ch->N * ch->W * ch->NC * ch->esize : 0;
+}
+
+int32_t rt_chain_tile_configs(rt_chain* ch) { return ch ? persist_config_count(ch->model) : 0; }
+
+int rt_chain_tile_config_info(rt_chain* ch, int32_t idx, int32_t* rows, int32_t* cols, int32_t* halo,
+                              int32_t* threads) {
+    if (!ch || !rows || !cols || !halo || !threads) return fail(RT_ERR_ARG, "null argument");
+    int tr, tc, h, nt;
+    if (!persist_config(ch->model, idx, &tr, &tc, &h, &nt)) return fail(RT_ERR_ARG, "config %d out of range", idx);
+    *rows = tr;
+    *cols = tc;
+    *halo = h;
+    *threads = nt;
+    return RT_OK;
+}
+
+// ---- Walk entry points ----------------------------------------------------------------
+
+static int walk(rt_ctx* ctx, int model, int rule, const double* rp, int nrp, int32_t* t, int32_t N,
+                const int64_t* steps, const int64_t* seeds, int K, int mode) {
+    rt_chain* ch = nullptr;
+    int rc = rt_chain_create(ctx, model, rule, rp, nrp, t, N, &ch);
+    if (rc) return rc;
+    for (int k = 0; k < K && rc == RT_OK; k++) {
+        rc = rt_chain_seed(ch, mode, seeds[k]);
+        if (rc == RT_OK) rc = rt_chain_run(ch, steps[k]);
+    }
+    if (rc == RT_OK) rc = rt_chain_read(ch, t);
+    rt_chain_destroy(ch);
+    return rc;
+}
+
+int rt_domino_walk(rt_ctx* ctx, int rule, const double* rparams, int32_t* tiling, int32_t N, int64_t steps,
+                   int64_t seed) {
+    return walk(ctx, RT_DOMINO, rule, rparams, rparams ? 4 : 0, tiling, N, &steps, &seed, 1, RT_SEED_WALK);
+}
+
+int rt_domino_walk_epochs(rt_ctx* ctx, int rule, const double* rparams, int32_t* tiling, int32_t N,
+                          const int64_t* steps, const int64_t* seeds, int32_t K) {
+    if (K < 0 || (K > 0 && (!steps || !seeds))) return fail(RT_ERR_ARG, "rt_domino_walk_epochs: bad epochs");
+    return walk(ctx, RT_DOMINO, rule, rparams, rparams ? 4 : 0, tiling, N, steps, seeds, K, RT_SEED_EPOCH);
+}
+
+int rt_lozenge_walk(rt_ctx* ctx, int rule, double q, int32_t* tiling, int32_t N, int64_t steps, int64_t seed) {
+    return walk(ctx, RT_LOZENGE, rule, &q, 1, tiling, N, &steps, &seed, 1, RT_SEED_WALK);
+}
+
+int rt_recttri_walk(rt_ctx* ctx, const float* weights16, int32_t* tiling, int32_t N, int64_t steps, int64_t seed) {
+    double w[16];
+    if (weights16)
+        for (int k = 0; k < 16; k++) w[k] = weights16[k];
+    return walk(ctx, RT_RECTTRI, 0, weights16 ? w : nullptr, weights16 ? 16 : 0, tiling, N, &steps, &seed, 1,
+                RT_SEED_WALK);
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// rt_device.hpp — device building blocks shared by every sweep kernel (gfx950).
+//
+//  * TinyMT32 held in VGPRs: one 127-bit state per site; the parameter triple either
+//    per generator (a table entry) or uniform (kernel arguments, i.e. SGPRs).
+//  * a bounded grid barrier for persistent kernels (every wait ends: it gives up after
+//    a fixed number of polls and raises an error flag the host reports).
+//  * XCD-aware block -> tile remapping.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rt {
+
+// Rule thresholds on k = u >> 9 (the 23-bit mantissa of the reference's rd), computed on
+// the host from the reference's float/double predicates.  A transition X -> Y fires iff
+// lo <= k < hi.  Indexed [colour class][row parity].
+struct FlipRule {
+    uint32_t ab_lo[3][2], ab_hi[3][2];  // A -> B (domino 3 -> 12, lozenge 25 -> 38)
+    uint32_t ba_lo[3][2], ba_hi[3][2];  // B -> A
+};
+
+// Rectangle-triangle rule (recttrianglekernel.cl:25-118) as integer thresholds.
+struct RectTriRule {
+    uint32_t gate;        // proceed iff k < gate  (rd < 0.9f)
+    uint32_t band[3];     // rd/0.9f < .25/.5/.75  iff  k < band[b]
+    uint32_t cyc_acc[5][4];  // 5-cycle state s, band b: accept iff k < cyc_acc[s][b]
+    uint32_t tog_acc[12];    // paired toggles: accept iff k < tog_acc[t]
+};
+
+struct RuleParams {
+    FlipRule fr;
+    RectTriRule rt;
+};
+
+// TinyMT32 step + temper (tinymt32.clh:39-55, 62-73), returning k = u >> 9.
+__device__ __forceinline__ uint32_t tm_draw(uint32_t& s0, uint32_t& s1, uint32_t& s2, uint32_t& s3, uint32_t mat1,
+                                            uint32_t mat2, uint32_t tmat) {
+    uint32_t x = (s0 & 0x7fffffffu) ^ s1 ^ s2;
+    uint32_t y = s3;
+    x ^= x << 1;
+    y ^= (y >> 1) ^ x;
+    const uint32_t m = 0u - (y & 1u);
+    const uint32_t n1 = s2 ^ (m & mat1);
+    const uint32_t n2 = x ^ (y << 10) ^ (m & mat2);
+    s0 = s1;
+    s1 = n1;
+    s2 = n2;
+    s3 = y;
+    const uint32_t t1 = s0 + (s2 >> 8);
+    const uint32_t t0 = s3 ^ t1 ^ ((0u - (t1 & 1u)) & tmat);
+    return t0 >> 9;
+}
+
+// tinymt32_init (tinymt32.clh:182-203)
+__device__ __forceinline__ void tm_init(uint32_t seed, uint32_t mat1, uint32_t mat2, uint32_t tmat, uint32_t& s0,
+                                        uint32_t& s1, uint32_t& s2, uint32_t& s3) {
+    uint32_t st[4] = {seed, mat1, mat2, tmat};
+#pragma unroll
+    for (int i = 1; i < 8; i++)
+        st[i & 3] ^= (uint32_t)i + 1812433253u * (st[(i - 1) & 3] ^ (st[(i - 1) & 3] >> 30));
+    s0 = st[0];
+    s1 = st[1];
+    s2 = st[2];
+    s3 = st[3];
+    if ((s0 & 0x7fffffffu) == 0 && s1 == 0 && s2 == 0 && s3 == 0) {
+        s0 = 'T';
+        s1 = 'I';
+        s2 = 'N';
+        s3 = 'Y';
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) (void)tm_draw(s0, s1, s2, s3, mat1, mat2, tmat);
+}
+
+// blockIdx -> tile index such that the blocks an XCD receives (round-robin dispatch:
+// block b runs on XCD b % 8) form one contiguous run of tiles, so halo exchanges between
+// neighbouring tiles mostly stay inside one XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    constexpr int X = 8;
+    const int x = b % X, r = b / X;
+    const int base = nb / X, extra = nb % X;
+    const int off = x * base + (x < extra ? x : extra);
+    return off + r;
+}
+
+// Grid barrier for persistent kernels whose blocks are all resident.  `ctr` counts
+// arrivals since the launch's reset; the k-th barrier waits for k * gridDim.x arrivals.
+// The wait is bounded: after `limit` polls it sets *err and returns, and once *err is set
+// every later barrier returns at once, so every wave reaches the kernel's end.
+// Every wave's global stores have completed to L2 at the __syncthreads(); thread 0's
+// agent-scope release then writes the XCD's L2 back so other XCDs see them, and its
+// agent-scope acquire invalidates this CU's L1 and the XCD's L2 before any wave reloads.
+__device__ __forceinline__ void grid_barrier(uint32_t* ctr, uint32_t target, int* err, uint32_t limit) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t polls = 0;
+        while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+            if (++polls > limit) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+}  // namespace rt

@@ -1,0 +1,50 @@
+// rt_internal.hpp — declarations shared by the kernel and host translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rt_hip.h"
+#include "rt_device.hpp"
+
+namespace rt {
+
+struct StepwiseArgs {
+    RuleParams rp;
+    void* cls;  // NC contiguous class arrays of N*W elements
+    int N, W;
+    uint4* gens;
+    const uint4* params;  // nullptr => uniform parameters m1/m2/tm
+    uint32_t m1, m2, tm;
+    const uint8_t* sched_host;  // colour class per step
+    int64_t nsteps;
+};
+
+struct PersistArgs {
+    RuleParams rp;
+    void* cls[2];  // two copies of the NC contiguous class arrays
+    const uint4* gen_in;
+    uint4* gen_out;
+    const uint4* params;
+    uint32_t m1, m2, tm;
+    const uint8_t* sched;  // device, colour class per step
+    int64_t nsteps;
+    int N, W;
+    int tiles_c;
+    int buf;  // which copy holds the current state
+    uint32_t* bar;
+    int* err;
+    uint32_t spin_limit;
+};
+
+hipError_t launch_init_gens(int64_t G, uint32_t seed, const uint4* params, uint32_t m1, uint32_t m2, uint32_t tm,
+                            uint4* gens, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t st);
+hipError_t launch_stepwise(int model, const StepwiseArgs& a, hipStream_t st);
+
+int persist_config_count(int model);
+bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt);
+int persist_capacity(int model, int idx, bool uni, int device);
+hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st);
+
+}  // namespace rt

@@ -1,0 +1,361 @@
+// rt_kernels.hip — gfx950 kernels of the planar flip chains (domino, lozenge,
+// rectangle-triangle) and their launchers.
+//
+// Two execution paths share the model rules of rt_models.hpp:
+//   * stepwise:   one Rotate launch and one Update launch per step, generator states in
+//                 HBM (16 B AoS per generator, coalesced) — the reference's shape.
+//   * persistent: one launch runs many steps.  Each workgroup owns a TR x TC tile of
+//                 positions and keeps the tile plus an H-wide halo of every colour class in
+//                 LDS and the TinyMT states of all those positions in VGPRs.  It advances the
+//                 halo generators redundantly (their streams do not depend on the tiling, so
+//                 every copy draws the same numbers), runs H steps locally, then publishes its
+//                 owned positions to a double-buffered global copy and meets the other
+//                 workgroups at a grid barrier before reloading its halo.  Per-step HBM
+//                 traffic is zero; bit-exactness holds because each position's state after a
+//                 block of H steps depends only on inputs within distance H.
+#include "rt_internal.hpp"
+#include "rt_models.hpp"
+
+namespace rt {
+
+// ---------------------------------------------------------------------------------
+// generator initialisation (InitTinyMT, e.g. dominokernel.cl:33-40)
+// ---------------------------------------------------------------------------------
+__global__ void k_init_gens(int64_t G, uint32_t seed, const uint4* __restrict__ params, uint32_t m1, uint32_t m2,
+                            uint32_t tm, uint4* __restrict__ gens) {
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= G) return;
+    uint32_t a = m1, b = m2, c = tm;
+    if (params) {
+        const uint4 p = params[id];
+        a = p.x;
+        b = p.y;
+        c = p.z;
+    }
+    uint32_t s0, s1, s2, s3;
+    tm_init(seed + (uint32_t)id, a, b, c, s0, s1, s2, s3);
+    gens[id] = make_uint4(s0, s1, s2, s3);
+}
+
+hipError_t launch_init_gens(int64_t G, uint32_t seed, const uint4* params, uint32_t m1, uint32_t m2, uint32_t tm,
+                            uint4* gens, hipStream_t st) {
+    if (G <= 0) return hipSuccess;
+    const int nt = 256;
+    const int64_t nb = (G + nt - 1) / nt;
+    hipLaunchKernelGGL(k_init_gens, dim3((unsigned)nb), dim3(nt), 0, st, G, seed, params, m1, m2, tm, gens);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// stepwise path
+// ---------------------------------------------------------------------------------
+template <class M>
+__global__ void k_rotate(RuleParams rp, int c, typename M::T* __restrict__ cls, int N, int W,
+                         uint4* __restrict__ gens, const uint4* __restrict__ params, uint32_t m1, uint32_t m2,
+                         uint32_t tm) {
+    const int gj = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gi = blockIdx.y;
+    if (gj >= W || !M::has_gen(gi, gj, N, W)) return;
+    const int id = M::gen_id(gi, gj, N, W);
+    uint4 s = gens[id];
+    uint32_t a = m1, b = m2, t = tm;
+    if (params) {
+        const uint4 p = params[id];
+        a = p.x;
+        b = p.y;
+        t = p.z;
+    }
+    const uint32_t k = tm_draw(s.x, s.y, s.z, s.w, a, b, t);
+    gens[id] = s;
+    typename M::T* arr = cls + (size_t)c * N * W;
+    const typename M::T st = arr[gi * W + gj];
+    const typename M::T ns = M::flip(M::step_rule(rp, c), gi, st, k);
+    if (ns != st) arr[gi * W + gj] = ns;
+}
+
+template <class M>
+__global__ void k_update(int c, typename M::T* __restrict__ cls, int N, int W) {
+    const int gj = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gi = blockIdx.y;
+    if (gj >= W || !M::updatable(gi, gj, N, W)) return;
+    M::update(c, gi, cls, N * W, gi * W + gj, W);
+}
+
+template <class M>
+hipError_t launch_stepwise_t(const StepwiseArgs& a, hipStream_t st) {
+    using T = typename M::T;
+    T* cls = (T*)a.cls;
+    const int nt = 256;
+    dim3 grid((a.W + nt - 1) / nt, a.N);
+    for (int64_t s = 0; s < a.nsteps; s++) {
+        const int c = a.sched_host[s];
+        hipLaunchKernelGGL(k_rotate<M>, grid, dim3(nt), 0, st, a.rp, c, cls, a.N, a.W, a.gens, a.params, a.m1, a.m2,
+                           a.tm);
+        hipLaunchKernelGGL(k_update<M>, grid, dim3(nt), 0, st, c, cls, a.N, a.W);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stepwise(int model, const StepwiseArgs& a, hipStream_t st) {
+    switch (model) {
+        case RT_DOMINO: return launch_stepwise_t<DominoModel>(a, st);
+        case RT_LOZENGE: return launch_stepwise_t<LozengeModel>(a, st);
+        case RT_RECTTRI: return launch_stepwise_t<RectTriModel>(a, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------------
+// persistent temporal-blocked path
+// ---------------------------------------------------------------------------------
+template <class M, int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
+    using T = typename M::T;
+    constexpr int NC = M::NC;
+    constexpr int ER = TR + 2 * H, EC = TC + 2 * H;
+    constexpr int AREA = ER * EC;
+    constexpr int GPT = (AREA + NT - 1) / NT;
+    static_assert(GPT <= 32, "position masks are 32 bits");
+    __shared__ T lds[NC * AREA];
+
+    const int N = a.N, W = a.W;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tr = tile / a.tiles_c, tc = tile - (tile / a.tiles_c) * a.tiles_c;
+    const int r0 = tr * TR - H, c0 = tc * TC - H;  // global coordinates of LDS (0, 0)
+    const int tid = threadIdx.x;
+    const RuleParams& rp = a.rp;
+
+    // generator states of every tile + halo position this thread handles
+    uint32_t s0[GPT], s1[GPT], s2[GPT], s3[GPT];
+    uint32_t pm1[UNI ? 1 : GPT], pm2[UNI ? 1 : GPT], pmt[UNI ? 1 : GPT];
+    uint32_t gmask = 0, umask = 0, omask = 0;
+#pragma unroll
+    for (int g = 0; g < GPT; g++) {
+        const int p = tid + g * NT;
+        s0[g] = s1[g] = s2[g] = s3[g] = 0;
+        if (!UNI) pm1[g] = pm2[g] = pmt[g] = 0;
+        if (p < AREA) {
+            const int li = p / EC, lj = p - (p / EC) * EC;
+            const int gi = r0 + li, gj = c0 + lj;
+            if (M::has_gen(gi, gj, N, W)) {
+                gmask |= 1u << g;
+                const int id = M::gen_id(gi, gj, N, W);
+                const uint4 s = a.gen_in[id];
+                s0[g] = s.x;
+                s1[g] = s.y;
+                s2[g] = s.z;
+                s3[g] = s.w;
+                if (!UNI) {
+                    const uint4 q = a.params[id];
+                    pm1[g] = q.x;
+                    pm2[g] = q.y;
+                    pmt[g] = q.z;
+                }
+            }
+            if (li >= 1 && li <= ER - 2 && lj >= 1 && lj <= EC - 2 && M::updatable(gi, gj, N, W)) umask |= 1u << g;
+            if (li >= H && li < H + TR && lj >= H && lj < H + TC && gi < N && gj < W) omask |= 1u << g;
+        }
+    }
+
+    const size_t CS = (size_t)N * W;  // class stride in the global arrays
+    int buf = a.buf;
+    // load tile + halo of every class
+    {
+        const T* src = (const T*)a.cls[buf];
+        for (int p = tid; p < AREA; p += NT) {
+            const int li = p / EC, lj = p - (p / EC) * EC;
+            const int gi = r0 + li, gj = c0 + lj;
+            const bool in = gi >= 0 && gi < N && gj >= 0 && gj < W;
+#pragma unroll
+            for (int c = 0; c < NC; c++) lds[c * AREA + p] = in ? src[c * CS + (size_t)gi * W + gj] : (T)0;
+        }
+    }
+    __syncthreads();
+
+    uint32_t nbar = 0;
+    int64_t s = 0;
+    while (s < a.nsteps) {
+        const int64_t blk = (a.nsteps - s < H) ? (a.nsteps - s) : H;
+        for (int64_t q = 0; q < blk; q++, s++) {
+            const int c = a.sched[s];
+            T* lc = lds + c * AREA;
+            const auto sr = M::step_rule(rp, c);
+#pragma unroll
+            for (int g = 0; g < GPT; g++) {
+                if (gmask & (1u << g)) {
+                    const int p = tid + g * NT;
+                    const uint32_t k = tm_draw(s0[g], s1[g], s2[g], s3[g], UNI ? a.m1 : pm1[g], UNI ? a.m2 : pm2[g],
+                                               UNI ? a.tm : pmt[g]);
+                    const int gi = r0 + p / EC;
+                    const T st = lc[p];
+                    const T ns = M::flip(sr, gi, st, k);
+                    if (ns != st) lc[p] = ns;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int g = 0; g < GPT; g++) {
+                if (umask & (1u << g)) {
+                    const int p = tid + g * NT;
+                    const int gi = r0 + p / EC;
+                    M::update(c, gi, lds, AREA, p, EC);
+                }
+            }
+            __syncthreads();
+        }
+        // publish owned positions into the other buffer
+        {
+            T* dst = (T*)a.cls[buf ^ 1];
+            for (int p = tid; p < TR * TC; p += NT) {
+                const int oi = p / TC, oj = p - (p / TC) * TC;
+                const int gi = tr * TR + oi, gj = tc * TC + oj;
+                if (gi < N && gj < W) {
+                    const int lp = (oi + H) * EC + (oj + H);
+#pragma unroll
+                    for (int c = 0; c < NC; c++) dst[c * CS + (size_t)gi * W + gj] = lds[c * AREA + lp];
+                }
+            }
+        }
+        buf ^= 1;
+        if (s < a.nsteps) {
+            nbar++;
+            grid_barrier(a.bar, nbar * gridDim.x, a.err, a.spin_limit);
+            const T* src = (const T*)a.cls[buf];
+            for (int p = tid; p < AREA; p += NT) {
+                const int li = p / EC, lj = p - (p / EC) * EC;
+                // owned positions are already current in LDS
+                if (li >= H && li < H + TR && lj >= H && lj < H + TC) continue;
+                const int gi = r0 + li, gj = c0 + lj;
+                const bool in = gi >= 0 && gi < N && gj >= 0 && gj < W;
+#pragma unroll
+                for (int c = 0; c < NC; c++) lds[c * AREA + p] = in ? src[c * CS + (size_t)gi * W + gj] : (T)0;
+            }
+            __syncthreads();
+        }
+    }
+
+    // owned generators out
+#pragma unroll
+    for (int g = 0; g < GPT; g++) {
+        if ((gmask & omask) & (1u << g)) {
+            const int p = tid + g * NT;
+            const int li = p / EC, lj = p - (p / EC) * EC;
+            const int id = M::gen_id(r0 + li, c0 + lj, N, W);
+            a.gen_out[id] = make_uint4(s0[g], s1[g], s2[g], s3[g]);
+        }
+    }
+}
+
+// tile configurations per model: {TR, TC, H, NT}
+struct Cfg {
+    int tr, tc, h, nt;
+};
+
+#define RT_DOMINO_CFGS(X) \
+    X(32, 32, 8, 256)     \
+    X(16, 32, 4, 256)     \
+    X(64, 64, 8, 512)     \
+    X(32, 64, 8, 512)
+#define RT_LOZENGE_CFGS(X) \
+    X(64, 84, 8, 1024)     \
+    X(64, 84, 4, 1024)     \
+    X(32, 42, 4, 256)      \
+    X(32, 84, 8, 512)
+#define RT_RECTTRI_CFGS(X) \
+    X(32, 32, 4, 256)      \
+    X(48, 48, 8, 512)      \
+    X(16, 32, 4, 256)
+
+#define RT_CFG_ENTRY(a, b, c, d) Cfg{a, b, c, d},
+static const Cfg kDominoCfgs[] = {RT_DOMINO_CFGS(RT_CFG_ENTRY)};
+static const Cfg kLozengeCfgs[] = {RT_LOZENGE_CFGS(RT_CFG_ENTRY)};
+static const Cfg kRectTriCfgs[] = {RT_RECTTRI_CFGS(RT_CFG_ENTRY)};
+
+static const Cfg* cfg_table(int model, int* n) {
+    switch (model) {
+        case RT_DOMINO: *n = sizeof(kDominoCfgs) / sizeof(Cfg); return kDominoCfgs;
+        case RT_LOZENGE: *n = sizeof(kLozengeCfgs) / sizeof(Cfg); return kLozengeCfgs;
+        case RT_RECTTRI: *n = sizeof(kRectTriCfgs) / sizeof(Cfg); return kRectTriCfgs;
+    }
+    *n = 0;
+    return nullptr;
+}
+
+int persist_config_count(int model) {
+    int n = 0;
+    cfg_table(model, &n);
+    return n;
+}
+
+bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt) {
+    int n = 0;
+    const Cfg* t = cfg_table(model, &n);
+    if (!t || idx < 0 || idx >= n) return false;
+    *tr = t[idx].tr;
+    *tc = t[idx].tc;
+    *h = t[idx].h;
+    *nt = t[idx].nt;
+    return true;
+}
+
+template <class M, int TR, int TC, int H, int NT>
+static const void* kernel_ptr(bool uni) {
+    return uni ? (const void*)k_persist<M, TR, TC, H, NT, true> : (const void*)k_persist<M, TR, TC, H, NT, false>;
+}
+
+static const void* persist_kernel(int model, int idx, bool uni) {
+    int i = 0;
+#define RT_PICK(M_, a, b, c, d)                              \
+    if (i++ == idx) return kernel_ptr<M_, a, b, c, d>(uni);
+    switch (model) {
+        case RT_DOMINO: {
+#define X(a, b, c, d) RT_PICK(DominoModel, a, b, c, d)
+            RT_DOMINO_CFGS(X)
+#undef X
+        } break;
+        case RT_LOZENGE: {
+#define X(a, b, c, d) RT_PICK(LozengeModel, a, b, c, d)
+            RT_LOZENGE_CFGS(X)
+#undef X
+        } break;
+        case RT_RECTTRI: {
+#define X(a, b, c, d) RT_PICK(RectTriModel, a, b, c, d)
+            RT_RECTTRI_CFGS(X)
+#undef X
+        } break;
+    }
+#undef RT_PICK
+    return nullptr;
+}
+
+// Largest number of co-resident workgroups of this configuration on the device.
+int persist_capacity(int model, int idx, bool uni, int device) {
+    const void* k = persist_kernel(model, idx, uni);
+    int tr, tc, h, nt;
+    if (!k || !persist_config(model, idx, &tr, &tc, &h, &nt)) return 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, nt, 0) != hipSuccess) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
+    return per_cu * prop.multiProcessorCount;
+}
+
+hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st) {
+    const void* k = persist_kernel(model, idx, uni);
+    int tr, tc, h, nt;
+    if (!k || !persist_config(model, idx, &tr, &tc, &h, &nt)) return hipErrorInvalidValue;
+    void* args[] = {(void*)&a};
+    return hipLaunchKernel(k, dim3(nblocks), dim3(nt), args, 0, st);
+}
+
+}  // namespace rt

@@ -1,0 +1,187 @@
+"""Bit-exact parity of the HIP chains against the CPU oracle (through the C ABI).
+
+Every case runs the reference-compatible Walk on the GPU and the oracle's sequential
+restatement on the CPU with the same tiling, seed and TinyMT parameter table, and requires
+identical final tilings.  Both execution paths (stepwise launches and the persistent
+LDS-tiled kernel, every compiled tile configuration) are covered, with the uniform
+parameter table (kernel-argument parameters) and a per-generator scrambled table.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import randomtilings_amd as rt
+from randomtilings_amd.params import default_table, scrambled_table
+from tests.tilings import domino_legal, lozenge_legal
+
+pytestmark = pytest.mark.gpu
+
+BIG = 1 << 20
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def gctx():
+    c = rt.Context(0)
+    c.load_params(default_table(BIG))
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def sctx():
+    c = rt.Context(0)
+    c.load_params(scrambled_table(BIG, 11))
+    yield c
+    c.close()
+
+
+def paths(model_cfgs):
+    out = [(rt.RT_PATH_STEPWISE, -1)]
+    out += [(rt.RT_PATH_PERSISTENT, i) for i in range(model_cfgs)]
+    return out
+
+
+N_DOMINO_CFGS = 4
+N_LOZENGE_CFGS = 4
+N_RECTTRI_CFGS = 3
+
+
+@pytest.mark.parametrize("path,cfg", paths(N_DOMINO_CFGS))
+@pytest.mark.parametrize("rule", [rt.RT_DOMINO_LAZY, rt.RT_DOMINO_CFTP])
+def test_domino_walk_parity(gctx, golden, rule, path, cfg):
+    for n, steps in [(1, 37), (3, 200), (16, 301), (64, 517)]:
+        t0 = golden[f"domino_aztec{n}_min"]
+        say("domino", rule, path, cfg, n, steps)
+        gctx.set_path(path, cfg)
+        try:
+            g = gctx.domino_walk(t0, steps, 1234 + n, rule=rule)
+        finally:
+            gctx.set_path(rt.RT_PATH_AUTO, -1)
+        o = O.domino_walk(t0, steps, 1234 + n, default_table(BIG), rule=rule)
+        assert np.array_equal(g, o), (n, steps)
+        assert domino_legal(g, golden[f"domino_aztec{n}_domain"])
+
+
+def test_domino_aztec128_long(gctx, golden):
+    t0 = golden["domino_aztec128_min"]
+    say("domino aztec128 5000 steps")
+    g = gctx.domino_walk(t0, 5000, 20261203, rule=rt.RT_DOMINO_CFTP)
+    o = O.domino_walk(t0, 5000, 20261203, default_table(BIG), rule=1)
+    assert np.array_equal(g, o)
+
+
+def test_domino_scrambled_params(sctx, golden):
+    t0 = golden["domino_aztec16_max"]
+    for path, cfg in [(rt.RT_PATH_STEPWISE, -1), (rt.RT_PATH_PERSISTENT, 0), (rt.RT_PATH_PERSISTENT, 2)]:
+        say("domino scrambled", path, cfg)
+        sctx.set_path(path, cfg)
+        try:
+            g = sctx.domino_walk(t0, 403, 99, rule=rt.RT_DOMINO_LAZY)
+        finally:
+            sctx.set_path(rt.RT_PATH_AUTO, -1)
+        o = O.domino_walk(t0, 403, 99, scrambled_table(BIG, 11), rule=0)
+        assert np.array_equal(g, o)
+
+
+def test_domino_epochs_parity(gctx, golden):
+    t0 = golden["domino_aztec8_max"]
+    steps = [256, 256, 128]
+    seeds = [345903, 17, 2**40 + 5]
+    say("domino epochs")
+    g = gctx.domino_walk_epochs(t0, steps, seeds, rule=rt.RT_DOMINO_CFTP)
+    o = O.domino_walk_epochs(t0, steps, seeds, default_table(BIG), rule=1)
+    assert np.array_equal(g, o)
+
+
+def test_domino_weighted_parity(gctx, golden):
+    t0 = golden["domino_aztec16_min"]
+    p12 = [0.5, 0.5, 0.2, 0.8]
+    say("domino weighted")
+    g = gctx.domino_walk(t0, 600, 5, rule=rt.RT_DOMINO_WEIGHTED, p12=p12)
+    o = O.domino_walk(t0, 600, 5, default_table(BIG), rule=2, p12=p12)
+    assert np.array_equal(g, o)
+
+
+@pytest.mark.parametrize("path,cfg", paths(N_LOZENGE_CFGS))
+@pytest.mark.parametrize("rule,q", [(rt.RT_LOZENGE_UNIFORM, 0.99), (rt.RT_LOZENGE_QREF, 0.999),
+                                    (rt.RT_LOZENGE_QHEAT, 0.7)])
+def test_lozenge_walk_parity(gctx, golden, rule, q, path, cfg):
+    for key, steps in [("hex3_0_min", 50), ("hex10_2_max", 333), ("hex50_0_min", 407)]:
+        t0 = golden[f"lozenge_{key}"]
+        say("lozenge", rule, q, path, cfg, key, steps)
+        gctx.set_path(path, cfg)
+        try:
+            g = gctx.lozenge_walk(t0, steps, 77, rule=rule, q=q)
+        finally:
+            gctx.set_path(rt.RT_PATH_AUTO, -1)
+        o = O.lozenge_walk(t0, steps, 77, default_table(BIG), rule=rule, q=q)
+        assert np.array_equal(g, o), key
+        assert lozenge_legal(g)
+
+
+def test_lozenge_scrambled_params(sctx, golden):
+    t0 = golden["lozenge_hex20_0_max"]
+    for path, cfg in [(rt.RT_PATH_STEPWISE, -1), (rt.RT_PATH_PERSISTENT, 0), (rt.RT_PATH_PERSISTENT, 2)]:
+        say("lozenge scrambled", path, cfg)
+        sctx.set_path(path, cfg)
+        try:
+            g = sctx.lozenge_walk(t0, 250, 3, rule=rt.RT_LOZENGE_QREF, q=0.9)
+        finally:
+            sctx.set_path(rt.RT_PATH_AUTO, -1)
+        o = O.lozenge_walk(t0, 250, 3, scrambled_table(BIG, 11), rule=1, q=0.9)
+        assert np.array_equal(g, o)
+
+
+@pytest.mark.parametrize("path,cfg", paths(N_RECTTRI_CFGS))
+def test_recttri_walk_parity(gctx, golden, path, cfg):
+    for key, steps in [("minHex5", 100), ("maxHex10", 300), ("slopeHex20", 250)]:
+        t0 = golden[f"recttri_{key}"]
+        say("recttri", path, cfg, key, steps)
+        gctx.set_path(path, cfg)
+        try:
+            g = gctx.recttri_walk(t0, steps, 31)
+        finally:
+            gctx.set_path(rt.RT_PATH_AUTO, -1)
+        o = O.recttri_walk(t0, steps, 31, default_table(BIG))
+        assert np.array_equal(g, o), key
+
+
+def test_recttri_weighted_parity(gctx, golden):
+    w = np.random.default_rng(3).uniform(0.3, 1.7, 16).astype(np.float32)
+    w[0] = 0
+    t0 = golden["recttri_minHex10"]
+    say("recttri weighted")
+    g = gctx.recttri_walk(t0, 400, 8, weights16=w)
+    o = O.recttri_walk(t0, 400, 8, default_table(BIG), weights16=w)
+    assert np.array_equal(g, o)
+
+
+def test_chain_runs_compose(gctx, golden):
+    """run(a) then run(b) on a device chain equals one Walk of a+b steps."""
+    t0 = golden["lozenge_hex20_0_min"]
+    ch = gctx.chain(rt.RT_LOZENGE, rt.RT_LOZENGE_QREF, t0, [0.999])
+    ch.seed(42)
+    for s in [1, 7, 100, 13]:
+        ch.run(s)
+    g = ch.read()
+    ch.close()
+    o = O.lozenge_walk(t0, 121, 42, default_table(BIG), rule=1, q=0.999)
+    assert np.array_equal(g, o)
+
+
+def test_zero_steps_is_identity(gctx, golden):
+    t0 = golden["domino_aztec5_max"]
+    assert np.array_equal(gctx.domino_walk(t0, 0, 1), t0)
+
+
+def test_short_param_table_is_an_error(golden):
+    c = rt.Context(0)
+    c.load_params(default_table(10))
+    with pytest.raises(rt.RTError) as e:
+        c.domino_walk(golden["domino_aztec5_max"], 10, 1)
+    assert e.value.code == rt.RT_ERR_PARAMS
+    c.close()
