@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Benchmark: lozenge q-volume chain on the a = b = c = 1000 hexagon, one MI355X.
+
+Workload: LozengeTiler::AlmostHexDomain(1000, 0) (tiling side N = 2007, 1,342,683
+generators, three 2007 x 669 uint8 colour classes), started from the empty room (the
+minimal tiling), the reference's q-rule (qlozengekernel.cl) with q = 0.999, the
+reference's colour schedule (minstd_rand0 + uniform_int_distribution(0, 2)), TinyMT32 with
+the default parameter set per generator.  Synthetic: no data is read.
+
+One bench "step" = one rt_chain_run of --chain-steps chain steps (each chain step flips one
+colour class: 1,342,683 flip attempts, padding included as in the reference).  The timed
+region is bracketed by stream synchronizations; `value` is flip attempts per second of
+wall-clock time with the state resident in HBM.  The roofline block uses HIP events recorded
+on the library's own stream around the same region.
+
+    python bench.py [--steps K] [--warmup W] [--chain-steps S] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "flip attempts/sec + HBM GB/s fraction; sweeps-to-equilibrium warm vs plain"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--chain-steps", type=int, default=8192)
+    ap.add_argument("--hex", type=int, default=1000)
+    ap.add_argument("--q", type=float, default=0.999)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--cpu-steps", type=int, default=400)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-config", type=int, default=-1)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (independent chains shard "
+              "across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import randomtilings_amd as rt
+    from randomtilings_amd import statics
+    from randomtilings_amd.params import default_table
+
+    d = statics.call("lozenge.AlmostHexDomain", args=[args.hex, 0])
+    t0 = statics.call("lozenge.MinTiling", d)
+    N = int(round(np.sqrt(t0.size)))
+    G = N * (N // 3)
+    params = default_table(G)
+
+    ctx = rt.Context(0)
+    ctx.load_params(params)
+    ch = ctx.chain(rt.RT_LOZENGE, rt.RT_LOZENGE_QREF, t0, [args.q])
+    if args.tile_config >= 0:
+        ch.set_path(rt.RT_PATH_PERSISTENT, args.tile_config)
+    ch.seed(args.seed)
+    S = args.chain_steps
+    print(f"# bench: lozenge hexagon {args.hex} (N={N}), q={args.q}, {S} chain steps per bench step, "
+          f"{args.warmup} warmup + {args.steps} timed", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ch.run(S)
+    ch.sync()
+
+    ctx.sync()
+    ctx.mark(0)
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        ch.run(S)
+    ctx.mark(1)
+    ch.sync()
+    w1 = time.perf_counter()
+    gpu_ms = ctx.elapsed_ms(0, 1)
+    wall = w1 - w0
+    attempts = ch.attempts_per_step
+    total_attempts = args.steps * S * attempts
+    value = total_attempts / wall
+
+    # roofline: algorithmic bytes = read + write of the packed state per chain step
+    state_bytes = ch.state_bytes
+    launches_per_run = max(1, ch.last_launches)
+    n_launch = args.steps * launches_per_run
+    steps_per_launch = S / launches_per_run
+    launch_s = gpu_ms / 1e3 / n_launch
+    bytes_per_launch = 2.0 * state_bytes * steps_per_launch
+    achieved = bytes_per_launch / launch_s / 1e9
+
+    traffic = None
+    tf = ROOT / "profiles" / "traffic_lozenge_hex1000.json"
+    if tf.exists():
+        try:
+            tj = json.loads(tf.read_text())
+            traffic = tj.get("bytes_per_chain_step")
+            if traffic is not None:
+                traffic = float(traffic) * steps_per_launch
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        import oracle
+
+        cpu_s = oracle.lozenge_time(t0, args.cpu_steps, args.seed, params, rule=rt.RT_LOZENGE_QREF, q=args.q)
+        # parity on the same sample: the GPU walk of the same steps must match the oracle bit for bit
+        g = ctx.lozenge_walk(t0, args.cpu_steps, args.seed, rule=rt.RT_LOZENGE_QREF, q=args.q)
+        o = oracle.lozenge_walk(t0, args.cpu_steps, args.seed, params, rule=rt.RT_LOZENGE_QREF, q=args.q)
+        cpu = {
+            "value": args.cpu_steps * attempts / cpu_s,
+            "unit": "flip attempts/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"oracle (sequential C++ restatement) LozengeTiler Walk of {args.cpu_steps} steps on the same "
+                      f"N={N} hexagon, q-rule q={args.q}, host split/merge included; GPU result bit-identical: "
+                      f"{bool(np.array_equal(g, o))}",
+        }
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "flip attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (deterministic hexagon domain, TinyMT default parameter table)",
+        "config": {
+            "workload": f"lozenge q-volume chain, hexagon a=b=c={args.hex} (AlmostHexDomain({args.hex},0), N={N}), "
+                        f"q={args.q} reference q-rule, empty-room start, {S} chain steps per bench step",
+            "chain_steps_per_step": S,
+            "attempts_per_chain_step": attempts,
+            "sweeps_per_step": S / 3.0,
+            "parallelism": "single GPU",
+            "tile_config": ch.tile_configs()[ch.last_config] if ch.last_config >= 0 else "stepwise",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+        "gpu_kernel_ms_per_chain_step": gpu_ms / (args.steps * S),
+    }
+    print(json.dumps(line), flush=True)
+    ch.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -69,6 +69,12 @@ const char* rt_last_error(void);
 /* Library version string. */
 const char* rt_version(void);
 
+/* Number of HIP devices (0 when none or the runtime is unavailable). */
+int rt_device_count(void);
+
+/* Marketing name of HIP device `device` into buf (NUL-terminated, truncated to cap). */
+int rt_device_name(int device, char* buf, int32_t cap);
+
 /* Create a context on HIP device `device` (XTiler(ctx, queue, devices, kernelPath, err),
  * e.g. DominoTiler.h:61-67, minus the runtime OpenCL compile: kernels are AOT gfx950). */
 int rt_create(int device, rt_ctx** out);
@@ -99,6 +105,15 @@ int rt_ctx_set_path(rt_ctx* ctx, int path, int32_t tile_config);
  * to check the derivation against the reference predicates without a GPU. */
 int32_t rt_rule_thresholds(int model, int rule, const double* rparams, int32_t nrparams, uint32_t* out,
                            int32_t cap);
+
+/* Record HIP event `slot` (0..15) on the context's stream, and read the GPU time between
+ * two recorded slots (synchronizes on `b`).  Used to time a region of queued work on the
+ * stream the kernels actually run on. */
+int rt_ctx_mark(rt_ctx* ctx, int32_t slot);
+int rt_ctx_elapsed_ms(rt_ctx* ctx, int32_t a, int32_t b, double* ms);
+
+/* Wait for all work queued on the context's stream. */
+int rt_ctx_sync(rt_ctx* ctx);
 
 /* ---- whole-walk entry points (what XTiler::Walk calls) ----------------------------- */
 
@@ -154,6 +169,11 @@ int64_t rt_chain_attempts_per_step(rt_chain* ch);
 
 /* Bytes of packed tiling state on the device (all colour classes). */
 int64_t rt_chain_state_bytes(rt_chain* ch);
+
+/* Kernel launches issued by the most recent rt_chain_run, and the tile configuration it
+ * used (-1 for the stepwise path). */
+int64_t rt_chain_last_launches(rt_chain* ch);
+int32_t rt_chain_last_config(rt_chain* ch);
 
 /* Number of persistent tile configurations compiled for this chain's model. */
 int32_t rt_chain_tile_configs(rt_chain* ch);

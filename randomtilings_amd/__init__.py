@@ -36,12 +36,17 @@ _FP = C.POINTER(C.c_float)
 SIGNATURES = {
     "rt_last_error": (C.c_char_p, []),
     "rt_version": (C.c_char_p, []),
+    "rt_device_count": (C.c_int, []),
+    "rt_device_name": (C.c_int, [C.c_int, C.c_char_p, C.c_int32]),
     "rt_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "rt_destroy": (C.c_int, [_P]),
     "rt_load_params": (C.c_int, [_P, _U32P, C.c_int64]),
     "rt_load_params_file": (C.c_int, [_P, C.c_char_p, C.c_int64]),
     "rt_params_count": (C.c_int64, [_P]),
     "rt_ctx_set_path": (C.c_int, [_P, C.c_int, C.c_int32]),
+    "rt_ctx_mark": (C.c_int, [_P, C.c_int32]),
+    "rt_ctx_elapsed_ms": (C.c_int, [_P, C.c_int32, C.c_int32, _DP]),
+    "rt_ctx_sync": (C.c_int, [_P]),
     "rt_rule_thresholds": (C.c_int32, [C.c_int, C.c_int, _DP, C.c_int32, _U32P, C.c_int32]),
     "rt_domino_walk": (C.c_int, [_P, C.c_int, _DP, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_domino_walk_epochs": (C.c_int, [_P, C.c_int, _DP, _I32P, C.c_int32, _I64P, _I64P, C.c_int32]),
@@ -57,6 +62,8 @@ SIGNATURES = {
     "rt_chain_read": (C.c_int, [_P, _I32P]),
     "rt_chain_attempts_per_step": (C.c_int64, [_P]),
     "rt_chain_state_bytes": (C.c_int64, [_P]),
+    "rt_chain_last_launches": (C.c_int64, [_P]),
+    "rt_chain_last_config": (C.c_int32, [_P]),
     "rt_chain_tile_configs": (C.c_int32, [_P]),
     "rt_chain_tile_config_info": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
 }
@@ -150,6 +157,17 @@ class Context:
     def set_path(self, path: int, tile_config: int = -1) -> None:
         _check(self._lib.rt_ctx_set_path(self.h, path, tile_config))
 
+    def mark(self, slot: int) -> None:
+        _check(self._lib.rt_ctx_mark(self.h, slot))
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        v = C.c_double()
+        _check(self._lib.rt_ctx_elapsed_ms(self.h, a, b, C.byref(v)))
+        return v.value
+
+    def sync(self) -> None:
+        _check(self._lib.rt_ctx_sync(self.h))
+
     # -- reference Walk entry points ------------------------------------------------
     def domino_walk(self, t: np.ndarray, steps: int, seed: int, rule: int = RT_DOMINO_LAZY, p12=None) -> np.ndarray:
         t = np.ascontiguousarray(t, dtype=np.int32).copy()
@@ -242,6 +260,14 @@ class Chain:
     @property
     def state_bytes(self) -> int:
         return int(self._lib.rt_chain_state_bytes(self.h))
+
+    @property
+    def last_launches(self) -> int:
+        return int(self._lib.rt_chain_last_launches(self.h))
+
+    @property
+    def last_config(self) -> int:
+        return int(self._lib.rt_chain_last_config(self.h))
 
     def tile_configs(self):
         out = []

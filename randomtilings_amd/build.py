@@ -37,7 +37,7 @@ HIP_UNITS = ["rt_kernels.hip", "rt_tridimer.hip", "rt_capi.cpp", "rt_rules.cpp"]
 CXX_UNITS = [
     "common/common.cpp", "TinyMT/file_reader.cpp", "Domino/DominoTiler.cpp",
     "Lozenge/LozengeTiler.cpp", "RectTriangle/RectTriangleTiler.cpp",
-    "TriangleDimer/TriangleDimerTiler.cpp",
+    "TriangleDimer/TriangleDimerTiler.cpp", "statics_capi.cpp",
 ]
 
 
@@ -59,7 +59,12 @@ def _hip_obj(unit: str) -> Path:
 def _cxx_obj(unit: str) -> Path:
     src = SRC / unit
     out = OBJ / ("api_" + Path(unit).stem + ".o")
-    _run(["g++", *HOST_FLAGS, "-c", str(src), "-o", str(out)])
+    have = []
+    if (SRC / "RectTriangle" / "RectTriangleTiler.cpp").exists():
+        have.append("-DRT_HAVE_RECTTRI")
+    if (SRC / "TriangleDimer" / "TriangleDimerTiler.cpp").exists():
+        have.append("-DRT_HAVE_TRIDIMER")
+    _run(["g++", *HOST_FLAGS, *have, "-c", str(src), "-o", str(out)])
     return out
 
 

@@ -25,6 +25,10 @@ using namespace rt;
 
 static thread_local std::string g_err;
 
+// exchange flags per chain: one 128-byte line per tile of the persistent grid
+static constexpr int kMaxBlocks = 4096;
+static constexpr size_t kBarBytes = (size_t)kMaxBlocks * kLine * sizeof(uint32_t);
+
 static int fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -50,12 +54,14 @@ struct rt_ctx {
     uint32_t um1 = 0, um2 = 0, utm = 0;
     int default_path = RT_PATH_AUTO;
     int default_cfg = -1;
+    hipEvent_t marks[16] = {};
 };
 
 struct rt_chain {
     rt_ctx* ctx = nullptr;
     int model = 0, rule = 0;
     int N = 0, W = 0, NC = 0, esize = 0;
+    int P = 0;  // row pitch in elements: rows start 16-byte aligned
     int64_t G = 0;  // generators used by Rotate
     RuleParams rp{};
     void* cls[2] = {nullptr, nullptr};
@@ -64,7 +70,9 @@ struct rt_chain {
     int gcur = 0;
     uint8_t* dsched = nullptr;
     int64_t dsched_cap = 0;
-    uint32_t* misc = nullptr;  // [0] barrier counter, [1] error flag
+    uint32_t* misc = nullptr;  // [0] unused, [1] error flag
+    uint32_t* bar = nullptr;   // barrier lines: kMaxBlocks arrival lines + 8 release lines
+    uint32_t epoch = 0;        // barrier epochs completed once queued launches finish
     bool seeded = false;
     std::mt19937 mt;                      // domino schedule (DominoTiler.cpp:31,81)
     std::minstd_rand0 lcg;                // lozenge / rect-tri schedule (LozengeTiler.cpp:50-52)
@@ -76,6 +84,8 @@ struct rt_chain {
     bool sched_pending = false;
     bool timed = false;
     std::vector<uint8_t> hsched;
+    int64_t last_launches = 0;
+    int32_t last_cfg = -1;
 };
 
 extern "C" {
@@ -83,6 +93,20 @@ extern "C" {
 const char* rt_last_error(void) { return g_err.c_str(); }
 
 const char* rt_version(void) { return "randomtilings_amd 0.1 (gfx950)"; }
+
+int rt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int rt_device_name(int device, char* buf, int32_t cap) {
+    if (!buf || cap <= 0) return fail(RT_ERR_ARG, "rt_device_name: bad buffer");
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, device));
+    snprintf(buf, (size_t)cap, "%s (%s)", p.name, p.gcnArchName);
+    return RT_OK;
+}
 
 int rt_create(int device, rt_ctx** out) {
     if (!out) return fail(RT_ERR_ARG, "rt_create: null out");
@@ -105,9 +129,36 @@ int rt_destroy(rt_ctx* c) {
     if (!c) return RT_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    for (hipEvent_t& e : c->marks)
+        if (e) hipEventDestroy(e);
     if (c->params) hipFree(c->params);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
+    return RT_OK;
+}
+
+int rt_ctx_mark(rt_ctx* c, int32_t slot) {
+    if (!c || slot < 0 || slot >= 16) return fail(RT_ERR_ARG, "rt_ctx_mark: bad slot");
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->marks[slot]) HIPCHK(hipEventCreate(&c->marks[slot]));
+    HIPCHK(hipEventRecord(c->marks[slot], c->stream));
+    return RT_OK;
+}
+
+int rt_ctx_elapsed_ms(rt_ctx* c, int32_t a, int32_t b, double* ms) {
+    if (!c || !ms || a < 0 || b < 0 || a >= 16 || b >= 16 || !c->marks[a] || !c->marks[b])
+        return fail(RT_ERR_ARG, "rt_ctx_elapsed_ms: unrecorded slot");
+    HIPCHK(hipEventSynchronize(c->marks[b]));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->marks[a], c->marks[b]));
+    *ms = f;
+    return RT_OK;
+}
+
+int rt_ctx_sync(rt_ctx* c) {
+    if (!c) return fail(RT_ERR_ARG, "rt_ctx_sync: null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return RT_OK;
 }
 
@@ -213,7 +264,7 @@ int32_t rt_rule_thresholds(int model, int rule, const double* rparams, int32_t n
 static int mod3(int x) { return ((x % 3) + 3) % 3; }
 
 static void split(const rt_chain* ch, const int32_t* t, std::vector<uint8_t>& out) {
-    const int N = ch->N, W = ch->W;
+    const int N = ch->N, W = ch->P;
     const size_t CS = (size_t)N * W;
     out.assign(CS * ch->NC * ch->esize, 0);
     for (int i = 0; i < N; i++)
@@ -236,7 +287,7 @@ static void split(const rt_chain* ch, const int32_t* t, std::vector<uint8_t>& ou
 }
 
 static void merge(const rt_chain* ch, const std::vector<uint8_t>& in, int32_t* t) {
-    const int N = ch->N, W = ch->W;
+    const int N = ch->N, W = ch->P;
     const size_t CS = (size_t)N * W;
     for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++) {
@@ -267,6 +318,7 @@ int rt_chain_destroy(rt_chain* ch) {
     }
     if (ch->dsched) hipFree(ch->dsched);
     if (ch->misc) hipFree(ch->misc);
+    if (ch->bar) hipFree(ch->bar);
     if (ch->ev0) hipEventDestroy(ch->ev0);
     if (ch->ev1) hipEventDestroy(ch->ev1);
     if (ch->ev_sched) hipEventDestroy(ch->ev_sched);
@@ -346,11 +398,13 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
                 return fail(RT_ERR_ARG, "domino vertex state %d out of the 8-bit range", tiling[i]);
             }
     }
+    ch->P = (int)((((size_t)ch->W * ch->esize + 15) / 16) * 16 / ch->esize);
     hipError_t e = hipSetDevice(ctx->device);
-    const size_t bytes = (size_t)N * ch->W * ch->NC * ch->esize;
+    const size_t bytes = (size_t)N * ch->P * ch->NC * ch->esize;
     for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&ch->cls[b], bytes);
     for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&ch->gens[b], sizeof(uint4) * (size_t)ch->G);
     if (e == hipSuccess) e = hipMalloc(&ch->misc, 64);
+    if (e == hipSuccess) e = hipMalloc(&ch->bar, kBarBytes);
     if (e == hipSuccess) e = hipEventCreate(&ch->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ch->ev1);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_sched, hipEventDisableTiming);
@@ -361,7 +415,9 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
     std::vector<uint8_t> h;
     split(ch, tiling, h);
     e = hipMemcpy(ch->cls[0], h.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ch->cls[1], h.data(), bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(ch->misc, 0, 64);
+    if (e == hipSuccess) e = hipMemset(ch->bar, 0, kBarBytes);
     if (e != hipSuccess) {
         rt_chain_destroy(ch);
         return fail(RT_ERR_HIP, "chain upload: %s", hipGetErrorString(e));
@@ -413,7 +469,7 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c) {
         if (!persist_config(ch->model, idx, &tr, &tc, &h, &nt)) return false;
         const int trn = (ch->N + tr - 1) / tr, tcn = (ch->W + tc - 1) / tc;
         const int cap = persist_capacity(ch->model, idx, uni, ch->ctx->device);
-        if (trn * tcn > cap) return false;
+        if (trn * tcn > cap || trn * tcn > kMaxBlocks) return false;
         *nblocks = trn * tcn;
         *tiles_c = tcn;
         return true;
@@ -461,6 +517,7 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         }
     }
     ch->timed = false;
+    ch->last_launches = 0;
     if (steps == 0) return RT_OK;
 
     int nblocks = 0, tiles_c = 0, cfg = -1;
@@ -468,13 +525,14 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
     if (ch->path == RT_PATH_PERSISTENT && cfg < 0)
         return fail(RT_ERR_ARG, "no persistent tile configuration fits this chain on the device");
 
-    HIPCHK(hipEventRecord(ch->ev0, c->stream));
     if (cfg < 0) {
+        HIPCHK(hipEventRecord(ch->ev0, c->stream));
         StepwiseArgs a{};
         a.rp = ch->rp;
         a.cls = ch->cls[ch->cur];
         a.N = ch->N;
         a.W = ch->W;
+        a.P = ch->P;
         a.gens = ch->gens[ch->gcur];
         a.params = c->uniform ? nullptr : c->params;
         a.m1 = c->um1;
@@ -483,6 +541,8 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         a.sched_host = ch->hsched.data();
         a.nsteps = steps;
         HIPCHK(launch_stepwise(ch->model, a, c->stream));
+        ch->last_launches = 2 * steps;
+        ch->last_cfg = -1;
     } else {
         if (ch->dsched_cap < steps) {
             if (ch->dsched) {
@@ -498,11 +558,11 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         ch->sched_pending = true;
         int tr, tc, h, nt;
         persist_config(ch->model, cfg, &tr, &tc, &h, &nt);
+        HIPCHK(hipEventRecord(ch->ev0, c->stream));
         // one launch per chunk keeps single kernels short; chunk is a multiple of h
         const int64_t chunk = (int64_t)h * 2048;
         for (int64_t s0 = 0; s0 < steps; s0 += chunk) {
             const int64_t ns = std::min<int64_t>(chunk, steps - s0);
-            HIPCHK(launch_fill_u32(ch->misc, 1, 0u, c->stream));
             PersistArgs a{};
             a.rp = ch->rp;
             a.cls[0] = ch->cls[0];
@@ -517,16 +577,21 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
             a.nsteps = ns;
             a.N = ch->N;
             a.W = ch->W;
+            a.P = ch->P;
             a.tiles_c = tiles_c;
             a.buf = ch->cur;
-            a.bar = ch->misc;
+            a.flags = ch->bar;
+            a.epoch0 = ch->epoch;
             a.err = (int*)(ch->misc + 1);
             a.spin_limit = 1u << 22;
             HIPCHK(launch_persist(ch->model, cfg, c->uniform, a, nblocks, c->stream));
             ch->gcur ^= 1;
-            const int64_t nexch = (ns + h - 1) / h;  // each block of h steps publishes once
-            if (nexch & 1) ch->cur ^= 1;
+            const int64_t nblk = (ns + h - 1) / h;  // each block of h steps publishes once
+            ch->epoch += (uint32_t)(nblk - 1);
+            if (nblk & 1) ch->cur ^= 1;
+            ch->last_launches++;
         }
+        ch->last_cfg = cfg;
     }
     HIPCHK(hipEventRecord(ch->ev1, c->stream));
     ch->timed = true;
@@ -560,7 +625,7 @@ int rt_chain_read(rt_chain* ch, int32_t* tiling) {
     if (!ch || !tiling) return fail(RT_ERR_ARG, "rt_chain_read: null argument");
     int rc = rt_chain_sync(ch);
     if (rc) return rc;
-    std::vector<uint8_t> h((size_t)ch->N * ch->W * ch->NC * ch->esize);
+    std::vector<uint8_t> h((size_t)ch->N * ch->P * ch->NC * ch->esize);
     HIPCHK(hipMemcpy(h.data(), ch->cls[ch->cur], h.size(), hipMemcpyDeviceToHost));
     merge(ch, h, tiling);
     return RT_OK;
@@ -571,6 +636,10 @@ int64_t rt_chain_attempts_per_step(rt_chain* ch) { return ch ? ch->G : 0; }
 int64_t rt_chain_state_bytes(rt_chain* ch) {
     return ch ? (int64_t)ch->N * ch->W * ch->NC * ch->esize : 0;
 }
+
+int64_t rt_chain_last_launches(rt_chain* ch) { return ch ? ch->last_launches : 0; }
+
+int32_t rt_chain_last_config(rt_chain* ch) { return ch ? ch->last_cfg : -1; }
 
 int32_t rt_chain_tile_configs(rt_chain* ch) { return ch ? persist_config_count(ch->model) : 0; }
 

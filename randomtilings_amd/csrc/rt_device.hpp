@@ -2,8 +2,8 @@
 //
 //  * TinyMT32 held in VGPRs: one 127-bit state per site; the parameter triple either
 //    per generator (a table entry) or uniform (kernel arguments, i.e. SGPRs).
-//  * a bounded grid barrier for persistent kernels (every wait ends: it gives up after
-//    a fixed number of polls and raises an error flag the host reports).
+//  * epoch flags for the persistent kernels' neighbour synchronisation (every wait ends:
+//    it gives up after a fixed number of polls and raises an error flag the host reports).
 //  * XCD-aware block -> tile remapping.
 #pragma once
 
@@ -84,30 +84,14 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return off + r;
 }
 
-// Grid barrier for persistent kernels whose blocks are all resident.  `ctr` counts
-// arrivals since the launch's reset; the k-th barrier waits for k * gridDim.x arrivals.
-// The wait is bounded: after `limit` polls it sets *err and returns, and once *err is set
-// every later barrier returns at once, so every wave reaches the kernel's end.
-// Every wave's global stores have completed to L2 at the __syncthreads(); thread 0's
-// agent-scope release then writes the XCD's L2 back so other XCDs see them, and its
-// agent-scope acquire invalidates this CU's L1 and the XCD's L2 before any wave reloads.
-__device__ __forceinline__ void grid_barrier(uint32_t* ctr, uint32_t target, int* err, uint32_t limit) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t polls = 0;
-        while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-            if (++polls > limit) {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    __syncthreads();
-}
+// Flag lines of the persistent kernels' neighbour synchronisation: one 128-byte line per
+// tile holding the last block epoch the tile published.  Epochs increase monotonically
+// across launches (the host tracks them) and are compared modulo 2^32.  Every wait on a
+// flag is bounded: after a fixed number of polls the waiter raises an error flag and stops
+// waiting, and a raised error flag ends every later wait at once, so every wave reaches
+// the kernel's end.
+constexpr int kLine = 32;  // words per flag line
+
+__device__ __forceinline__ bool epoch_reached(uint32_t v, uint32_t epoch) { return (int32_t)(v - epoch) >= 0; }
 
 }  // namespace rt

@@ -11,8 +11,8 @@ namespace rt {
 
 struct StepwiseArgs {
     RuleParams rp;
-    void* cls;  // NC contiguous class arrays of N*W elements
-    int N, W;
+    void* cls;  // NC contiguous class arrays of N rows of pitch P elements
+    int N, W, P;
     uint4* gens;
     const uint4* params;  // nullptr => uniform parameters m1/m2/tm
     uint32_t m1, m2, tm;
@@ -29,10 +29,11 @@ struct PersistArgs {
     uint32_t m1, m2, tm;
     const uint8_t* sched;  // device, colour class per step
     int64_t nsteps;
-    int N, W;
+    int N, W, P;
     int tiles_c;
     int buf;  // which copy holds the current state
-    uint32_t* bar;
+    uint32_t* flags;  // one kLine-word line per tile: the last block epoch it published
+    uint32_t epoch0;  // block epochs completed before this launch
     int* err;
     uint32_t spin_limit;
 };

@@ -9,10 +9,12 @@
 //                 LDS and the TinyMT states of all those positions in VGPRs.  It advances the
 //                 halo generators redundantly (their streams do not depend on the tiling, so
 //                 every copy draws the same numbers), runs H steps locally, then publishes its
-//                 owned positions to a double-buffered global copy and meets the other
-//                 workgroups at a grid barrier before reloading its halo.  Per-step HBM
-//                 traffic is zero; bit-exactness holds because each position's state after a
-//                 block of H steps depends only on inputs within distance H.
+//                 owned positions to a double-buffered global copy, announces the block on its
+//                 flag line and waits only for its (up to 8) neighbour tiles' flags before
+//                 reloading its halo.  Per-step HBM traffic is zero; bit-exactness holds
+//                 because each position's state after a block of H steps depends only on
+//                 inputs within distance H.  Double buffering is race-free because a tile can
+//                 never be more than one block ahead of a neighbour.
 #include "rt_internal.hpp"
 #include "rt_models.hpp"
 
@@ -60,7 +62,7 @@ hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t st) {
 // stepwise path
 // ---------------------------------------------------------------------------------
 template <class M>
-__global__ void k_rotate(RuleParams rp, int c, typename M::T* __restrict__ cls, int N, int W,
+__global__ void k_rotate(RuleParams rp, int c, typename M::T* __restrict__ cls, int N, int W, int P,
                          uint4* __restrict__ gens, const uint4* __restrict__ params, uint32_t m1, uint32_t m2,
                          uint32_t tm) {
     const int gj = blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,18 +79,18 @@ __global__ void k_rotate(RuleParams rp, int c, typename M::T* __restrict__ cls, 
     }
     const uint32_t k = tm_draw(s.x, s.y, s.z, s.w, a, b, t);
     gens[id] = s;
-    typename M::T* arr = cls + (size_t)c * N * W;
-    const typename M::T st = arr[gi * W + gj];
+    typename M::T* arr = cls + (size_t)c * N * P;
+    const typename M::T st = arr[gi * P + gj];
     const typename M::T ns = M::flip(M::step_rule(rp, c), gi, st, k);
-    if (ns != st) arr[gi * W + gj] = ns;
+    if (ns != st) arr[gi * P + gj] = ns;
 }
 
 template <class M>
-__global__ void k_update(int c, typename M::T* __restrict__ cls, int N, int W) {
+__global__ void k_update(int c, typename M::T* __restrict__ cls, int N, int W, int P) {
     const int gj = blockIdx.x * blockDim.x + threadIdx.x;
     const int gi = blockIdx.y;
     if (gj >= W || !M::updatable(gi, gj, N, W)) return;
-    M::update(c, gi, cls, N * W, gi * W + gj, W);
+    M::update(c, gi, cls, N * P, gi * P + gj, P);
 }
 
 template <class M>
@@ -99,9 +101,9 @@ hipError_t launch_stepwise_t(const StepwiseArgs& a, hipStream_t st) {
     dim3 grid((a.W + nt - 1) / nt, a.N);
     for (int64_t s = 0; s < a.nsteps; s++) {
         const int c = a.sched_host[s];
-        hipLaunchKernelGGL(k_rotate<M>, grid, dim3(nt), 0, st, a.rp, c, cls, a.N, a.W, a.gens, a.params, a.m1, a.m2,
-                           a.tm);
-        hipLaunchKernelGGL(k_update<M>, grid, dim3(nt), 0, st, c, cls, a.N, a.W);
+        hipLaunchKernelGGL(k_rotate<M>, grid, dim3(nt), 0, st, a.rp, c, cls, a.N, a.W, a.P, a.gens, a.params, a.m1,
+                           a.m2, a.tm);
+        hipLaunchKernelGGL(k_update<M>, grid, dim3(nt), 0, st, c, cls, a.N, a.W, a.P);
     }
     return hipGetLastError();
 }
@@ -118,6 +120,16 @@ hipError_t launch_stepwise(int model, const StepwiseArgs& a, hipStream_t st) {
 // ---------------------------------------------------------------------------------
 // persistent temporal-blocked path
 // ---------------------------------------------------------------------------------
+// Exchange of tile borders between workgroups goes through the chain's global copy with
+// agent-coherent 32-bit accesses (relaxed atomic loads/stores at agent scope: coherent
+// across XCDs without writing back or invalidating whole L2s), rows being 16-byte aligned.
+__device__ __forceinline__ void xstore(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t xload(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <class M, int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
     using T = typename M::T;
@@ -125,13 +137,22 @@ __global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
     constexpr int ER = TR + 2 * H, EC = TC + 2 * H;
     constexpr int AREA = ER * EC;
     constexpr int GPT = (AREA + NT - 1) / NT;
+    constexpr int ES = (int)sizeof(T);
     static_assert(GPT <= 32, "position masks are 32 bits");
-    __shared__ T lds[NC * AREA];
+    static_assert((TC * ES) % 4 == 0 && (H * ES) % 4 == 0, "tile columns and halo must be whole dwords");
+    constexpr int EW = EC * ES / 4, TW = TC * ES / 4, HW = H * ES / 4;  // dwords per ext row, owned row, halo
+    constexpr int AREAW = ER * EW;
+    __shared__ __align__(16) T lds[NC * AREA];
+    uint32_t* lds32 = reinterpret_cast<uint32_t*>(lds);
 
     const int N = a.N, W = a.W;
+    const int PW = a.P * ES / 4;             // dwords per global row
+    const size_t CSW = (size_t)N * PW;       // dwords per class
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tiles_r = gridDim.x / a.tiles_c;
     const int tr = tile / a.tiles_c, tc = tile - (tile / a.tiles_c) * a.tiles_c;
     const int r0 = tr * TR - H, c0 = tc * TC - H;  // global coordinates of LDS (0, 0)
+    const int c0w = c0 * ES / 4;                   // ... in dwords (c0 is a whole dword)
     const int tid = threadIdx.x;
     const RuleParams& rp = a.rp;
 
@@ -167,22 +188,32 @@ __global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
         }
     }
 
-    const size_t CS = (size_t)N * W;  // class stride in the global arrays
-    int buf = a.buf;
-    // load tile + halo of every class
-    {
-        const T* src = (const T*)a.cls[buf];
-        for (int p = tid; p < AREA; p += NT) {
-            const int li = p / EC, lj = p - (p / EC) * EC;
-            const int gi = r0 + li, gj = c0 + lj;
-            const bool in = gi >= 0 && gi < N && gj >= 0 && gj < W;
+    // load tile + halo of every class (dwords outside the array read as zero; the row
+    // padding past W holds zeros)
+    auto load_ext = [&](int buf, bool skip_owned) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.cls[buf]);
+        for (int q = tid; q < AREAW; q += NT) {
+            const int li = q / EW, lw = q - (q / EW) * EW;
+            if (skip_owned && li >= H && li < H + TR && lw >= HW && lw < HW + TW) continue;
+            const int gi = r0 + li, gw = c0w + lw;
+            const bool in = gi >= 0 && gi < N && gw >= 0 && gw < PW;
 #pragma unroll
-            for (int c = 0; c < NC; c++) lds[c * AREA + p] = in ? src[c * CS + (size_t)gi * W + gj] : (T)0;
+            for (int c = 0; c < NC; c++) lds32[c * AREAW + q] = in ? xload(src + c * CSW + (size_t)gi * PW + gw) : 0u;
         }
-    }
+    };
+    load_ext(a.buf, false);
     __syncthreads();
 
-    uint32_t nbar = 0;
+    // neighbour tiles whose borders this tile reads
+    int nbr_tile = -1;
+    if (tid < 9 && tid != 4) {
+        const int dr = tid / 3 - 1, dc = tid % 3 - 1;
+        const int rr = tr + dr, cc = tc + dc;
+        if (rr >= 0 && rr < tiles_r && cc >= 0 && cc < a.tiles_c) nbr_tile = rr * a.tiles_c + cc;
+    }
+
+    int buf = a.buf;
+    uint32_t nblk = 0;
     int64_t s = 0;
     while (s < a.nsteps) {
         const int64_t blk = (a.nsteps - s < H) ? (a.nsteps - s) : H;
@@ -213,33 +244,41 @@ __global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
             }
             __syncthreads();
         }
-        // publish owned positions into the other buffer
+        // publish owned positions into the other copy
         {
-            T* dst = (T*)a.cls[buf ^ 1];
-            for (int p = tid; p < TR * TC; p += NT) {
-                const int oi = p / TC, oj = p - (p / TC) * TC;
-                const int gi = tr * TR + oi, gj = tc * TC + oj;
-                if (gi < N && gj < W) {
-                    const int lp = (oi + H) * EC + (oj + H);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(a.cls[buf ^ 1]);
+            for (int q = tid; q < TR * TW; q += NT) {
+                const int oi = q / TW, ow = q - (q / TW) * TW;
+                const int gi = tr * TR + oi, gw = tc * TW + ow;
+                if (gi < N && gw < PW) {
+                    const int lq = (oi + H) * EW + HW + ow;
 #pragma unroll
-                    for (int c = 0; c < NC; c++) dst[c * CS + (size_t)gi * W + gj] = lds[c * AREA + lp];
+                    for (int c = 0; c < NC; c++) xstore(dst + c * CSW + (size_t)gi * PW + gw, lds32[c * AREAW + lq]);
                 }
             }
         }
         buf ^= 1;
         if (s < a.nsteps) {
-            nbar++;
-            grid_barrier(a.bar, nbar * gridDim.x, a.err, a.spin_limit);
-            const T* src = (const T*)a.cls[buf];
-            for (int p = tid; p < AREA; p += NT) {
-                const int li = p / EC, lj = p - (p / EC) * EC;
-                // owned positions are already current in LDS
-                if (li >= H && li < H + TR && lj >= H && lj < H + TC) continue;
-                const int gi = r0 + li, gj = c0 + lj;
-                const bool in = gi >= 0 && gi < N && gj >= 0 && gj < W;
-#pragma unroll
-                for (int c = 0; c < NC; c++) lds[c * AREA + p] = in ? src[c * CS + (size_t)gi * W + gj] : (T)0;
+            // announce this block, wait for the neighbours' announcements, reload the halo
+            nblk++;
+            const uint32_t epoch = a.epoch0 + nblk;
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (tid == 0) xstore(a.flags + (size_t)tile * kLine, epoch);
+            if (nbr_tile >= 0) {
+                const uint32_t* f = a.flags + (size_t)nbr_tile * kLine;
+                uint32_t polls = 0;
+                while (!epoch_reached(xload(f), epoch)) {
+                    if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+                    if (++polls > a.spin_limit) {
+                        __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
             }
+            __syncthreads();
+            load_ext(buf, true);
             __syncthreads();
         }
     }
@@ -269,7 +308,7 @@ struct Cfg {
 #define RT_LOZENGE_CFGS(X) \
     X(64, 84, 8, 1024)     \
     X(64, 84, 4, 1024)     \
-    X(32, 42, 4, 256)      \
+    X(32, 44, 4, 256)      \
     X(32, 84, 8, 512)
 #define RT_RECTTRI_CFGS(X) \
     X(32, 32, 4, 256)      \
