@@ -175,6 +175,11 @@ int64_t rt_chain_state_bytes(rt_chain* ch);
 int64_t rt_chain_last_launches(rt_chain* ch);
 int32_t rt_chain_last_config(rt_chain* ch);
 
+/* Persistent tile configurations compiled for `model` (no device needed): count, and the
+ * rows, columns, halo width and threads per workgroup of configuration idx. */
+int32_t rt_tile_config_count(int model);
+int rt_tile_config(int model, int32_t idx, int32_t* rows, int32_t* cols, int32_t* halo, int32_t* threads);
+
 /* Number of persistent tile configurations compiled for this chain's model. */
 int32_t rt_chain_tile_configs(rt_chain* ch);
 

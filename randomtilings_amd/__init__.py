@@ -62,6 +62,8 @@ SIGNATURES = {
     "rt_chain_read": (C.c_int, [_P, _I32P]),
     "rt_chain_attempts_per_step": (C.c_int64, [_P]),
     "rt_chain_state_bytes": (C.c_int64, [_P]),
+    "rt_tile_config_count": (C.c_int32, [C.c_int]),
+    "rt_tile_config": (C.c_int, [C.c_int, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
     "rt_chain_last_launches": (C.c_int64, [_P]),
     "rt_chain_last_config": (C.c_int32, [_P]),
     "rt_chain_tile_configs": (C.c_int32, [_P]),
@@ -116,6 +118,16 @@ def rule_thresholds(model: int, rule: int, rparams=None) -> np.ndarray:
     if n < 0:
         _check(n)
     return out
+
+
+def tile_config_count(model: int) -> int:
+    return int(load().rt_tile_config_count(model))
+
+
+def tile_config(model: int, idx: int):
+    v = [C.c_int32() for _ in range(4)]
+    _check(load().rt_tile_config(model, idx, *[C.byref(x) for x in v]))
+    return tuple(x.value for x in v)
 
 
 class Context:

@@ -637,6 +637,19 @@ int64_t rt_chain_state_bytes(rt_chain* ch) {
     return ch ? (int64_t)ch->N * ch->W * ch->NC * ch->esize : 0;
 }
 
+int32_t rt_tile_config_count(int model) { return persist_config_count(model); }
+
+int rt_tile_config(int model, int32_t idx, int32_t* rows, int32_t* cols, int32_t* halo, int32_t* threads) {
+    if (!rows || !cols || !halo || !threads) return fail(RT_ERR_ARG, "null argument");
+    int tr, tc, h, nt;
+    if (!persist_config(model, idx, &tr, &tc, &h, &nt)) return fail(RT_ERR_ARG, "config %d out of range", idx);
+    *rows = tr;
+    *cols = tc;
+    *halo = h;
+    *threads = nt;
+    return RT_OK;
+}
+
 int64_t rt_chain_last_launches(rt_chain* ch) { return ch ? ch->last_launches : 0; }
 
 int32_t rt_chain_last_config(rt_chain* ch) { return ch ? ch->last_cfg : -1; }

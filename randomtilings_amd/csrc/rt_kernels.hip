@@ -16,6 +16,7 @@
 //                 inputs within distance H.  Double buffering is race-free because a tile can
 //                 never be more than one block ahead of a neighbour.
 #include "rt_internal.hpp"
+#include "rt_loz4.hpp"
 #include "rt_models.hpp"
 
 namespace rt {
@@ -121,14 +122,8 @@ hipError_t launch_stepwise(int model, const StepwiseArgs& a, hipStream_t st) {
 // persistent temporal-blocked path
 // ---------------------------------------------------------------------------------
 // Exchange of tile borders between workgroups goes through the chain's global copy with
-// agent-coherent 32-bit accesses (relaxed atomic loads/stores at agent scope: coherent
+// agent-coherent 32-bit accesses (xstore/xload: relaxed atomics at agent scope, coherent
 // across XCDs without writing back or invalidating whole L2s), rows being 16-byte aligned.
-__device__ __forceinline__ void xstore(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t xload(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 template <class M, int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
@@ -295,27 +290,28 @@ __global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
     }
 }
 
-// tile configurations per model: {TR, TC, H, NT}
+// tile configurations per model: {TR, TC, H, NT, packed}; packed selects the quad kernel
 struct Cfg {
-    int tr, tc, h, nt;
+    int tr, tc, h, nt, packed;
 };
 
-#define RT_DOMINO_CFGS(X) \
-    X(32, 32, 8, 256)     \
-    X(16, 32, 4, 256)     \
-    X(64, 64, 8, 512)     \
-    X(32, 64, 8, 512)
-#define RT_LOZENGE_CFGS(X) \
-    X(64, 84, 8, 1024)     \
-    X(64, 84, 4, 1024)     \
-    X(32, 44, 4, 256)      \
-    X(32, 84, 8, 512)
-#define RT_RECTTRI_CFGS(X) \
-    X(32, 32, 4, 256)      \
-    X(48, 48, 8, 512)      \
-    X(16, 32, 4, 256)
+#define RT_DOMINO_CFGS(X)    \
+    X(32, 32, 8, 256, 0)     \
+    X(16, 32, 4, 256, 0)     \
+    X(64, 64, 8, 512, 0)     \
+    X(32, 64, 8, 512, 0)
+#define RT_LOZENGE_CFGS(X)  \
+    X(64, 84, 8, 1024, 1)   \
+    X(64, 84, 4, 1024, 1)   \
+    X(32, 44, 4, 256, 1)    \
+    X(32, 84, 8, 512, 1)    \
+    X(64, 84, 8, 1024, 0)
+#define RT_RECTTRI_CFGS(X)  \
+    X(32, 32, 4, 256, 0)    \
+    X(48, 48, 8, 512, 0)    \
+    X(16, 32, 4, 256, 0)
 
-#define RT_CFG_ENTRY(a, b, c, d) Cfg{a, b, c, d},
+#define RT_CFG_ENTRY(a, b, c, d, e) Cfg{a, b, c, d, e},
 static const Cfg kDominoCfgs[] = {RT_DOMINO_CFGS(RT_CFG_ENTRY)};
 static const Cfg kLozengeCfgs[] = {RT_LOZENGE_CFGS(RT_CFG_ENTRY)};
 static const Cfg kRectTriCfgs[] = {RT_RECTTRI_CFGS(RT_CFG_ENTRY)};
@@ -347,28 +343,32 @@ bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt) {
     return true;
 }
 
-template <class M, int TR, int TC, int H, int NT>
+template <class M, int TR, int TC, int H, int NT, int PACKED>
 static const void* kernel_ptr(bool uni) {
-    return uni ? (const void*)k_persist<M, TR, TC, H, NT, true> : (const void*)k_persist<M, TR, TC, H, NT, false>;
+    if constexpr (PACKED != 0) {
+        return uni ? (const void*)k_persist_loz4<TR, TC, H, NT, true> : (const void*)k_persist_loz4<TR, TC, H, NT, false>;
+    } else {
+        return uni ? (const void*)k_persist<M, TR, TC, H, NT, true> : (const void*)k_persist<M, TR, TC, H, NT, false>;
+    }
 }
 
 static const void* persist_kernel(int model, int idx, bool uni) {
     int i = 0;
-#define RT_PICK(M_, a, b, c, d)                              \
-    if (i++ == idx) return kernel_ptr<M_, a, b, c, d>(uni);
+#define RT_PICK(M_, a, b, c, d, e)                              \
+    if (i++ == idx) return kernel_ptr<M_, a, b, c, d, e>(uni);
     switch (model) {
         case RT_DOMINO: {
-#define X(a, b, c, d) RT_PICK(DominoModel, a, b, c, d)
+#define X(a, b, c, d, e) RT_PICK(DominoModel, a, b, c, d, e)
             RT_DOMINO_CFGS(X)
 #undef X
         } break;
         case RT_LOZENGE: {
-#define X(a, b, c, d) RT_PICK(LozengeModel, a, b, c, d)
+#define X(a, b, c, d, e) RT_PICK(LozengeModel, a, b, c, d, e)
             RT_LOZENGE_CFGS(X)
 #undef X
         } break;
         case RT_RECTTRI: {
-#define X(a, b, c, d) RT_PICK(RectTriModel, a, b, c, d)
+#define X(a, b, c, d, e) RT_PICK(RectTriModel, a, b, c, d, e)
             RT_RECTTRI_CFGS(X)
 #undef X
         } break;

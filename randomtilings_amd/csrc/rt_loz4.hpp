@@ -1,0 +1,236 @@
+// rt_loz4.hpp — persistent lozenge sweep on packed quads (gfx950).
+//
+// Same scheme as the generic persistent kernel (tile + halo in LDS, generators in VGPRs,
+// neighbour-flag exchange), but each thread works on quads: four horizontally adjacent
+// positions of one colour-class row held in one 32-bit LDS word.  All four share the row
+// index, so the update's row-dependent column offsets (lozengekernel.cl:45-48) are uniform
+// over the quad and become byte funnel shifts of neighbouring words, and the update of a
+// quad is a handful of 32-bit mask/shift operations instead of four byte-wise passes.
+//
+// Flip (lozengekernel.cl:21-24 / qlozengekernel.cl:23-29 as thresholds): byte b of the quad
+// toggles between 25 and 38 (25 ^ 38 = 63) when its state and its own generator's draw
+// say so; the toggles of the quad are applied with one xor.
+#pragma once
+
+#include "rt_internal.hpp"
+#include "rt_models.hpp"
+
+namespace rt {
+
+__device__ __forceinline__ void xstore(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t xload(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// byte-aligned funnel shift: bytes [sh, sh+4) of the little-endian pair (lo, hi)
+__device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, int sh) {
+    return __builtin_amdgcn_alignbyte(hi, lo, sh);
+}
+
+template <int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
+    constexpr int NC = 3;
+    constexpr int ER = TR + 2 * H, EC = TC + 2 * H;
+    static_assert(TC % 4 == 0 && H % 4 == 0, "quads must tile the rows");
+    constexpr int EW = EC / 4, TW = TC / 4, HW = H / 4;  // words per ext row, owned row, halo
+    constexpr int AREAW = ER * EW;                       // words per class
+    constexpr int QPT = (AREAW + NT - 1) / NT;           // quads per thread
+    constexpr int GUARD = EW + 4;                        // words before and after the classes
+    __shared__ uint32_t lds[GUARD + NC * AREAW + GUARD];
+    uint32_t* L = lds + GUARD;
+
+    const int N = a.N, W = a.W;
+    const int PW = a.P / 4;
+    const size_t CSW = (size_t)N * PW;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tiles_r = gridDim.x / a.tiles_c;
+    const int tr = tile / a.tiles_c, tc = tile - (tile / a.tiles_c) * a.tiles_c;
+    const int r0 = tr * TR - H, c0 = tc * TC - H;
+    const int c0w = c0 / 4;
+    const int tid = threadIdx.x;
+
+    // per quad: 4 generators, the byte masks of positions with a generator / updatable /
+    // owned, and the row index mod 3
+    uint32_t s0[QPT][4], s1[QPT][4], s2[QPT][4], s3[QPT][4];
+    uint32_t pm1[UNI ? 1 : QPT][UNI ? 1 : 4], pm2[UNI ? 1 : QPT][UNI ? 1 : 4], pmt[UNI ? 1 : QPT][UNI ? 1 : 4];
+    uint32_t gbytes[QPT], ubytes[QPT];
+    uint32_t omask = 0;
+    int rmod[QPT];
+#pragma unroll
+    for (int g = 0; g < QPT; g++) {
+        const int w = tid + g * NT;
+        gbytes[g] = ubytes[g] = 0;
+        rmod[g] = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            s0[g][b] = s1[g][b] = s2[g][b] = s3[g][b] = 0;
+            if (!UNI) pm1[g][b] = pm2[g][b] = pmt[g][b] = 0;
+        }
+        if (w < AREAW) {
+            const int li = w / EW, lw = w - (w / EW) * EW;
+            const int gi = r0 + li;
+            rmod[g] = ((gi % 3) + 3) % 3;
+            if (li >= H && li < H + TR && lw >= HW && lw < HW + TW) omask |= 1u << g;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int lj = 4 * lw + b, gj = c0 + lj;
+                if (LozengeModel::has_gen(gi, gj, N, W)) {
+                    gbytes[g] |= 0xffu << (8 * b);
+                    const int id = LozengeModel::gen_id(gi, gj, N, W);
+                    const uint4 s = a.gen_in[id];
+                    s0[g][b] = s.x;
+                    s1[g][b] = s.y;
+                    s2[g][b] = s.z;
+                    s3[g][b] = s.w;
+                    if (!UNI) {
+                        const uint4 q = a.params[id];
+                        pm1[g][b] = q.x;
+                        pm2[g][b] = q.y;
+                        pmt[g][b] = q.z;
+                    }
+                }
+                if (li >= 1 && li <= ER - 2 && lj >= 1 && lj <= EC - 2 && LozengeModel::updatable(gi, gj, N, W))
+                    ubytes[g] |= 0xffu << (8 * b);
+            }
+        }
+    }
+
+    for (int q = tid; q < GUARD; q += NT) {
+        lds[q] = 0;
+        lds[GUARD + NC * AREAW + q] = 0;
+    }
+    auto load_ext = [&](int buf, bool skip_owned) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.cls[buf]);
+        for (int q = tid; q < AREAW; q += NT) {
+            const int li = q / EW, lw = q - (q / EW) * EW;
+            if (skip_owned && li >= H && li < H + TR && lw >= HW && lw < HW + TW) continue;
+            const int gi = r0 + li, gw = c0w + lw;
+            const bool in = gi >= 0 && gi < N && gw >= 0 && gw < PW;
+#pragma unroll
+            for (int c = 0; c < NC; c++) L[c * AREAW + q] = in ? xload(src + c * CSW + (size_t)gi * PW + gw) : 0u;
+        }
+    };
+    load_ext(a.buf, false);
+    __syncthreads();
+
+    int nbr_tile = -1;
+    if (tid < 9 && tid != 4) {
+        const int dr = tid / 3 - 1, dc = tid % 3 - 1;
+        const int rr = tr + dr, cc = tc + dc;
+        if (rr >= 0 && rr < tiles_r && cc >= 0 && cc < a.tiles_c) nbr_tile = rr * a.tiles_c + cc;
+    }
+
+    const uint32_t alo = a.rp.fr.ab_lo[0][0], aw = a.rp.fr.ab_hi[0][0] - alo;
+    const uint32_t blo = a.rp.fr.ba_lo[0][0], bw = a.rp.fr.ba_hi[0][0] - blo;
+
+    int buf = a.buf;
+    uint32_t nblk = 0;
+    int64_t s = 0;
+    while (s < a.nsteps) {
+        const int64_t blk = (a.nsteps - s < H) ? (a.nsteps - s) : H;
+        for (int64_t q = 0; q < blk; q++, s++) {
+            const int c = a.sched[s];
+            uint32_t* Ls = L + c * AREAW;
+            uint32_t* L2 = L + (c == 2 ? 0 : c + 1) * AREAW;
+            uint32_t* L3 = L + (c == 0 ? 2 : c - 1) * AREAW;
+            // -------- flip class c
+#pragma unroll
+            for (int g = 0; g < QPT; g++) {
+                const int w = tid + g * NT;
+                if (AREAW % NT == 0 || w < AREAW) {
+                    const uint32_t S = Ls[w];
+                    uint32_t tog = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const uint32_t k = tm_draw(s0[g][b], s1[g][b], s2[g][b], s3[g][b], UNI ? a.m1 : pm1[g][b],
+                                                   UNI ? a.m2 : pm2[g][b], UNI ? a.tm : pmt[g][b]);
+                        const uint32_t sb = (S >> (8 * b)) & 0xffu;
+                        const bool t = (sb == 25u && k - alo < aw) || (sb == 38u && k - blo < bw);
+                        tog |= t ? (63u << (8 * b)) : 0u;
+                    }
+                    tog &= gbytes[g];
+                    if (tog) Ls[w] = S ^ tog;
+                }
+            }
+            __syncthreads();
+            // -------- update classes c+1, c+2 from class c (lozengekernel.cl:45-52)
+            const int t2 = (2 * c + 1) % 3, t4 = (2 * c + 2) % 3, t8 = (2 * c) % 3;
+#pragma unroll
+            for (int g = 0; g < QPT; g++) {
+                const int w = tid + g * NT;
+                if (ubytes[g]) {
+                    const int r = rmod[g];
+                    const uint32_t U0 = Ls[w - EW], U1 = Ls[w - EW + 1];
+                    const uint32_t Mm = Ls[w - 1], M0 = Ls[w], Mp = Ls[w + 1];
+                    const uint32_t Dm = Ls[w + EW - 1], D0 = Ls[w + EW];
+                    const uint32_t up2 = (r == t2) ? funnel(U1, U0, 1) : U0;   // s[i-1][j+p2]
+                    const uint32_t mm4 = (r == t4) ? funnel(M0, Mm, 3) : M0;   // s[i][j-p4]
+                    const uint32_t mp8 = (r == t8) ? funnel(Mp, M0, 1) : M0;   // s[i][j+p8]
+                    const uint32_t dm16 = (r == t2) ? funnel(D0, Dm, 3) : D0;  // s[i+1][j-p16]
+                    const uint32_t n2 = ((up2 & 0x10101010u) >> 3) | ((mm4 & 0x08080808u) >> 1) |
+                                        ((D0 & 0x01010101u) << 5);
+                    const uint32_t n3 = ((U0 & 0x20202020u) >> 5) | ((mp8 & 0x04040404u) << 1) |
+                                        ((dm16 & 0x02020202u) << 3);
+                    const uint32_t m = ubytes[g];
+                    const uint32_t o2 = L2[w], o3 = L3[w];
+                    L2[w] = (o2 & ~(m & 0x26262626u)) | (n2 & m);
+                    L3[w] = (o3 & ~(m & 0x19191919u)) | (n3 & m);
+                }
+            }
+            __syncthreads();
+        }
+        // publish owned words into the other copy
+        {
+            uint32_t* dst = reinterpret_cast<uint32_t*>(a.cls[buf ^ 1]);
+            for (int q = tid; q < TR * TW; q += NT) {
+                const int oi = q / TW, ow = q - (q / TW) * TW;
+                const int gi = tr * TR + oi, gw = tc * TW + ow;
+                if (gi < N && gw < PW) {
+                    const int lq = (oi + H) * EW + HW + ow;
+#pragma unroll
+                    for (int c = 0; c < NC; c++) xstore(dst + c * CSW + (size_t)gi * PW + gw, L[c * AREAW + lq]);
+                }
+            }
+        }
+        buf ^= 1;
+        if (s < a.nsteps) {
+            nblk++;
+            const uint32_t epoch = a.epoch0 + nblk;
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (tid == 0) xstore(a.flags + (size_t)tile * kLine, epoch);
+            if (nbr_tile >= 0) {
+                const uint32_t* f = a.flags + (size_t)nbr_tile * kLine;
+                uint32_t polls = 0;
+                while (!epoch_reached(xload(f), epoch)) {
+                    if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+                    if (++polls > a.spin_limit) {
+                        __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __syncthreads();
+            load_ext(buf, true);
+            __syncthreads();
+        }
+    }
+
+#pragma unroll
+    for (int g = 0; g < QPT; g++) {
+        if (!(omask & (1u << g))) continue;
+        const int w = tid + g * NT;
+        const int li = w / EW, lw = w - (w / EW) * EW;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            if (!(gbytes[g] & (0xffu << (8 * b)))) continue;
+            const int id = LozengeModel::gen_id(r0 + li, c0 + 4 * lw + b, N, W);
+            a.gen_out[id] = make_uint4(s0[g][b], s1[g][b], s2[g][b], s3[g][b]);
+        }
+    }
+}
+
+}  // namespace rt

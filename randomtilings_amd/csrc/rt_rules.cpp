@@ -31,6 +31,16 @@ static uint32_t first_false(uint32_t lo, uint32_t hi, const std::function<bool(u
 
 static constexpr uint32_t K_END = 1u << 23;
 
+// Empty intervals are stored as lo == hi, so that the device may test lo <= k < hi as the
+// single unsigned comparison k - lo < hi - lo.
+static void normalize(FlipRule& fr) {
+    for (int c = 0; c < 3; c++)
+        for (int p = 0; p < 2; p++) {
+            if (fr.ab_hi[c][p] < fr.ab_lo[c][p]) fr.ab_hi[c][p] = fr.ab_lo[c][p];
+            if (fr.ba_hi[c][p] < fr.ba_lo[c][p]) fr.ba_hi[c][p] = fr.ba_lo[c][p];
+        }
+}
+
 static void set_all(FlipRule& fr, uint32_t alo, uint32_t ahi, uint32_t blo, uint32_t bhi) {
     for (int c = 0; c < 3; c++)
         for (int p = 0; p < 2; p++) {
@@ -49,6 +59,7 @@ bool domino_rule(int rule, const double* p12, FlipRule& fr) {
     if (rule == 0) {
         const uint32_t t = first_false(0, K_END, [](uint32_t k) { return (double)rd_of(k) < .8; });
         set_all(fr, 0, t, 0, t);
+        normalize(fr);
         return true;
     }
     if (rule == 1 || rule == 2) {
@@ -62,6 +73,7 @@ bool domino_rule(int rule, const double* p12, FlipRule& fr) {
                 fr.ba_lo[c][p] = t;
                 fr.ba_hi[c][p] = K_END;
             }
+        normalize(fr);
         return true;
     }
     return false;
@@ -76,6 +88,7 @@ bool lozenge_rule(int rule, double q, FlipRule& fr) {
         const uint32_t t4 = first_false(0, K_END, [](uint32_t k) { return (double)rd_of(k) < .4; });
         const uint32_t t8 = first_false(t4, K_END, [](uint32_t k) { return (double)rd_of(k) < .8; });
         set_all(fr, 0, t4, t4, t8);
+        normalize(fr);
         return true;
     }
     if (!(q > 0.0) || !std::isfinite(q)) return false;
@@ -90,12 +103,14 @@ bool lozenge_rule(int rule, double q, FlipRule& fr) {
             return 1 / q >= (double)r;
         });
         set_all(fr, 0, ta, half, tr);
+        normalize(fr);
         return true;
     }
     if (rule == 2) {
         const double p = q / (1.0 + q);
         const uint32_t t = first_false(0, K_END, [p](uint32_t k) { return (double)rd_of(k) < p; });
         set_all(fr, 0, t, t, K_END);
+        normalize(fr);
         return true;
     }
     return false;

@@ -39,18 +39,18 @@ def sctx():
     c.close()
 
 
-def paths(model_cfgs):
+# every compiled persistent tile configuration of each model (the C ABI enumerates them
+# without a device)
+N_CFGS = {m: rt.tile_config_count(m) for m in (rt.RT_DOMINO, rt.RT_LOZENGE, rt.RT_RECTTRI)}
+
+
+def paths(model):
     out = [(rt.RT_PATH_STEPWISE, -1)]
-    out += [(rt.RT_PATH_PERSISTENT, i) for i in range(model_cfgs)]
+    out += [(rt.RT_PATH_PERSISTENT, i) for i in range(N_CFGS[model])]
     return out
 
 
-N_DOMINO_CFGS = 4
-N_LOZENGE_CFGS = 4
-N_RECTTRI_CFGS = 3
-
-
-@pytest.mark.parametrize("path,cfg", paths(N_DOMINO_CFGS))
+@pytest.mark.parametrize("path,cfg", paths(rt.RT_DOMINO))
 @pytest.mark.parametrize("rule", [rt.RT_DOMINO_LAZY, rt.RT_DOMINO_CFTP])
 def test_domino_walk_parity(gctx, golden, rule, path, cfg):
     for n, steps in [(1, 37), (3, 200), (16, 301), (64, 517)]:
@@ -106,7 +106,7 @@ def test_domino_weighted_parity(gctx, golden):
     assert np.array_equal(g, o)
 
 
-@pytest.mark.parametrize("path,cfg", paths(N_LOZENGE_CFGS))
+@pytest.mark.parametrize("path,cfg", paths(rt.RT_LOZENGE))
 @pytest.mark.parametrize("rule,q", [(rt.RT_LOZENGE_UNIFORM, 0.99), (rt.RT_LOZENGE_QREF, 0.999),
                                     (rt.RT_LOZENGE_QHEAT, 0.7)])
 def test_lozenge_walk_parity(gctx, golden, rule, q, path, cfg):
@@ -136,7 +136,7 @@ def test_lozenge_scrambled_params(sctx, golden):
         assert np.array_equal(g, o)
 
 
-@pytest.mark.parametrize("path,cfg", paths(N_RECTTRI_CFGS))
+@pytest.mark.parametrize("path,cfg", paths(rt.RT_RECTTRI))
 def test_recttri_walk_parity(gctx, golden, path, cfg):
     for key, steps in [("minHex5", 100), ("maxHex10", 300), ("slopeHex20", 250)]:
         t0 = golden[f"recttri_{key}"]
