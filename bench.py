@@ -45,7 +45,21 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-config", type=int, default=-1)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (to scale HBM byte counters)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_each_way": n * 4, "dispatches": 3}), flush=True)
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (independent chains shard "
@@ -67,6 +81,8 @@ def main() -> int:
     ch = ctx.chain(rt.RT_LOZENGE, rt.RT_LOZENGE_QREF, t0, [args.q])
     if args.tile_config >= 0:
         ch.set_path(rt.RT_PATH_PERSISTENT, args.tile_config)
+    else:
+        ch.autotune(2048)  # on a scratch clone; the measured chain's trajectory is unchanged
     ch.seed(args.seed)
     S = args.chain_steps
     print(f"# bench: lozenge hexagon {args.hex} (N={N}), q={args.q}, {S} chain steps per bench step, "

@@ -147,6 +147,13 @@ int rt_chain_destroy(rt_chain* ch);
  * (-1 = the autotuned/default choice). */
 int rt_chain_set_path(rt_chain* ch, int path, int32_t tile_config);
 
+/* Autotune the persistent tile configuration for this chain's shape: every configuration
+ * whose tile grid fits on the device runs `probe_steps` steps on a scratch clone of the
+ * chain (the chain's own state and generators are untouched) and the fastest is selected
+ * for this chain and remembered for later chains of the same model, side and parameter
+ * mode.  Writes the chosen index to *best (may be NULL). */
+int rt_chain_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best);
+
 /* (Re)seed: initialise the TinyMT generators on the device and the host colour-schedule
  * RNG exactly as the reference's Walk does for `mode` (RT_SEED_WALK / RT_SEED_EPOCH). */
 int rt_chain_seed(rt_chain* ch, int mode, int64_t seed);

@@ -56,6 +56,7 @@ SIGNATURES = {
     "rt_chain_destroy": (C.c_int, [_P]),
     "rt_chain_set_path": (C.c_int, [_P, C.c_int, C.c_int32]),
     "rt_chain_seed": (C.c_int, [_P, C.c_int, C.c_int64]),
+    "rt_chain_autotune": (C.c_int, [_P, C.c_int64, _I32P]),
     "rt_chain_run": (C.c_int, [_P, C.c_int64]),
     "rt_chain_sync": (C.c_int, [_P]),
     "rt_chain_elapsed_ms": (C.c_int, [_P, _DP]),
@@ -245,6 +246,11 @@ class Chain:
 
     def set_path(self, path: int, tile_config: int = -1) -> None:
         _check(self._lib.rt_chain_set_path(self.h, path, tile_config))
+
+    def autotune(self, probe_steps: int = 2048) -> int:
+        best = C.c_int32(-1)
+        _check(self._lib.rt_chain_autotune(self.h, probe_steps, C.byref(best)))
+        return best.value
 
     def seed(self, seed: int, mode: int = RT_SEED_WALK) -> None:
         _check(self._lib.rt_chain_seed(self.h, mode, seed))

@@ -9,7 +9,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <mutex>
 #include <random>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -461,6 +464,10 @@ int rt_chain_seed(rt_chain* ch, int mode, int64_t seed) {
     return RT_OK;
 }
 
+// Tile configurations chosen by rt_chain_autotune, keyed by (model, side, uniform params).
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, bool>, int> g_tuned;
+
 static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c) {
     const int n = persist_config_count(ch->model);
     const bool uni = ch->ctx->uniform;
@@ -475,6 +482,11 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c) {
         return true;
     };
     if (ch->cfg >= 0) return fits(ch->cfg) ? ch->cfg : -1;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(std::make_tuple(ch->model, ch->N, uni));
+        if (it != g_tuned.end() && fits(it->second)) return it->second;
+    }
     // default: the configuration with the fewest redundant halo positions per owned one
     // among those whose tile grid fits in one wave of resident workgroups
     int best = -1;
@@ -494,6 +506,72 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c) {
     }
     if (best >= 0) fits(best);
     return best;
+}
+
+int rt_chain_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
+    if (!ch || probe_steps <= 0) return fail(RT_ERR_ARG, "rt_chain_autotune: bad arguments");
+    rt_ctx* c = ch->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->params || c->nparams < ch->G) return fail(RT_ERR_PARAMS, "autotune needs the parameter table loaded");
+    // scratch clone: same shape and rule, the chain's current state, throwaway generators
+    rt_chain* t = new rt_chain();
+    t->ctx = c;
+    t->model = ch->model;
+    t->rule = ch->rule;
+    t->N = ch->N;
+    t->W = ch->W;
+    t->P = ch->P;
+    t->NC = ch->NC;
+    t->esize = ch->esize;
+    t->G = ch->G;
+    t->rp = ch->rp;
+    const size_t bytes = (size_t)ch->N * ch->P * ch->NC * ch->esize;
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&t->cls[b], bytes);
+    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&t->gens[b], sizeof(uint4) * (size_t)t->G);
+    if (e == hipSuccess) e = hipMalloc(&t->misc, 64);
+    if (e == hipSuccess) e = hipMalloc(&t->bar, kBarBytes);
+    if (e == hipSuccess) e = hipEventCreate(&t->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&t->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->ev_sched, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMemsetAsync(t->misc, 0, 64, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(t->bar, 0, kBarBytes, c->stream);
+    for (int b = 0; b < 2 && e == hipSuccess; b++)
+        e = hipMemcpyAsync(t->cls[b], ch->cls[ch->cur], bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) {
+        rt_chain_destroy(t);
+        return fail(RT_ERR_HIP, "autotune clone: %s", hipGetErrorString(e));
+    }
+    int rc = rt_chain_seed(t, RT_SEED_WALK, 12345);
+    int chosen = -1;
+    double best_ms = 1e30;
+    const int n = persist_config_count(ch->model);
+    for (int i = 0; i < n && rc == RT_OK; i++) {
+        t->path = RT_PATH_PERSISTENT;
+        t->cfg = i;
+        int nb = 0, tcn = 0;
+        if (choose_persist_cfg(t, &nb, &tcn) != i) continue;  // does not fit
+        rc = rt_chain_run(t, probe_steps / 4 + 1);  // warm-up
+        if (rc == RT_OK) rc = rt_chain_run(t, probe_steps);
+        double ms = 0;
+        if (rc == RT_OK) rc = rt_chain_elapsed_ms(t, &ms);
+        if (rc == RT_OK) rc = rt_chain_sync(t);
+        if (rc == RT_OK && ms < best_ms) {
+            best_ms = ms;
+            chosen = i;
+        }
+    }
+    rt_chain_destroy(t);
+    if (rc != RT_OK) return rc;
+    if (chosen < 0) return fail(RT_ERR_ARG, "no persistent tile configuration fits this chain");
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[std::make_tuple(ch->model, ch->N, c->uniform)] = chosen;
+    }
+    ch->path = RT_PATH_PERSISTENT;
+    ch->cfg = chosen;
+    if (best) *best = chosen;
+    return RT_OK;
 }
 
 int rt_chain_run(rt_chain* ch, int64_t steps) {
@@ -550,7 +628,8 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
                 HIPCHK(hipFree(ch->dsched));
                 ch->dsched = nullptr;
             }
-            HIPCHK(hipMalloc(&ch->dsched, (size_t)steps));
+            // 16 bytes of slack: kernels fetch a block's colours as one aligned 8-byte word
+            HIPCHK(hipMalloc(&ch->dsched, (size_t)steps + 16));
             ch->dsched_cap = steps;
         }
         HIPCHK(hipMemcpyAsync(ch->dsched, ch->hsched.data(), (size_t)steps, hipMemcpyHostToDevice, c->stream));

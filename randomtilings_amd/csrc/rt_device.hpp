@@ -52,6 +52,38 @@ __device__ __forceinline__ uint32_t tm_draw(uint32_t& s0, uint32_t& s1, uint32_t
     return t0 >> 9;
 }
 
+// gfx950 three-input bitwise op.  The truth table is indexed by (a << 2) | (b << 1) | c.
+template <class F>
+constexpr uint32_t tt3(F f) {
+    uint32_t t = 0;
+    for (int i = 0; i < 8; i++)
+        if (f((i >> 2) & 1, (i >> 1) & 1, i & 1)) t |= 1u << i;
+    return t;
+}
+#define RT_BITOP3(a, b, c, expr) \
+    __builtin_amdgcn_bitop3_b32((a), (b), (c), tt3([](int A, int B, int C) { return (expr); }))
+
+// The same TinyMT32 step as tm_draw, returning the full tempered output t0 (k = t0 >> 9),
+// written with three-input bitwise ops: 16 VALU operations per draw.
+__device__ __forceinline__ uint32_t tm_draw_raw(uint32_t& s0, uint32_t& s1, uint32_t& s2, uint32_t& s3, uint32_t mat1,
+                                                uint32_t mat2, uint32_t tmat) {
+    const uint32_t a = s0 & 0x7fffffffu;
+    const uint32_t x1 = RT_BITOP3(a, s1, s2, A ^ B ^ C);
+    const uint32_t x = x1 ^ (x1 << 1);
+    const uint32_t y = RT_BITOP3(s3, s3 >> 1, x, A ^ B ^ C);
+    const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int)y, 0, 1);  // 0 or ~0 from bit 0
+    const uint32_t n1 = RT_BITOP3(s2, m, mat1, A ^ (B & C));
+    const uint32_t z = RT_BITOP3(y << 10, m, mat2, A ^ (B & C));
+    const uint32_t n2 = z ^ x;
+    s0 = s1;
+    s1 = n1;
+    s2 = n2;
+    s3 = y;
+    const uint32_t t1 = s0 + (n2 >> 8);
+    const uint32_t m2 = (uint32_t)__builtin_amdgcn_sbfe((int)t1, 0, 1);
+    return RT_BITOP3(t1, m2, tmat, A ^ (B & C)) ^ y;
+}
+
 // tinymt32_init (tinymt32.clh:182-203)
 __device__ __forceinline__ void tm_init(uint32_t seed, uint32_t mat1, uint32_t mat2, uint32_t tmat, uint32_t& s0,
                                         uint32_t& s1, uint32_t& s2, uint32_t& s3) {

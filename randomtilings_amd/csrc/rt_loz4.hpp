@@ -122,20 +122,47 @@ __global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
         if (rr >= 0 && rr < tiles_r && cc >= 0 && cc < a.tiles_c) nbr_tile = rr * a.tiles_c + cc;
     }
 
-    const uint32_t alo = a.rp.fr.ab_lo[0][0], aw = a.rp.fr.ab_hi[0][0] - alo;
-    const uint32_t blo = a.rp.fr.ba_lo[0][0], bw = a.rp.fr.ba_hi[0][0] - blo;
+    // transition intervals on the raw tempered output t0 = k * 512 + (low 9 bits):
+    // lo <= k < hi  <=>  lo*512 <= t0 <= hi*512 - 1; an empty interval disables its state
+    const uint32_t alo = a.rp.fr.ab_lo[0][0], ahi = a.rp.fr.ab_hi[0][0];
+    const uint32_t blo = a.rp.fr.ba_lo[0][0], bhi = a.rp.fr.ba_hi[0][0];
+    const uint32_t aL = alo << 9, aH = (ahi << 9) - 1u, aS = (ahi > alo) ? 25u : 256u;
+    const uint32_t bL = blo << 9, bH = (bhi << 9) - 1u, bS = (bhi > blo) ? 38u : 256u;
+    // per quad: byte masks of the bits each update rewrites
+    uint32_t um2[QPT], um3[QPT];
+#pragma unroll
+    for (int g = 0; g < QPT; g++) {
+        um2[g] = ubytes[g] & 0x26262626u;
+        um3[g] = ubytes[g] & 0x19191919u;
+    }
+
+    // The colours of a block (H <= 8 steps) are fetched as one 8-byte word before the
+    // previous block's exchange, so no step waits on a global load.  The schedule buffer is
+    // padded so the last (partial) block may read past the final step.
+    static_assert(H <= 8, "a block's colours are fetched as one 8-byte word");
+    auto fetch_block = [&](int64_t s_at) -> uint64_t {
+        const uint2 v = *reinterpret_cast<const uint2*>(a.sched + s_at);
+        return ((uint64_t)v.y << 32) | v.x;
+    };
 
     int buf = a.buf;
     uint32_t nblk = 0;
     int64_t s = 0;
+    uint64_t colours = fetch_block(0);
     while (s < a.nsteps) {
         const int64_t blk = (a.nsteps - s < H) ? (a.nsteps - s) : H;
+        int c_prev = -1;
         for (int64_t q = 0; q < blk; q++, s++) {
-            const int c = a.sched[s];
+            const int c = (int)((colours >> (8 * q)) & 0xffu);
             uint32_t* Ls = L + c * AREAW;
             uint32_t* L2 = L + (c == 2 ? 0 : c + 1) * AREAW;
             uint32_t* L3 = L + (c == 0 ? 2 : c - 1) * AREAW;
-            // -------- flip class c
+            // A thread flips only its own quads, whose other classes it updated itself in
+            // the previous step; only when the same class is rotated twice in a row could a
+            // flip overwrite a neighbour that another thread's update is still reading.
+            if (c == c_prev) __syncthreads();
+            c_prev = c;
+            // -------- flip class c on this thread's quads
 #pragma unroll
             for (int g = 0; g < QPT; g++) {
                 const int w = tid + g * NT;
@@ -144,10 +171,10 @@ __global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
                     uint32_t tog = 0;
 #pragma unroll
                     for (int b = 0; b < 4; b++) {
-                        const uint32_t k = tm_draw(s0[g][b], s1[g][b], s2[g][b], s3[g][b], UNI ? a.m1 : pm1[g][b],
-                                                   UNI ? a.m2 : pm2[g][b], UNI ? a.tm : pmt[g][b]);
+                        const uint32_t t0 = tm_draw_raw(s0[g][b], s1[g][b], s2[g][b], s3[g][b], UNI ? a.m1 : pm1[g][b],
+                                                        UNI ? a.m2 : pm2[g][b], UNI ? a.tm : pmt[g][b]);
                         const uint32_t sb = (S >> (8 * b)) & 0xffu;
-                        const bool t = (sb == 25u && k - alo < aw) || (sb == 38u && k - blo < bw);
+                        const bool t = ((sb == aS) & (t0 >= aL) & (t0 <= aH)) | ((sb == bS) & (t0 >= bL) & (t0 <= bH));
                         tog |= t ? (63u << (8 * b)) : 0u;
                     }
                     tog &= gbytes[g];
@@ -169,18 +196,18 @@ __global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
                     const uint32_t mm4 = (r == t4) ? funnel(M0, Mm, 3) : M0;   // s[i][j-p4]
                     const uint32_t mp8 = (r == t8) ? funnel(Mp, M0, 1) : M0;   // s[i][j+p8]
                     const uint32_t dm16 = (r == t2) ? funnel(D0, Dm, 3) : D0;  // s[i+1][j-p16]
-                    const uint32_t n2 = ((up2 & 0x10101010u) >> 3) | ((mm4 & 0x08080808u) >> 1) |
-                                        ((D0 & 0x01010101u) << 5);
-                    const uint32_t n3 = ((U0 & 0x20202020u) >> 5) | ((mp8 & 0x04040404u) << 1) |
-                                        ((dm16 & 0x02020202u) << 3);
-                    const uint32_t m = ubytes[g];
-                    const uint32_t o2 = L2[w], o3 = L3[w];
-                    L2[w] = (o2 & ~(m & 0x26262626u)) | (n2 & m);
-                    L3[w] = (o3 & ~(m & 0x19191919u)) | (n3 & m);
+                    const uint32_t n2 = RT_BITOP3((up2 >> 3) & 0x02020202u, (mm4 >> 1) & 0x04040404u,
+                                                  (D0 << 5) & 0x20202020u, A | B | C);
+                    const uint32_t n3 = RT_BITOP3((U0 >> 5) & 0x01010101u, (mp8 << 1) & 0x08080808u,
+                                                  (dm16 << 3) & 0x10101010u, A | B | C);
+                    L2[w] = (L2[w] & ~um2[g]) | (n2 & um2[g]);
+                    L3[w] = (L3[w] & ~um3[g]) | (n3 & um3[g]);
                 }
             }
-            __syncthreads();
         }
+        // the next block's colours are in flight during the exchange
+        const uint64_t next_colours = (s < a.nsteps) ? fetch_block(s) : 0;
+        __syncthreads();  // every update of the block is in LDS before it is published
         // publish owned words into the other copy
         {
             uint32_t* dst = reinterpret_cast<uint32_t*>(a.cls[buf ^ 1]);
@@ -217,6 +244,7 @@ __global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
             load_ext(buf, true);
             __syncthreads();
         }
+        colours = next_colours;
     }
 
 #pragma unroll

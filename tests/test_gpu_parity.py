@@ -185,3 +185,18 @@ def test_short_param_table_is_an_error(golden):
         c.domino_walk(golden["domino_aztec5_max"], 10, 1)
     assert e.value.code == rt.RT_ERR_PARAMS
     c.close()
+
+
+def test_autotune_leaves_chain_untouched(gctx, golden):
+    """Autotuning probes a scratch clone: the tuned chain's trajectory equals the oracle's."""
+    t0 = golden["lozenge_hex50_0_min"]
+    ch = gctx.chain(rt.RT_LOZENGE, rt.RT_LOZENGE_QREF, t0, [0.999])
+    ch.seed(9)
+    ch.run(50)
+    best = ch.autotune(64)
+    assert 0 <= best < N_CFGS[rt.RT_LOZENGE]
+    ch.run(70)
+    g = ch.read()
+    ch.close()
+    o = O.lozenge_walk(t0, 120, 9, default_table(BIG), rule=1, q=0.999)
+    assert np.array_equal(g, o)
