@@ -135,6 +135,14 @@ int rt_lozenge_walk(rt_ctx* ctx, int rule, double q, int32_t* tiling, int32_t N,
  * all 1.0), or the 16-entry per-digit weight table. */
 int rt_recttri_walk(rt_ctx* ctx, const float* weights16, int32_t* tiling, int32_t N, int64_t steps, int64_t seed);
 
+/* TriangleDimerTiler::Walk(tiling&, int steps, long seed) — TriangleDimerTiler.cpp:86-483.
+ * The N x N lozenge-state tiling (N even, states 0..15); triangle and butterfly states are
+ * derived on the device as TilingToTriangleFlips / TilingToButterflyFlips do.  Chains of
+ * model RT_TRIDIMER: RT_PATH_PERSISTENT = every step in one single-workgroup launch (move
+ * lattice N/2 <= 64), RT_PATH_STEPWISE = one tiled launch per step (tile_config selects the
+ * tile), RT_PATH_AUTO = the former when it fits. */
+int rt_tridimer_walk(rt_ctx* ctx, int32_t* tiling, int32_t N, int64_t steps, int64_t seed);
+
 /* ---- device-resident chains (benchmarks, CFTP, batching) --------------------------- */
 
 /* Upload a tiling and split it into colour classes on the device.  rparams as in the

@@ -50,6 +50,9 @@ def lib() -> C.CDLL:
         L.or_lozenge_time.restype = C.c_double
         L.or_recttri_walk.argtypes = [_FP, _I32P, C.c_int, C.c_long, C.c_long, _U32P, C.c_int64]
         L.or_recttri_walk.restype = C.c_int
+        L.or_tridimer_walk.argtypes = [_I32P, C.c_int, C.c_long, C.c_long, _U32P, C.c_int64, C.POINTER(C.c_uint8)]
+        L.or_tridimer_walk.restype = C.c_int
+        L.or_tridimer_flips.argtypes = [_I32P, C.c_int, _I32P, _I32P]
         _or = L
     return _or
 
@@ -183,3 +186,27 @@ def recttri_walk(t, steps: int, seed: int, params, weights16=None) -> np.ndarray
     if rc:
         raise ValueError(f"or_recttri_walk rc={rc}")
     return t
+
+
+def tridimer_walk(t, steps: int, seed: int, params, return_moves: bool = False):
+    """TriangleDimerTiler::Walk restated; optionally also the per-step move codes
+    (type | r1<<2 | r2<<4 | p1<<5 | p2<<7)."""
+    t = np.ascontiguousarray(t, dtype=np.int32).copy()
+    N = int(round(np.sqrt(t.size)))
+    p, pp, n = _params(params)
+    mv = np.zeros(max(int(steps), 1), np.uint8)
+    rc = lib().or_tridimer_walk(t.ctypes.data_as(_I32P), N, steps, seed, pp, n,
+                                mv.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc:
+        raise ValueError(f"or_tridimer_walk rc={rc}")
+    return (t, mv[:steps]) if return_moves else t
+
+
+def tridimer_flips(t):
+    """(TilingToTriangleFlips, TilingToButterflyFlips) of a tiling, restated."""
+    t = np.ascontiguousarray(t, dtype=np.int32)
+    N = int(round(np.sqrt(t.size)))
+    tri = np.zeros((N // 2) * N, np.int32)
+    bf = np.zeros(N * N, np.int32)
+    lib().or_tridimer_flips(t.ctypes.data_as(_I32P), N, tri.ctypes.data_as(_I32P), bf.ctypes.data_as(_I32P))
+    return tri, bf

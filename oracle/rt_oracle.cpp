@@ -349,6 +349,434 @@ void recttri_update2(const int32_t* t1, int32_t* t2, int N, int t) {  // recttri
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Triangle dimers (src/TriangleDimer/TriangleDimerTiler.cpp:86-483, triangledimerkernel.cl)
+//
+// Eight M x M int arrays (M = N/2): lozenge-move states eH, eL, eR; triangle-move states
+// tU, tD; butterfly-move states bH, bL, bR.  Each function below is one kernel of
+// triangledimerkernel.cl run as a sequential loop over its NDRange(M, M); every kernel
+// writes only cells it does not read in the same launch, so the loop order is immaterial.
+// ---------------------------------------------------------------------------------
+struct TdArrays {
+    int M = 0;
+    std::vector<int32_t> eH, eL, eR, tU, tD, bH, bL, bR;
+};
+
+inline bool td_inner(int M, int i, int j) { return i > 0 && j > 0 && i < M - 1 && j < M - 1; }
+inline int td_sel(bool c, int v) { return c ? v : 0; }
+
+// RotateLozenges, triangledimerkernel.cl:4-22 (every work-item draws)
+void td_rotate_lozenges(std::vector<TM>& g, int32_t* a, int M, int t, int c) {
+    for (int i = 0; i < M; i++)
+        for (int j = 0; j < M; j++) {
+            const float rd = tm_single01(g[(size_t)i * M + j]);
+            if (rd < .5 && ((1 - (t & 1)) * i + ((1 + t) / 2) * j) % 2 == c) {
+                int32_t& v = a[(size_t)i * M + j];
+                if (v == 5) v = 10;
+                else if (v == 10) v = 5;
+            }
+        }
+}
+
+// UpdateLozengesFlipped, :24-34
+void td_update_lozenges_flipped(int32_t* a, int M, int t, int c) {
+    for (int i = 0; i < M; i++)
+        for (int j = 0; j < M; j++) {
+            if (!td_inner(M, i, j) || ((1 - (t & 1)) * i + ((1 + t) / 2) * j) % 2 == c) continue;
+            const int k = (2 - t) / 2, s = t & 1;
+            a[i * M + j] = (a[(i - 1) * M + j + s] & 4) / 4 + (a[(i - k) * M + j + 1] & 8) / 4 +
+                           (a[(i + 1) * M + j - s] & 1) * 4 + (a[(i + k) * M + j - 1] & 2) * 4;
+        }
+}
+
+// UpdateLozenges0/1/2, :36-81: after flipping orientation `t`, rewrite the bits of the other
+// two orientations that the flipped ones determine
+void td_update_lozenges(int32_t* a, int32_t* b, int32_t* c, int M, int t) {
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            const int x = i * M + j;
+            if (t == 0) {
+                b[x] &= ~10;
+                b[x] |= (a[(i + 1) * M + j] & 1) * 2 + (a[(i + 1) * M + j - 1] & 1) * 8;
+                c[x] &= ~10;
+                c[x] |= (a[(i + 1) * M + j] & 2) + (a[x] & 8);
+            } else if (t == 1) {
+                b[x] &= ~5;
+                b[x] |= (a[x] & 1) + (a[(i + 1) * M + j] & 1) * 4;
+                c[x] &= ~5;
+                c[x] |= (a[(i - 1) * M + j] & 2) / 2 + (a[x] & 2) * 2;
+            } else {
+                b[x] &= ~10;
+                b[x] |= (a[(i - 1) * M + j] & 2) + (a[x - 1] & 2) * 4;
+                c[x] &= ~5;
+                c[x] |= (a[x] & 1) + (a[x - 1] & 4);
+            }
+        }
+}
+
+// UpdateTriangleUFromLozenges, :100-121
+void td_triangle_u_from_lozenges(const int32_t* H, const int32_t* L, int32_t* U, int M) {
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            const int32_t l0 = L[i * M + j], l1 = L[i * M + j + 1], h0 = H[i * M + j], h1 = H[(i + 1) * M + j];
+            const int L1 = td_sel((l0 & 1) == 1, 1), L2 = td_sel((h0 & 8) == 8, 2), L3 = td_sel((h1 & 1) == 1, 3);
+            const int R1 = td_sel((l1 & 1) == 1, 1), R2 = td_sel((h1 & 2) == 2, 2), R3 = td_sel((l1 & 2) == 2, 3);
+            const int V1 = td_sel((l1 & 4) == 4, 1), V2 = td_sel((h1 & 8) == 8, 2), V3 = td_sel((h1 & 4) == 4, 3);
+            U[i * M + j] = L1 + L2 + L3 + 4 * (R1 + R2 + R3) + 16 * (V1 + V2 + V3);
+        }
+}
+
+// UpdateButterflys{H,L,R}FromLozenge, :123-190
+void td_butterflies_from_lozenges(int which, int32_t* B, const int32_t* LH, const int32_t* LL, const int32_t* LR,
+                                  int M) {
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            const int x = i * M + j;
+            int c, t1, t4, t16, t64;
+            if (which == 0) {
+                const int lh = LH[x], lr1 = LR[x - M], ll1 = LL[x - M + 1], lr2 = LR[x], ll2 = LL[x];
+                c = (ll1 & 4) / 4;
+                t1 = (lr1 & 8) / 8 + 2 * (lr1 & 1) + 3 * (lh & 1);
+                t4 = (ll1 & 1) + (ll1 & 2) + ((lh & 2) / 2) * 3;
+                t16 = (lr2 & 2) / 2 + (lr2 & 4) / 2 + ((lh & 4) / 4) * 3;
+                t64 = (ll2 & 4) / 4 + (ll2 & 8) / 4 + ((lh & 8) / 8) * 3;
+            } else if (which == 1) {
+                const int ll = LL[x], lh1 = LH[x], lr1 = LR[x], lh2 = LH[x + M - 1], lr2 = LR[x - 1];
+                c = (lr1 & 8) / 8;
+                t1 = (lh1 & 1) + (lh1 & 2) + 3 * (ll & 1);
+                t4 = (lr1 & 2) / 2 + (lr1 & 4) / 2 + ((ll & 2) / 2) * 3;
+                t16 = (lh2 & 4) / 4 + (lh2 & 8) / 4 + ((ll & 4) / 4) * 3;
+                t64 = (lr2 & 8) / 8 + 2 * (lr2 & 1) + ((ll & 8) / 8) * 3;
+            } else {
+                const int lr = LR[x], lh1 = LH[x], ll1 = LL[x + 1], lh2 = LH[x + M], ll2 = LL[x];
+                c = (ll1 & 8) / 8;
+                t1 = (lh1 & 1) + (lh1 & 2) + 3 * (lr & 1);
+                t4 = (ll1 & 1) + (ll1 & 2) + ((lr & 2) / 2) * 3;
+                t16 = (lh2 & 4) / 4 + (lh2 & 8) / 4 + ((lr & 4) / 4) * 3;
+                t64 = (ll2 & 4) / 4 + (ll2 & 8) / 4 + ((lr & 8) / 8) * 3;
+            }
+            B[x] = t1 + 4 * t4 + 16 * t16 + 64 * t64 + 256 * c;
+        }
+}
+
+// RotateTriangles, :192-212
+void td_rotate_triangles(std::vector<TM>& g, int32_t* a, int M, int c) {
+    for (int i = 0; i < M; i++)
+        for (int j = 0; j < M; j++) {
+            const float rd = tm_single01(g[(size_t)i * M + j]);
+            if (rd < 0.5 && mod3(j - i) == c) {
+                int32_t& v = a[(size_t)i * M + j];
+                if (v == 45) v = 54;
+                else if (v == 54) v = 45;
+            }
+        }
+}
+
+// UpdateTrianglesFlipped0 (:214-234, up) and UpdateTrianglesFlipped1 (:236-256, down)
+void td_update_triangles_flipped(int32_t* a, int M, int c, bool down) {
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            const int cl = mod3(j - i);
+            if (cl == c) continue;
+            const int p = (cl == (c + 1) % 3) ? 1 : 0;
+            const int t1 = (4 - 3 * p) * 12, m1 = (4 - 3 * p) * 4;
+            const int t4 = (15 * p + 1) * 3, m4 = 15 * p + 1;
+            const int t16 = (4 - 3 * p) * 3, m16 = 4 - 3 * p;
+            int x1, x4, x16;
+            if (!down) {
+                x1 = a[(i - (1 - p)) * M + j - p];
+                x4 = a[(i - p) * M + j + 1];
+                x16 = a[(i + 1) * M + j - 1 + p];
+            } else {
+                x1 = a[(i + 1 - p) * M + j - 1];
+                x4 = a[(i + p) * M + j + 1 - p];
+                x16 = a[(i - 1) * M + j + p];
+            }
+            a[i * M + j] = (x1 & t1) / m1 + ((x4 & t4) / m4) * 4 + ((x16 & t16) / m16) * 16;
+        }
+}
+
+// UpdateTriangles, :258-281: t = 0 writes down triangles from up ones, t = 1 the reverse
+void td_update_triangles(const int32_t* s, int32_t* d, int M, int t) {
+    const int p = (t == 0) ? -1 : 1;
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            const int A = s[i * M + j - 1 + t], B = s[i * M + j + t], V = s[(i + p) * M + j];
+            auto dig = [](int v) { return td_sel((v & 48) == 16, 1) + td_sel((v & 3) == 3, 2) + td_sel((v & 12) == 8, 3); };
+            d[i * M + j] = dig(A) + 4 * dig(B) + 16 * dig(V);
+        }
+}
+
+// UpdateLozenge{H,L,R}FromTriangles, :283-328
+void td_lozenges_from_triangles(const int32_t* U, const int32_t* D, int32_t* H, int32_t* L, int32_t* R, int M) {
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            const int x = i * M + j;
+            {
+                const int d = D[x], u = U[x - M];
+                H[x] = td_sel((d & 48) == 32, 1) + 2 * td_sel((d & 48) == 48, 1) + 4 * td_sel((u & 48) == 48, 1) +
+                       8 * td_sel((u & 48) == 32, 1);
+            }
+            {
+                const int d = D[x], u = U[x - 1];
+                L[x] = td_sel((u & 12) == 4, 1) + 2 * td_sel((u & 12) == 12, 1) + 4 * td_sel((d & 3) == 1, 1) +
+                       8 * td_sel((d & 3) == 2, 1);
+            }
+            {
+                const int d = D[x], u = U[x];
+                R[x] = td_sel((u & 3) == 1, 1) + 2 * td_sel((d & 12) == 12, 1) + 4 * td_sel((d & 12) == 4, 1) +
+                       8 * td_sel((u & 3) == 2, 1);
+            }
+        }
+}
+
+// RotateButterflys, :330-352
+void td_rotate_butterflies(std::vector<TM>& g, int32_t* a, int M, int t, int p1, int p2) {
+    const int b1 = 1 - (t & 1), b2 = (1 + t) / 2, b3 = t & 1;
+    for (int i = 0; i < M; i++)
+        for (int j = 0; j < M; j++) {
+            const float rd = tm_single01(g[(size_t)i * M + j]);
+            if (rd < 0.5 && (b1 * i + b2 * j) % 3 == p1) {
+                int32_t& v = a[(size_t)i * M + j];
+                const bool par = ((b1 * j + b3 * i) & 1) == p2;
+                if (v == 170) v = par ? 85 : 170;
+                else if (v == 85) v = par ? 170 : 85;
+            }
+        }
+}
+
+inline int td_pack(int t1, int t4, int t16, int t64, int c) { return t1 + 4 * t4 + 16 * t16 + 64 * t64 + 256 * c; }
+
+// UpdateButterflysFlipped{H,L,R}1, :355-375, :454-472, :552-572: the flipped slice's other
+// parity, from its two neighbours along the slice
+void td_bfly_flipped1(int32_t* a, int M, int t, int p1, int p2) {
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            const int x = i * M + j;
+            const int c = (a[x] & 256) / 256;
+            int t1, t4, t16, t64;
+            if (t == 0 || t == 1) {
+                if (t == 0 && !(i % 3 == p1 && (j & 1) != p2)) continue;
+                if (t == 1 && !(j % 3 == p1 && (i & 1) != p2)) continue;
+                const int B1 = (t == 0) ? a[x - 1] : a[x - M];
+                const int B2 = (t == 0) ? a[x + 1] : a[x + M];
+                t1 = ((B1 & 12) / 4 == 0) ? 0 : ((B1 & 12) / 4) % 3 + 1;
+                t4 = ((B2 & 3) == 0) ? 0 : ((B2 & 3) + 1) % 3 + 1;
+                t16 = ((B2 & 192) / 64 == 0) ? 0 : ((B2 & 192) / 64) % 3 + 1;
+                t64 = ((B1 & 48) / 16 == 0) ? 0 : ((B1 & 48) / 16 + 1) % 3 + 1;
+            } else {
+                if (!((i + j) % 3 == p1 && (j & 1) != p2)) continue;
+                const int B1 = a[x + M - 1], B2 = a[x - M + 1];
+                t1 = ((B2 & 192) / 64 == 0) ? 0 : ((B2 & 192) / 64 + 1) % 3 + 1;
+                t4 = ((B2 & 48) / 16 == 0) ? 0 : ((B2 & 48) / 16) % 3 + 1;
+                t16 = ((B1 & 12) / 4 == 0) ? 0 : ((B1 & 12) / 4 + 1) % 3 + 1;
+                t64 = ((B1 & 3) == 0) ? 0 : (B1 & 3) % 3 + 1;
+            }
+            a[x] = td_pack(t1, t4, t16, t64, c);
+        }
+}
+
+// slice index of butterfly cell (i, j) for orientation t: rows (H), columns (L), i+j (R)
+inline int td_slice(int t, int i, int j) { return t == 0 ? i % 3 : t == 1 ? j % 3 : (i + j) % 3; }
+
+// UpdateButterflysFlipped{H,L}21 (:377-398, :474-495) and R21 (:574-595)
+void td_bfly_flipped21(int32_t* a, int M, int t, int p1) {
+    const int want = (t == 0) ? (p1 + 1) % 3 : (p1 + 2) % 3;
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            if (td_slice(t, i, j) != want) continue;
+            const int x = i * M + j, Bf = a[x];
+            if (t == 2) {
+                const int B2 = a[x + 1], B3 = a[x + M];
+                const int c = td_sel((B3 & 3) == 1, 1);
+                const int t4 = ((Bf & 12) / 4 == 2) ? (Bf & 12) / 4 : (B2 & 3) / 3 + ((B2 & 192) / 192) * 3;
+                const int t16 = ((Bf & 48) / 16 == 1) ? (Bf & 48) / 16 : ((B3 & 3) / 3) * 3 + ((B3 & 192) / 192) * 2;
+                a[x] = td_pack(Bf & 3, t4, t16, (Bf & 192) / 64, c);
+            } else {
+                const int B1 = (t == 0) ? a[x - M] : a[x - M + 1];
+                const int B2 = (t == 0) ? a[x - M + 1] : a[x + 1];
+                const int c = td_sel((B1 & 48) / 16 == 2, 1);
+                const int t1 = ((Bf & 3) == 2) ? (Bf & 3) : (B1 & 192) / 192 + ((B1 & 48) / 48) * 3;
+                const int t4 = ((Bf & 12) / 4 == 1) ? (Bf & 12) / 4 : ((B2 & 192) / 192) * 3 + ((B2 & 48) / 48) * 2;
+                a[x] = td_pack(t1, t4, (Bf & 48) / 16, (Bf & 192) / 64, c);
+            }
+        }
+}
+
+// UpdateButterflysFlipped{H,L,R}22, :400-428, :497-527, :597-627
+void td_bfly_flipped22(int32_t* a, int M, int t, int p1) {
+    const int want = (t == 0) ? (p1 + 2) % 3 : (p1 + 1) % 3;
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            if (td_slice(t, i, j) != want) continue;
+            const int x = i * M + j, Bf = a[x];
+            if (t == 2) {
+                const int s4 = td_sel((Bf & 12) / 4 != 2, 1), s16 = td_sel((Bf & 48) / 16 != 1, 1);
+                const int B1 = a[x - M], B2 = a[x + 1], B3 = a[x + M], B4 = a[x - 1];
+                const int c = td_sel((B1 & 48) / 16 == 1, 1);
+                const int t1 = ((Bf & 3) == 1) ? (Bf & 3) : ((B1 & 48) / 48) * 3 + ((B1 & 12) / 12) * 2;
+                const int t4 = ((B2 & 256) / 256 == 1) ? 2 : s4 * (Bf & 12) / 4;
+                const int t16 = ((B3 & 256) / 256 == 1) ? 1 : s16 * (Bf & 48) / 16;
+                const int t64 = ((Bf & 192) / 64 == 2) ? (Bf & 192) / 64 : (B4 & 48) / 48 + ((B4 & 12) / 12) * 3;
+                a[x] = td_pack(t1, t4, t16, t64, c);
+            } else {
+                const int s1 = td_sel((Bf & 3) != 2, 1), s4 = td_sel((Bf & 12) / 4 != 1, 1);
+                const int B1 = (t == 0) ? a[x - M] : a[x - M + 1];
+                const int B2 = (t == 0) ? a[x - M + 1] : a[x + 1];
+                const int B3 = (t == 0) ? a[x + M] : a[x + M - 1];
+                const int B4 = (t == 0) ? a[x + M - 1] : a[x - 1];
+                const int c = td_sel((B3 & 3) == 2, 1);
+                const int t1 = ((B1 & 256) == 256) ? 2 : s1 * (Bf & 3);
+                const int t4 = ((B2 & 256) == 256) ? 1 : s4 * ((Bf & 12) / 4);
+                const int t16 = ((Bf & 48) / 16 == 2) ? (Bf & 48) / 16 : ((B3 & 3) / 3) * 3 + (B3 & 12) / 12;
+                const int t64 = ((Bf & 192) / 64 == 1) ? (Bf & 192) / 64 : ((B4 & 3) / 3) * 2 + ((B4 & 12) / 12) * 3;
+                a[x] = td_pack(t1, t4, t16, t64, c);
+            }
+        }
+}
+
+// UpdateButterflysFlipped{H,L,R}23, :430-452, :529-550, :629-651
+void td_bfly_flipped23(int32_t* a, int M, int t, int p1) {
+    const int want = (t == 0) ? (p1 + 1) % 3 : (p1 + 2) % 3;
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            if (td_slice(t, i, j) != want) continue;
+            const int x = i * M + j, Bf = a[x];
+            const int c = (Bf & 256) / 256;
+            if (t == 2) {
+                const int s1 = td_sel((Bf & 3) != 1, 1), s64 = td_sel((Bf & 192) / 64 != 2, 1);
+                const int B1 = a[x - M], B4 = a[x - 1];
+                const int t1 = ((B1 & 256) / 256 == 1) ? 1 : s1 * (Bf & 3);
+                const int t64 = ((B4 & 256) / 256 == 1) ? 2 : s64 * (Bf & 192) / 64;
+                a[x] = td_pack(t1, (Bf & 12) / 4, (Bf & 48) / 16, t64, c);
+            } else {
+                const int s16 = td_sel((Bf & 48) / 16 != 2, 1), s64 = td_sel((Bf & 192) / 64 != 1, 1);
+                const int B3 = (t == 0) ? a[x + M] : a[x + M - 1];
+                const int B4 = (t == 0) ? a[x + M - 1] : a[x - 1];
+                const int t16 = ((B3 & 256) == 256) ? 2 : s16 * ((Bf & 48) / 16);
+                const int t64 = ((B4 & 256) == 256) ? 1 : s64 * ((Bf & 192) / 64);
+                a[x] = td_pack(Bf & 3, (Bf & 12) / 4, t16, t64, c);
+            }
+        }
+}
+
+// UpdateLozengeFromButterflys{H,L,R}, :654-738
+void td_lozenges_from_butterflies(int t, const int32_t* B, int32_t* LH, int32_t* LL, int32_t* LR, int M) {
+    auto f = [](bool b) { return b ? 1 : 0; };
+    for (int i = 1; i < M - 1; i++)
+        for (int j = 1; j < M - 1; j++) {
+            const int x = i * M + j, F = B[x];
+            const int full = f((F & 3) == 3) + 2 * f((F & 12) == 12) + 4 * f((F & 48) == 48) + 8 * f((F & 192) == 192);
+            if (t == 0) {
+                LH[x] = full;
+                LL[x] = f((F & 256) == 256) + 2 * f((F & 48) == 48) + 4 * f((F & 192) == 64) + 8 * f((F & 192) == 128);
+                LR[x] = f((F & 256) == 256) + 2 * f((F & 48) == 16) + 4 * f((F & 48) == 32) + 8 * f((F & 192) == 192);
+            } else if (t == 1) {
+                LL[x] = full;
+                LR[x] = f((F & 3) == 3) + 2 * f((F & 12) == 4) + 4 * f((F & 12) == 8) + 8 * f((F & 256) == 256);
+                LH[x] = f((F & 3) == 1) + 2 * f((F & 3) == 2) + 4 * f((F & 12) == 12) + 8 * f((F & 256) == 256);
+            } else {
+                LR[x] = full;
+                LH[x] = f((F & 3) == 1) + 2 * f((F & 3) == 2) + 4 * f((F & 256) == 256) + 8 * f((F & 192) == 192);
+                LL[x] = f((F & 3) == 3) + 2 * f((F & 256) == 256) + 4 * f((F & 192) == 64) + 8 * f((F & 192) == 128);
+            }
+        }
+}
+
+// TilingToTriangleFlips, TriangleDimerTiler.cpp:600-631 (tiling side N, output N/2 x N)
+std::vector<int32_t> td_triangle_flips(const int32_t* t, int N) {
+    std::vector<int32_t> o((size_t)(N / 2) * N, 0);
+    for (int i = 1; i < N / 2 - 1; i++)
+        for (int j = 1; j < N - 1; j++) {
+            const int a = t[(2 * i + 1) * N + j], b = t[(2 * i + 1) * N + j + 1];
+            if (j % 2 == 0) {
+                const int c = t[2 * i * N + j + 1];
+                o[i * N + j] = ((a & 4) / 4 + (a & 8) / 4 + ((b & 8) / 8) * 3) + 4 * ((b & 4) / 4 + (a & 2) + ((b & 2) / 2) * 3) +
+                               16 * ((c & 1) * 2 + (a & 1) + ((c & 2) / 2) * 3);
+            } else {
+                const int d = t[(2 * i + 2) * N + j];
+                o[i * N + j] = ((a & 1) + (a & 8) / 4 + ((b & 8) / 8) * 3) + 4 * ((b & 1) + (a & 2) + ((b & 2) / 2) * 3) +
+                               16 * ((d & 8) / 4 + ((d & 4) / 4) * 3 + (a & 4) / 4);
+            }
+        }
+    return o;
+}
+
+// TilingToButterflyFlips, TriangleDimerTiler.cpp:633-680
+std::vector<int32_t> td_butterfly_flips(const int32_t* t, int N) {
+    std::vector<int32_t> o((size_t)N * N, 0);
+    for (int i = 2; i < N - 2; i++)
+        for (int j = 2; j < N - 2; j++) {
+            const int x = i * N + j, s = t[x];
+            int f1, f4, f16, f64, c;
+            if (i % 2 == 0) {
+                if (j % 2 == 0) continue;
+                const int u = t[x - N], ur = t[x - N + 1], d = t[x + N], dl = t[x + N - 1];
+                f1 = (u & 8) / 8 + (u & 1) * 2 + (s & 1) * 3;
+                f4 = (ur & 1) + (ur & 2) + ((s & 2) / 2) * 3;
+                f16 = (d & 2) / 2 + (d & 4) / 2 + ((s & 4) / 4) * 3;
+                f64 = (dl & 4) / 4 + (dl & 8) / 4 + ((s & 8) / 8) * 3;
+                c = (ur & 4) / 4;
+            } else if (j % 2 == 0) {
+                const int ur = t[x - N + 1], r = t[x + 1], dl = t[x + N - 1], l = t[x - 1];
+                f1 = (ur & 1) + (ur & 2) + (s & 1) * 3;
+                f4 = (r & 2) / 2 + (r & 4) / 2 + ((s & 2) / 2) * 3;
+                f16 = (dl & 4) / 4 + (dl & 8) / 4 + ((s & 4) / 4) * 3;
+                f64 = (l & 8) / 8 + (l & 1) * 2 + ((s & 8) / 8) * 3;
+                c = (r & 8) / 8;
+            } else {
+                const int u = t[x - N], r = t[x + 1], d = t[x + N], l = t[x - 1];
+                f1 = (u & 1) + (u & 2) + (s & 1) * 3;
+                f4 = (r & 1) + (r & 2) + ((s & 2) / 2) * 3;
+                f16 = (d & 4) / 4 + (d & 8) / 4 + ((s & 4) / 4) * 3;
+                f64 = (l & 4) / 4 + (l & 8) / 4 + ((s & 8) / 8) * 3;
+                c = (r & 8) / 8;
+            }
+            o[x] = f1 + 4 * f4 + 16 * f16 + 64 * f64 + 256 * c;
+        }
+    return o;
+}
+
+// One step of TriangleDimerTiler::Walk (:217-338) given the host draws.
+void td_step(TdArrays& s, std::vector<TM>& g, int type, int r1, int r2, int p1, int p2) {
+    const int M = s.M;
+    int32_t *eH = s.eH.data(), *eL = s.eL.data(), *eR = s.eR.data(), *tU = s.tU.data(), *tD = s.tD.data();
+    int32_t *bH = s.bH.data(), *bL = s.bL.data(), *bR = s.bR.data();
+    if (type == 0) {
+        int32_t* e[3] = {eH, eL, eR};
+        td_rotate_lozenges(g, e[r1], M, r1, r2);
+        td_update_lozenges_flipped(e[r1], M, r1, r2);
+        td_update_lozenges(e[r1], e[(r1 + 1) % 3], e[(r1 + 2) % 3], M, r1);
+        td_triangle_u_from_lozenges(eH, eL, tU, M);
+        td_update_triangles(tU, tD, M, 0);
+        td_butterflies_from_lozenges(0, bH, eH, eL, eR, M);
+        td_butterflies_from_lozenges(1, bL, eH, eL, eR, M);
+        td_butterflies_from_lozenges(2, bR, eH, eL, eR, M);
+    } else if (type == 1) {
+        int32_t* a = (r2 == 1) ? tD : tU;
+        td_rotate_triangles(g, a, M, r1);
+        td_update_triangles_flipped(a, M, r1, r2 == 1);
+        if (r2 == 1) td_update_triangles(tD, tU, M, 1);
+        else td_update_triangles(tU, tD, M, 0);
+        td_lozenges_from_triangles(tU, tD, eH, eL, eR, M);
+        td_butterflies_from_lozenges(0, bH, eH, eL, eR, M);
+        td_butterflies_from_lozenges(1, bL, eH, eL, eR, M);
+        td_butterflies_from_lozenges(2, bR, eH, eL, eR, M);
+    } else {
+        int32_t* b[3] = {bH, bL, bR};
+        td_rotate_butterflies(g, b[r1], M, r1, p1, p2);
+        td_bfly_flipped1(b[r1], M, r1, p1, p2);
+        td_bfly_flipped21(b[r1], M, r1, p1);
+        td_bfly_flipped22(b[r1], M, r1, p1);
+        td_bfly_flipped23(b[r1], M, r1, p1);
+        td_lozenges_from_butterflies(r1, b[r1], eH, eL, eR, M);
+        for (int o = 0; o < 3; o++)
+            if (o != r1) td_butterflies_from_lozenges(o, b[o], eH, eL, eR, M);
+        td_triangle_u_from_lozenges(eH, eL, tU, M);
+        td_update_triangles(tU, tD, M, 0);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -511,6 +939,79 @@ double or_lozenge_time(int rule, double q, int32_t* t, int N, long steps, long s
     auto t1 = std::chrono::steady_clock::now();
     if (rc) return -1.0;
     return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// ---- Triangle dimers -------------------------------------------------------------------
+// TriangleDimerTiler::Walk(tiling&, int steps, long seed), TriangleDimerTiler.cpp:86-483:
+// split the N x N lozenge-state tiling into eH (even rows, odd columns), eL (odd, even) and
+// eR (odd, odd), derive the triangle and butterfly states, and per step draw
+// type = d3, r1 = d3, r2 = d2 (and p1 = d3, p2 = d2 for butterfly moves) from one
+// minstd_rand0 seeded with `seed`.  Only the lozenge states are written back.
+// If `moves` is non-null it receives type | r1<<2 | r2<<4 | p1<<5 | p2<<7 per step.
+int or_tridimer_walk(int32_t* t, int N, long steps, long seed, const uint32_t* params, int64_t nparams,
+                     uint8_t* moves) {
+    if (N < 8 || N % 2) return -1;
+    const int M = N / 2;
+    const int64_t G = (int64_t)M * M;
+    if (G > nparams) return -2;
+    TdArrays s;
+    s.M = M;
+    for (auto* v : {&s.eH, &s.eL, &s.eR, &s.tU, &s.tD, &s.bH, &s.bL, &s.bR}) v->assign((size_t)G, 0);
+    const std::vector<int32_t> tri = td_triangle_flips(t, N);
+    const std::vector<int32_t> bf = td_butterfly_flips(t, N);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            const size_t k = (size_t)(i / 2) * M + j / 2;
+            if (i % 2 == 0) {
+                if (j % 2 == 1) s.eH[k] = t[i * N + j], s.bH[k] = bf[i * N + j];
+            } else if (j % 2 == 0) {
+                s.eL[k] = t[i * N + j], s.bL[k] = bf[i * N + j];
+            } else {
+                s.eR[k] = t[i * N + j], s.bR[k] = bf[i * N + j];
+            }
+        }
+    for (int i = 0; i < N / 2; i++)
+        for (int j = 0; j < N; j++) (j % 2 == 0 ? s.tD : s.tU)[(size_t)i * M + j / 2] = tri[(size_t)i * N + j];
+    Table tb(params, nparams);
+    std::default_random_engine generator;
+    std::uniform_int_distribution<int> d1(0, 2);
+    std::uniform_int_distribution<int> d2(0, 1);
+    generator.seed(seed);
+    tb.init(low32(seed), G);
+    for (long st = 0; st < steps; st++) {
+        const int type = d1(generator);
+        const int r1 = d1(generator);
+        const int r2 = d2(generator);
+        int p1 = 0, p2 = 0;
+        if (type == 2) {
+            p1 = d1(generator);
+            p2 = d2(generator);
+        }
+        if (moves) moves[st] = (uint8_t)(type | r1 << 2 | r2 << 4 | p1 << 5 | p2 << 7);
+        td_step(s, tb.g, type, r1, r2, p1, p2);
+    }
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            const size_t k = (size_t)(i / 2) * M + j / 2;
+            if (i % 2 == 0) {
+                if (j % 2 == 1) t[i * N + j] = s.eH[k];
+            } else {
+                t[i * N + j] = (j % 2 == 0) ? s.eL[k] : s.eR[k];
+            }
+        }
+    return 0;
+}
+
+// The derived triangle / butterfly arrays of a tiling (TilingToTriangleFlips,
+// TilingToButterflyFlips), for tests of the product's statics.
+void or_tridimer_flips(const int32_t* t, int N, int32_t* tri, int32_t* bfly) {
+    const std::vector<int32_t> a = td_triangle_flips(t, N), b = td_butterfly_flips(t, N);
+    std::memcpy(tri, a.data(), a.size() * 4);
+    std::memcpy(bfly, b.data(), b.size() * 4);
 }
 
 }  // extern "C"

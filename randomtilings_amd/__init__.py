@@ -52,6 +52,7 @@ SIGNATURES = {
     "rt_domino_walk_epochs": (C.c_int, [_P, C.c_int, _DP, _I32P, C.c_int32, _I64P, _I64P, C.c_int32]),
     "rt_lozenge_walk": (C.c_int, [_P, C.c_int, C.c_double, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_recttri_walk": (C.c_int, [_P, _FP, _I32P, C.c_int32, C.c_int64, C.c_int64]),
+    "rt_tridimer_walk": (C.c_int, [_P, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_chain_create": (C.c_int, [_P, C.c_int, C.c_int, _DP, C.c_int32, _I32P, C.c_int32, C.POINTER(_P)]),
     "rt_chain_destroy": (C.c_int, [_P]),
     "rt_chain_set_path": (C.c_int, [_P, C.c_int, C.c_int32]),
@@ -213,6 +214,12 @@ class Context:
         w = None if weights16 is None else np.ascontiguousarray(weights16, dtype=np.float32)
         _check(self._lib.rt_recttri_walk(self.h, None if w is None else w.ctypes.data_as(_FP), _i32(t), N, steps,
                                          seed))
+        return t
+
+    def tridimer_walk(self, t: np.ndarray, steps: int, seed: int) -> np.ndarray:
+        t = np.ascontiguousarray(t, dtype=np.int32).copy()
+        N = int(round(np.sqrt(t.size)))
+        _check(self._lib.rt_tridimer_walk(self.h, _i32(t), N, steps, seed))
         return t
 
     def chain(self, model: int, rule: int, tiling: np.ndarray, rparams=None) -> "Chain":

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "rt_internal.hpp"
+#include "rt_tridimer.hpp"
 
 namespace rt {
 bool domino_rule(int rule, const double* p12, FlipRule& fr);
@@ -80,6 +81,8 @@ struct rt_chain {
     std::mt19937 mt;                      // domino schedule (DominoTiler.cpp:31,81)
     std::minstd_rand0 lcg;                // lozenge / rect-tri schedule (LozengeTiler.cpp:50-52)
     std::uniform_int_distribution<int> d3{0, 2};
+    std::uniform_int_distribution<int> d2{0, 1};  // triangle-dimer schedule (TriangleDimerTiler.cpp:211-213)
+    int32_t* tsave = nullptr;  // triangle dimers: the caller's tiling; merge writes the move states into it
     int path = RT_PATH_AUTO;
     int cfg = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -90,6 +93,186 @@ struct rt_chain {
     int64_t last_launches = 0;
     int32_t last_cfg = -1;
 };
+
+// Tile configurations chosen by rt_chain_autotune, keyed by (model, side, uniform params).
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, bool>, int> g_tuned;
+
+// ---- triangle dimers (TriangleDimerTiler::Walk, TriangleDimerTiler.cpp:86-483) ----------
+// The chain state is M x M cells of eight 16-bit move states (M = N/2), derived on the
+// device from the uploaded tiling; the host draws each step's move exactly as Walk does.
+
+static int td_create(rt_chain* ch, const int32_t* tiling) {
+    const int N = ch->N;
+    // TriangleDimerTiler.h:61-63: N even, three rows/columns of zero padding
+    if (N < 8 || N % 2) return fail(RT_ERR_ARG, "triangle-dimer tiling side %d must be even and >= 8", N);
+    for (int64_t i = 0; i < (int64_t)N * N; i++)
+        if (tiling[i] < 0 || tiling[i] > 15)
+            return fail(RT_ERR_ARG, "triangle-dimer lozenge state %d out of the 4-bit range", tiling[i]);
+    const int M = N / 2;
+    ch->NC = 8;
+    ch->W = M;
+    ch->P = M;
+    ch->esize = 2;
+    ch->G = (int64_t)M * M;
+    hipError_t e = hipSetDevice(ch->ctx->device);
+    const size_t cbytes = sizeof(uint4) * (size_t)ch->G;
+    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&ch->cls[b], cbytes);
+    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&ch->gens[b], sizeof(uint4) * (size_t)ch->G);
+    if (e == hipSuccess) e = hipMalloc(&ch->tsave, sizeof(int32_t) * (size_t)N * N);
+    if (e == hipSuccess) e = hipMalloc(&ch->misc, 64);
+    if (e == hipSuccess) e = hipEventCreate(&ch->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ch->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_sched, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMemcpy(ch->tsave, tiling, sizeof(int32_t) * (size_t)N * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(ch->misc, 0, 64);
+    if (e == hipSuccess) e = td_launch_derive(ch->tsave, N, (uint4*)ch->cls[0], ch->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ch->ctx->stream);
+    if (e != hipSuccess) return fail(RT_ERR_HIP, "triangle-dimer chain setup: %s", hipGetErrorString(e));
+    ch->cur = 0;
+    return RT_OK;
+}
+
+static int td_run(rt_chain* ch, int64_t steps) {
+    rt_ctx* c = ch->ctx;
+    // TriangleDimerTiler.cpp:217-222, :291-292: type, r1, r2, then p1, p2 for butterflies
+    for (int64_t s = 0; s < steps; s++) {
+        const int type = ch->d3(ch->lcg), r1 = ch->d3(ch->lcg), r2 = ch->d2(ch->lcg);
+        int p1 = 0, p2 = 0;
+        if (type == 2) {
+            p1 = ch->d3(ch->lcg);
+            p2 = ch->d2(ch->lcg);
+        }
+        ch->hsched[s] = td_move(type, r1, r2, p1, p2);
+    }
+    TdArgs a{};
+    a.params = c->uniform ? nullptr : c->params;
+    a.m1 = c->um1;
+    a.m2 = c->um2;
+    a.tm = c->utm;
+    a.M = ch->W;
+    const bool small = ch->W <= kSmallMax && ch->path != RT_PATH_STEPWISE;
+    if (ch->path == RT_PATH_PERSISTENT && !small)
+        return fail(RT_ERR_ARG, "the single-workgroup triangle-dimer kernel holds at most %d x %d sites", kSmallMax,
+                    kSmallMax);
+    HIPCHK(hipEventRecord(ch->ev0, c->stream));
+    if (small) {
+        if (ch->dsched_cap < steps) {
+            if (ch->dsched) {
+                HIPCHK(hipStreamSynchronize(c->stream));
+                HIPCHK(hipFree(ch->dsched));
+                ch->dsched = nullptr;
+            }
+            HIPCHK(hipMalloc(&ch->dsched, (size_t)steps + 16));
+            ch->dsched_cap = steps;
+        }
+        HIPCHK(hipMemcpyAsync(ch->dsched, ch->hsched.data(), (size_t)steps, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipEventRecord(ch->ev_sched, c->stream));
+        ch->sched_pending = true;
+        const int64_t chunk = 1 << 16;  // keeps single launches short
+        for (int64_t s0 = 0; s0 < steps; s0 += chunk) {
+            a.cells_in = (const uint4*)ch->cls[ch->cur];
+            a.cells_out = (uint4*)ch->cls[ch->cur ^ 1];
+            a.gen_in = ch->gens[ch->gcur];
+            a.gen_out = ch->gens[ch->gcur ^ 1];
+            a.sched = ch->dsched + s0;
+            a.nsteps = std::min<int64_t>(chunk, steps - s0);
+            HIPCHK(td_launch_small(a, c->uniform, c->stream));
+            ch->cur ^= 1;
+            ch->gcur ^= 1;
+            ch->last_launches++;
+        }
+        ch->last_cfg = -1;
+    } else {
+        int cfg = ch->cfg;
+        if (cfg < 0) {
+            cfg = 0;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned.find(std::make_tuple(ch->model, ch->N, c->uniform));
+            if (it != g_tuned.end()) cfg = it->second;
+        }
+        for (int64_t s = 0; s < steps; s++) {
+            a.cells_in = (const uint4*)ch->cls[ch->cur];
+            a.cells_out = (uint4*)ch->cls[ch->cur ^ 1];
+            a.gen_in = ch->gens[ch->gcur];
+            a.gen_out = ch->gens[ch->gcur ^ 1];
+            HIPCHK(td_launch_step(cfg, a, c->uniform, ch->hsched[s], c->stream));
+            ch->cur ^= 1;
+            ch->gcur ^= 1;
+            ch->last_launches++;
+        }
+        ch->last_cfg = cfg;
+    }
+    HIPCHK(hipEventRecord(ch->ev1, c->stream));
+    ch->timed = true;
+    return RT_OK;
+}
+
+static int td_read(rt_chain* ch, int32_t* tiling) {
+    HIPCHK(hipSetDevice(ch->ctx->device));
+    HIPCHK(td_launch_merge((const uint4*)ch->cls[ch->cur], ch->N, ch->tsave, ch->ctx->stream));
+    HIPCHK(hipStreamSynchronize(ch->ctx->stream));
+    HIPCHK(hipMemcpy(tiling, ch->tsave, sizeof(int32_t) * (size_t)ch->N * ch->N, hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+// Measure every one-step tile configuration on a scratch copy of the chain (its own state is
+// not advanced); chains that fit one workgroup always run the single-workgroup kernel.
+static int td_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
+    rt_ctx* c = ch->ctx;
+    if (ch->W <= kSmallMax) {
+        if (best) *best = -1;
+        return RT_OK;
+    }
+    rt_chain* t = new rt_chain();
+    t->ctx = c;
+    t->model = ch->model;
+    t->N = ch->N;
+    t->W = ch->W;
+    t->P = ch->P;
+    t->NC = ch->NC;
+    t->esize = ch->esize;
+    t->G = ch->G;
+    const size_t cbytes = sizeof(uint4) * (size_t)ch->G;
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&t->cls[b], cbytes);
+    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&t->gens[b], cbytes);
+    if (e == hipSuccess) e = hipMalloc(&t->misc, 64);
+    if (e == hipSuccess) e = hipEventCreate(&t->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&t->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->ev_sched, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMemsetAsync(t->misc, 0, 64, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(t->cls[0], ch->cls[ch->cur], cbytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e != hipSuccess) {
+        rt_chain_destroy(t);
+        return fail(RT_ERR_HIP, "autotune clone: %s", hipGetErrorString(e));
+    }
+    int rc = rt_chain_seed(t, RT_SEED_WALK, 12345);
+    int chosen = -1;
+    double best_ms = 1e30;
+    for (int i = 0; i < td_config_count() && rc == RT_OK; i++) {
+        t->path = RT_PATH_STEPWISE;
+        t->cfg = i;
+        rc = rt_chain_run(t, probe_steps / 4 + 1);
+        if (rc == RT_OK) rc = rt_chain_run(t, probe_steps);
+        double ms = 0;
+        if (rc == RT_OK) rc = rt_chain_elapsed_ms(t, &ms);
+        if (rc == RT_OK) rc = rt_chain_sync(t);
+        if (rc == RT_OK && ms < best_ms) {
+            best_ms = ms;
+            chosen = i;
+        }
+    }
+    rt_chain_destroy(t);
+    if (rc != RT_OK) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[std::make_tuple(ch->model, ch->N, c->uniform)] = chosen;
+    }
+    ch->cfg = chosen;
+    if (best) *best = chosen;
+    return RT_OK;
+}
 
 extern "C" {
 
@@ -320,6 +503,7 @@ int rt_chain_destroy(rt_chain* ch) {
         if (ch->gens[b]) hipFree(ch->gens[b]);
     }
     if (ch->dsched) hipFree(ch->dsched);
+    if (ch->tsave) hipFree(ch->tsave);
     if (ch->misc) hipFree(ch->misc);
     if (ch->bar) hipFree(ch->bar);
     if (ch->ev0) hipEventDestroy(ch->ev0);
@@ -384,6 +568,14 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
             }
             recttri_rule(wp, ch->rp.rt);
         }
+    } else if (model == RT_TRIDIMER) {
+        const int rc = td_create(ch, tiling);
+        if (rc != RT_OK) {
+            rt_chain_destroy(ch);
+            return rc;
+        }
+        *out = ch;
+        return RT_OK;
     } else {
         delete ch;
         return fail(RT_ERR_ARG, "rt_chain_create: model %d has no planar chain", model);
@@ -431,7 +623,8 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
 
 int rt_chain_set_path(rt_chain* ch, int path, int32_t cfg) {
     if (!ch || path < RT_PATH_AUTO || path > RT_PATH_STEPWISE) return fail(RT_ERR_ARG, "rt_chain_set_path: bad args");
-    if (cfg >= persist_config_count(ch->model)) return fail(RT_ERR_ARG, "tile config %d out of range", cfg);
+    const int ncfg = ch->model == RT_TRIDIMER ? td_config_count() : persist_config_count(ch->model);
+    if (cfg >= ncfg) return fail(RT_ERR_ARG, "tile config %d out of range", cfg);
     ch->path = path;
     ch->cfg = cfg;
     return RT_OK;
@@ -455,6 +648,7 @@ int rt_chain_seed(rt_chain* ch, int mode, int64_t seed) {
         // LozengeTiler.cpp:47-52, RectTriangleTiler.cpp:41-45
         ch->lcg.seed(lseed);
         ch->d3.reset();
+        ch->d2.reset();
         tmseed = (uint32_t)(uint64_t)seed;
     }
     HIPCHK(hipSetDevice(c->device));
@@ -464,9 +658,6 @@ int rt_chain_seed(rt_chain* ch, int mode, int64_t seed) {
     return RT_OK;
 }
 
-// Tile configurations chosen by rt_chain_autotune, keyed by (model, side, uniform params).
-static std::mutex g_tune_mu;
-static std::map<std::tuple<int, int, bool>, int> g_tuned;
 
 static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c) {
     const int n = persist_config_count(ch->model);
@@ -513,6 +704,7 @@ int rt_chain_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
     rt_ctx* c = ch->ctx;
     HIPCHK(hipSetDevice(c->device));
     if (!c->params || c->nparams < ch->G) return fail(RT_ERR_PARAMS, "autotune needs the parameter table loaded");
+    if (ch->model == RT_TRIDIMER) return td_autotune(ch, probe_steps, best);
     // scratch clone: same shape and rule, the chain's current state, throwaway generators
     rt_chain* t = new rt_chain();
     t->ctx = c;
@@ -584,8 +776,13 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         HIPCHK(hipEventSynchronize(ch->ev_sched));
         ch->sched_pending = false;
     }
-    // host colour schedule, drawn exactly as the reference's Walk loop draws it
     ch->hsched.resize((size_t)std::max<int64_t>(steps, 1));
+    if (ch->model == RT_TRIDIMER) {
+        ch->timed = false;
+        ch->last_launches = 0;
+        return steps == 0 ? RT_OK : td_run(ch, steps);
+    }
+    // host colour schedule, drawn exactly as the reference's Walk loop draws it
     for (int64_t s = 0; s < steps; s++) {
         if (ch->model == RT_DOMINO) {
             const int r = ch->mt() % 2;  // DominoTiler.cpp:35-36: r==1 rotates B (class 0)
@@ -704,6 +901,7 @@ int rt_chain_read(rt_chain* ch, int32_t* tiling) {
     if (!ch || !tiling) return fail(RT_ERR_ARG, "rt_chain_read: null argument");
     int rc = rt_chain_sync(ch);
     if (rc) return rc;
+    if (ch->model == RT_TRIDIMER) return td_read(ch, tiling);
     std::vector<uint8_t> h((size_t)ch->N * ch->P * ch->NC * ch->esize);
     HIPCHK(hipMemcpy(h.data(), ch->cls[ch->cur], h.size(), hipMemcpyDeviceToHost));
     merge(ch, h, tiling);
@@ -713,15 +911,19 @@ int rt_chain_read(rt_chain* ch, int32_t* tiling) {
 int64_t rt_chain_attempts_per_step(rt_chain* ch) { return ch ? ch->G : 0; }
 
 int64_t rt_chain_state_bytes(rt_chain* ch) {
+    if (ch && ch->model == RT_TRIDIMER) return ch->G * (int64_t)sizeof(uint4);
     return ch ? (int64_t)ch->N * ch->W * ch->NC * ch->esize : 0;
 }
 
-int32_t rt_tile_config_count(int model) { return persist_config_count(model); }
+int32_t rt_tile_config_count(int model) {
+    return model == RT_TRIDIMER ? td_config_count() : persist_config_count(model);
+}
 
 int rt_tile_config(int model, int32_t idx, int32_t* rows, int32_t* cols, int32_t* halo, int32_t* threads) {
     if (!rows || !cols || !halo || !threads) return fail(RT_ERR_ARG, "null argument");
-    int tr, tc, h, nt;
-    if (!persist_config(model, idx, &tr, &tc, &h, &nt)) return fail(RT_ERR_ARG, "config %d out of range", idx);
+    int tr, tc, h = kHalo, nt;
+    const bool ok = model == RT_TRIDIMER ? td_config(idx, &tr, &tc, &nt) : persist_config(model, idx, &tr, &tc, &h, &nt);
+    if (!ok) return fail(RT_ERR_ARG, "config %d out of range", idx);
     *rows = tr;
     *cols = tc;
     *halo = h;
@@ -733,18 +935,12 @@ int64_t rt_chain_last_launches(rt_chain* ch) { return ch ? ch->last_launches : 0
 
 int32_t rt_chain_last_config(rt_chain* ch) { return ch ? ch->last_cfg : -1; }
 
-int32_t rt_chain_tile_configs(rt_chain* ch) { return ch ? persist_config_count(ch->model) : 0; }
+int32_t rt_chain_tile_configs(rt_chain* ch) { return ch ? rt_tile_config_count(ch->model) : 0; }
 
 int rt_chain_tile_config_info(rt_chain* ch, int32_t idx, int32_t* rows, int32_t* cols, int32_t* halo,
                               int32_t* threads) {
-    if (!ch || !rows || !cols || !halo || !threads) return fail(RT_ERR_ARG, "null argument");
-    int tr, tc, h, nt;
-    if (!persist_config(ch->model, idx, &tr, &tc, &h, &nt)) return fail(RT_ERR_ARG, "config %d out of range", idx);
-    *rows = tr;
-    *cols = tc;
-    *halo = h;
-    *threads = nt;
-    return RT_OK;
+    if (!ch) return fail(RT_ERR_ARG, "null argument");
+    return rt_tile_config(ch->model, idx, rows, cols, halo, threads);
 }
 
 // ---- Walk entry points ----------------------------------------------------------------
@@ -776,6 +972,10 @@ int rt_domino_walk_epochs(rt_ctx* ctx, int rule, const double* rparams, int32_t*
 
 int rt_lozenge_walk(rt_ctx* ctx, int rule, double q, int32_t* tiling, int32_t N, int64_t steps, int64_t seed) {
     return walk(ctx, RT_LOZENGE, rule, &q, 1, tiling, N, &steps, &seed, 1, RT_SEED_WALK);
+}
+
+int rt_tridimer_walk(rt_ctx* ctx, int32_t* tiling, int32_t N, int64_t steps, int64_t seed) {
+    return walk(ctx, RT_TRIDIMER, 0, nullptr, 0, tiling, N, &steps, &seed, 1, RT_SEED_WALK);
 }
 
 int rt_recttri_walk(rt_ctx* ctx, const float* weights16, int32_t* tiling, int32_t N, int64_t steps, int64_t seed) {

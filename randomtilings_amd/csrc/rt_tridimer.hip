@@ -1,0 +1,641 @@
+// rt_tridimer.hip — the triangle-dimer chain (TriangleDimerTiler::Walk,
+// TriangleDimerTiler.cpp:86-483, kernels triangledimerkernel.cl) on gfx950.
+//
+// State: per site (a, b) of the M x M move lattice (M = N/2) eight small integers — the
+// lozenge-move states eH, eL, eR, the triangle-move states tU, tD and the butterfly-move
+// states bH, bL, bR — packed as eight 16-bit fields in one 16-byte cell, so a site moves
+// between HBM and LDS as one vector access.  Values stay below 2^10 whenever the input
+// tiling's states are 4-bit (checked on the host), so 16 bits reproduce the reference's
+// int arithmetic exactly.
+//
+// A reference step is one Rotate kernel plus three to nine update kernels, each reading the
+// previous one's output at distance <= 1.  Here a step is one kernel: every workgroup loads
+// its tile plus a halo of kHalo sites into LDS, draws every loaded site's TinyMT number,
+// applies the rotation, and replays the update kernels as LDS phases separated by workgroup
+// barriers.  Phase p only computes sites at least p away from a halo edge that cuts the
+// domain, which is exactly the region whose inputs are still valid; kHalo = 6 covers the
+// longest chain (butterfly moves).  Generators are double-buffered: halo copies read the
+// previous states while owners write the new ones.
+//
+// Domains up to 64 x 64 sites run every step in one launch of one workgroup (the whole
+// domain in LDS, generator states in VGPRs), with only workgroup barriers between phases.
+#include "rt_internal.hpp"
+#include "rt_tridimer.hpp"
+
+namespace rt {
+namespace td {
+
+enum Plane { EH = 0, EL = 1, ER = 2, TU = 3, TD = 4, BH = 5, BL = 6, BR = 7 };
+
+struct Move {
+    int type, r1, r2, p1, p2;
+};
+__device__ __forceinline__ Move decode(uint32_t m) {
+    return {(int)(m & 3), (int)((m >> 2) & 3), (int)((m >> 4) & 1), (int)((m >> 5) & 3), (int)((m >> 7) & 1)};
+}
+
+__device__ __forceinline__ int mod3(int x) { return ((x % 3) + 3) % 3; }
+
+// Region of the move lattice held in LDS: eight uint16 planes of `rows` x pitch.
+struct View {
+    uint16_t* base;
+    int pitch, plane;
+    __device__ __forceinline__ int g(int p, int i, int j) const { return base[p * plane + i * pitch + j]; }
+    __device__ __forceinline__ void s(int p, int i, int j, int v) const { base[p * plane + i * pitch + j] = (uint16_t)v; }
+};
+
+// ---- Rotate kernels (triangledimerkernel.cl:4-22, :192-212, :330-352) ------------------
+// The plane a move rotates, and the rotated value of state v at global site (gi, gj) given
+// the draw's low half (rd < 0.5 <=> tempered output < 2^31).
+__device__ __forceinline__ int rotate_plane(const Move& m) {
+    return m.type == 0 ? EH + m.r1 : m.type == 1 ? (m.r2 ? TD : TU) : BH + m.r1;
+}
+__device__ __forceinline__ int rotate_value(const Move& m, int v, int gi, int gj, bool low) {
+    if (!low) return v;
+    if (m.type == 0) {
+        const int t = m.r1;
+        if (((1 - (t & 1)) * gi + ((1 + t) / 2) * gj) % 2 != m.r2) return v;
+        return v == 5 ? 10 : v == 10 ? 5 : v;
+    }
+    if (m.type == 1) {
+        if (mod3(gj - gi) != m.r1) return v;
+        return v == 45 ? 54 : v == 54 ? 45 : v;
+    }
+    const int t = m.r1, b1 = 1 - (t & 1), b2 = (1 + t) / 2, b3 = t & 1;
+    if ((b1 * gi + b2 * gj) % 3 != m.p1) return v;
+    const bool par = ((b1 * gj + b3 * gi) & 1) == m.p2;
+    if (v == 170) return par ? 85 : 170;
+    if (v == 85) return par ? 170 : 85;
+    return v;
+}
+
+// ---- update kernels, per site (local i, j; global gi, gj) -------------------------------
+
+// UpdateLozengesFlipped, :24-34
+__device__ __forceinline__ void loz_flipped(const View& V, int i, int j, int gi, int gj, int t, int c) {
+    if (((1 - (t & 1)) * gi + ((1 + t) / 2) * gj) % 2 == c) return;
+    const int a = EH + t, k = (2 - t) / 2, s = t & 1;
+    V.s(a, i, j,
+        (V.g(a, i - 1, j + s) & 4) / 4 + (V.g(a, i - k, j + 1) & 8) / 4 + (V.g(a, i + 1, j - s) & 1) * 4 +
+            (V.g(a, i + k, j - 1) & 2) * 4);
+}
+
+// UpdateLozenges0/1/2, :36-81
+__device__ __forceinline__ void loz_update(const View& V, int i, int j, int t) {
+    const int A = EH + t, B = EH + (t + 1) % 3, C = EH + (t + 2) % 3;
+    int b = V.g(B, i, j), c = V.g(C, i, j);
+    if (t == 0) {
+        b = (b & ~10) | ((V.g(A, i + 1, j) & 1) * 2 + (V.g(A, i + 1, j - 1) & 1) * 8);
+        c = (c & ~10) | ((V.g(A, i + 1, j) & 2) + (V.g(A, i, j) & 8));
+    } else if (t == 1) {
+        b = (b & ~5) | ((V.g(A, i, j) & 1) + (V.g(A, i + 1, j) & 1) * 4);
+        c = (c & ~5) | ((V.g(A, i - 1, j) & 2) / 2 + (V.g(A, i, j) & 2) * 2);
+    } else {
+        b = (b & ~10) | ((V.g(A, i - 1, j) & 2) + (V.g(A, i, j - 1) & 2) * 4);
+        c = (c & ~5) | ((V.g(A, i, j) & 1) + (V.g(A, i, j - 1) & 4));
+    }
+    V.s(B, i, j, b);
+    V.s(C, i, j, c);
+}
+
+__device__ __forceinline__ int sel(bool c, int v) { return c ? v : 0; }
+
+// UpdateTriangleUFromLozenges, :100-121
+__device__ __forceinline__ void tri_u_from_loz(const View& V, int i, int j) {
+    const int l0 = V.g(EL, i, j), l1 = V.g(EL, i, j + 1), h0 = V.g(EH, i, j), h1 = V.g(EH, i + 1, j);
+    const int L = sel(l0 & 1, 1) + sel(h0 & 8, 2) + sel(h1 & 1, 3);
+    const int R = sel(l1 & 1, 1) + sel(h1 & 2, 2) + sel(l1 & 2, 3);
+    const int U = sel(l1 & 4, 1) + sel(h1 & 8, 2) + sel(h1 & 4, 3);
+    V.s(TU, i, j, L + 4 * R + 16 * U);
+}
+
+__device__ __forceinline__ int pack(int t1, int t4, int t16, int t64, int c) {
+    return t1 + 4 * t4 + 16 * t16 + 64 * t64 + 256 * c;
+}
+
+// UpdateButterflys{H,L,R}FromLozenge, :123-190
+__device__ __forceinline__ void bfly_from_loz(const View& V, int i, int j, int w) {
+    int c, t1, t4, t16, t64;
+    if (w == 0) {
+        const int lh = V.g(EH, i, j), lr1 = V.g(ER, i - 1, j), ll1 = V.g(EL, i - 1, j + 1), lr2 = V.g(ER, i, j),
+                  ll2 = V.g(EL, i, j);
+        c = (ll1 & 4) / 4;
+        t1 = (lr1 & 8) / 8 + 2 * (lr1 & 1) + 3 * (lh & 1);
+        t4 = (ll1 & 1) + (ll1 & 2) + ((lh & 2) / 2) * 3;
+        t16 = (lr2 & 2) / 2 + (lr2 & 4) / 2 + ((lh & 4) / 4) * 3;
+        t64 = (ll2 & 4) / 4 + (ll2 & 8) / 4 + ((lh & 8) / 8) * 3;
+    } else if (w == 1) {
+        const int ll = V.g(EL, i, j), lh1 = V.g(EH, i, j), lr1 = V.g(ER, i, j), lh2 = V.g(EH, i + 1, j - 1),
+                  lr2 = V.g(ER, i, j - 1);
+        c = (lr1 & 8) / 8;
+        t1 = (lh1 & 1) + (lh1 & 2) + 3 * (ll & 1);
+        t4 = (lr1 & 2) / 2 + (lr1 & 4) / 2 + ((ll & 2) / 2) * 3;
+        t16 = (lh2 & 4) / 4 + (lh2 & 8) / 4 + ((ll & 4) / 4) * 3;
+        t64 = (lr2 & 8) / 8 + 2 * (lr2 & 1) + ((ll & 8) / 8) * 3;
+    } else {
+        const int lr = V.g(ER, i, j), lh1 = V.g(EH, i, j), ll1 = V.g(EL, i, j + 1), lh2 = V.g(EH, i + 1, j),
+                  ll2 = V.g(EL, i, j);
+        c = (ll1 & 8) / 8;
+        t1 = (lh1 & 1) + (lh1 & 2) + 3 * (lr & 1);
+        t4 = (ll1 & 1) + (ll1 & 2) + ((lr & 2) / 2) * 3;
+        t16 = (lh2 & 4) / 4 + (lh2 & 8) / 4 + ((lr & 4) / 4) * 3;
+        t64 = (ll2 & 4) / 4 + (ll2 & 8) / 4 + ((lr & 8) / 8) * 3;
+    }
+    V.s(BH + w, i, j, pack(t1, t4, t16, t64, c));
+}
+
+// UpdateTrianglesFlipped0 (:214-234) / UpdateTrianglesFlipped1 (:236-256)
+__device__ __forceinline__ void tri_flipped(const View& V, int i, int j, int gi, int gj, int c, bool down) {
+    const int cl = mod3(gj - gi);
+    if (cl == c) return;
+    const int p = (cl == (c + 1) % 3) ? 1 : 0;
+    const int t1 = (4 - 3 * p) * 12, m1 = (4 - 3 * p) * 4;
+    const int t4 = (15 * p + 1) * 3, m4 = 15 * p + 1;
+    const int t16 = (4 - 3 * p) * 3, m16 = 4 - 3 * p;
+    const int a = down ? TD : TU;
+    int x1, x4, x16;
+    if (!down) {
+        x1 = V.g(a, i - (1 - p), j - p);
+        x4 = V.g(a, i - p, j + 1);
+        x16 = V.g(a, i + 1, j - 1 + p);
+    } else {
+        x1 = V.g(a, i + 1 - p, j - 1);
+        x4 = V.g(a, i + p, j + 1 - p);
+        x16 = V.g(a, i - 1, j + p);
+    }
+    V.s(a, i, j, (x1 & t1) / m1 + ((x4 & t4) / m4) * 4 + ((x16 & t16) / m16) * 16);
+}
+
+// UpdateTriangles, :258-281: t = 0 down from up, t = 1 up from down
+__device__ __forceinline__ int tri_digit(int v) {
+    return sel((v & 48) == 16, 1) + sel((v & 3) == 3, 2) + sel((v & 12) == 8, 3);
+}
+__device__ __forceinline__ void tri_update(const View& V, int i, int j, int t) {
+    const int src = t ? TD : TU, dst = t ? TU : TD, p = t ? 1 : -1;
+    V.s(dst, i, j,
+        tri_digit(V.g(src, i, j - 1 + t)) + 4 * tri_digit(V.g(src, i, j + t)) + 16 * tri_digit(V.g(src, i + p, j)));
+}
+
+// UpdateLozenge{H,L,R}FromTriangles, :283-328
+__device__ __forceinline__ void loz_from_tri(const View& V, int i, int j) {
+    const int d = V.g(TD, i, j), uh = V.g(TU, i - 1, j), ul = V.g(TU, i, j - 1), ur = V.g(TU, i, j);
+    V.s(EH, i, j, sel((d & 48) == 32, 1) + sel((d & 48) == 48, 2) + sel((uh & 48) == 48, 4) + sel((uh & 48) == 32, 8));
+    V.s(EL, i, j, sel((ul & 12) == 4, 1) + sel((ul & 12) == 12, 2) + sel((d & 3) == 1, 4) + sel((d & 3) == 2, 8));
+    V.s(ER, i, j, sel((ur & 3) == 1, 1) + sel((d & 12) == 12, 2) + sel((d & 12) == 4, 4) + sel((ur & 3) == 2, 8));
+}
+
+__device__ __forceinline__ int bslice(int t, int gi, int gj) { return t == 0 ? gi % 3 : t == 1 ? gj % 3 : (gi + gj) % 3; }
+
+// UpdateButterflysFlipped{H,L,R}1, :355-375, :454-472, :552-572
+__device__ __forceinline__ void bfly_flipped1(const View& V, int i, int j, int gi, int gj, int t, int p1, int p2) {
+    const int a = BH + t;
+    const int par = (t == 1) ? (gi & 1) : (gj & 1);
+    if (bslice(t, gi, gj) != p1 || par == p2) return;
+    const int c = (V.g(a, i, j) & 256) / 256;
+    int t1, t4, t16, t64;
+    if (t != 2) {
+        const int B1 = (t == 0) ? V.g(a, i, j - 1) : V.g(a, i - 1, j);
+        const int B2 = (t == 0) ? V.g(a, i, j + 1) : V.g(a, i + 1, j);
+        t1 = ((B1 & 12) / 4 == 0) ? 0 : ((B1 & 12) / 4) % 3 + 1;
+        t4 = ((B2 & 3) == 0) ? 0 : ((B2 & 3) + 1) % 3 + 1;
+        t16 = ((B2 & 192) / 64 == 0) ? 0 : ((B2 & 192) / 64) % 3 + 1;
+        t64 = ((B1 & 48) / 16 == 0) ? 0 : ((B1 & 48) / 16 + 1) % 3 + 1;
+    } else {
+        const int B1 = V.g(a, i + 1, j - 1), B2 = V.g(a, i - 1, j + 1);
+        t1 = ((B2 & 192) / 64 == 0) ? 0 : ((B2 & 192) / 64 + 1) % 3 + 1;
+        t4 = ((B2 & 48) / 16 == 0) ? 0 : ((B2 & 48) / 16) % 3 + 1;
+        t16 = ((B1 & 12) / 4 == 0) ? 0 : ((B1 & 12) / 4 + 1) % 3 + 1;
+        t64 = ((B1 & 3) == 0) ? 0 : (B1 & 3) % 3 + 1;
+    }
+    V.s(a, i, j, pack(t1, t4, t16, t64, c));
+}
+
+// UpdateButterflysFlipped{H,L}21 (:377-398, :474-495), R21 (:574-595)
+__device__ __forceinline__ void bfly_flipped21(const View& V, int i, int j, int gi, int gj, int t, int p1) {
+    if (bslice(t, gi, gj) != ((t == 0) ? (p1 + 1) % 3 : (p1 + 2) % 3)) return;
+    const int a = BH + t, Bf = V.g(a, i, j);
+    if (t == 2) {
+        const int B2 = V.g(a, i, j + 1), B3 = V.g(a, i + 1, j);
+        const int c = sel((B3 & 3) == 1, 1);
+        const int t4 = ((Bf & 12) / 4 == 2) ? (Bf & 12) / 4 : (B2 & 3) / 3 + ((B2 & 192) / 192) * 3;
+        const int t16 = ((Bf & 48) / 16 == 1) ? (Bf & 48) / 16 : ((B3 & 3) / 3) * 3 + ((B3 & 192) / 192) * 2;
+        V.s(a, i, j, pack(Bf & 3, t4, t16, (Bf & 192) / 64, c));
+    } else {
+        const int B1 = (t == 0) ? V.g(a, i - 1, j) : V.g(a, i - 1, j + 1);
+        const int B2 = (t == 0) ? V.g(a, i - 1, j + 1) : V.g(a, i, j + 1);
+        const int c = sel((B1 & 48) / 16 == 2, 1);
+        const int t1 = ((Bf & 3) == 2) ? (Bf & 3) : (B1 & 192) / 192 + ((B1 & 48) / 48) * 3;
+        const int t4 = ((Bf & 12) / 4 == 1) ? (Bf & 12) / 4 : ((B2 & 192) / 192) * 3 + ((B2 & 48) / 48) * 2;
+        V.s(a, i, j, pack(t1, t4, (Bf & 48) / 16, (Bf & 192) / 64, c));
+    }
+}
+
+// UpdateButterflysFlipped{H,L,R}22, :400-428, :497-527, :597-627
+__device__ __forceinline__ void bfly_flipped22(const View& V, int i, int j, int gi, int gj, int t, int p1) {
+    if (bslice(t, gi, gj) != ((t == 0) ? (p1 + 2) % 3 : (p1 + 1) % 3)) return;
+    const int a = BH + t, Bf = V.g(a, i, j);
+    if (t == 2) {
+        const int s4 = sel((Bf & 12) / 4 != 2, 1), s16 = sel((Bf & 48) / 16 != 1, 1);
+        const int B1 = V.g(a, i - 1, j), B2 = V.g(a, i, j + 1), B3 = V.g(a, i + 1, j), B4 = V.g(a, i, j - 1);
+        const int c = sel((B1 & 48) / 16 == 1, 1);
+        const int t1 = ((Bf & 3) == 1) ? (Bf & 3) : ((B1 & 48) / 48) * 3 + ((B1 & 12) / 12) * 2;
+        const int t4 = ((B2 & 256) / 256 == 1) ? 2 : s4 * (Bf & 12) / 4;
+        const int t16 = ((B3 & 256) / 256 == 1) ? 1 : s16 * (Bf & 48) / 16;
+        const int t64 = ((Bf & 192) / 64 == 2) ? (Bf & 192) / 64 : (B4 & 48) / 48 + ((B4 & 12) / 12) * 3;
+        V.s(a, i, j, pack(t1, t4, t16, t64, c));
+    } else {
+        const int s1 = sel((Bf & 3) != 2, 1), s4 = sel((Bf & 12) / 4 != 1, 1);
+        const int B1 = (t == 0) ? V.g(a, i - 1, j) : V.g(a, i - 1, j + 1);
+        const int B2 = (t == 0) ? V.g(a, i - 1, j + 1) : V.g(a, i, j + 1);
+        const int B3 = (t == 0) ? V.g(a, i + 1, j) : V.g(a, i + 1, j - 1);
+        const int B4 = (t == 0) ? V.g(a, i + 1, j - 1) : V.g(a, i, j - 1);
+        const int c = sel((B3 & 3) == 2, 1);
+        const int t1 = ((B1 & 256) == 256) ? 2 : s1 * (Bf & 3);
+        const int t4 = ((B2 & 256) == 256) ? 1 : s4 * ((Bf & 12) / 4);
+        const int t16 = ((Bf & 48) / 16 == 2) ? (Bf & 48) / 16 : ((B3 & 3) / 3) * 3 + (B3 & 12) / 12;
+        const int t64 = ((Bf & 192) / 64 == 1) ? (Bf & 192) / 64 : ((B4 & 3) / 3) * 2 + ((B4 & 12) / 12) * 3;
+        V.s(a, i, j, pack(t1, t4, t16, t64, c));
+    }
+}
+
+// UpdateButterflysFlipped{H,L,R}23, :430-452, :529-550, :629-651
+__device__ __forceinline__ void bfly_flipped23(const View& V, int i, int j, int gi, int gj, int t, int p1) {
+    if (bslice(t, gi, gj) != ((t == 0) ? (p1 + 1) % 3 : (p1 + 2) % 3)) return;
+    const int a = BH + t, Bf = V.g(a, i, j);
+    const int c = (Bf & 256) / 256;
+    if (t == 2) {
+        const int s1 = sel((Bf & 3) != 1, 1), s64 = sel((Bf & 192) / 64 != 2, 1);
+        const int B1 = V.g(a, i - 1, j), B4 = V.g(a, i, j - 1);
+        const int t1 = ((B1 & 256) / 256 == 1) ? 1 : s1 * (Bf & 3);
+        const int t64 = ((B4 & 256) / 256 == 1) ? 2 : s64 * (Bf & 192) / 64;
+        V.s(a, i, j, pack(t1, (Bf & 12) / 4, (Bf & 48) / 16, t64, c));
+    } else {
+        const int s16 = sel((Bf & 48) / 16 != 2, 1), s64 = sel((Bf & 192) / 64 != 1, 1);
+        const int B3 = (t == 0) ? V.g(a, i + 1, j) : V.g(a, i + 1, j - 1);
+        const int B4 = (t == 0) ? V.g(a, i + 1, j - 1) : V.g(a, i, j - 1);
+        const int t16 = ((B3 & 256) == 256) ? 2 : s16 * ((Bf & 48) / 16);
+        const int t64 = ((B4 & 256) == 256) ? 1 : s64 * ((Bf & 192) / 64);
+        V.s(a, i, j, pack(Bf & 3, (Bf & 12) / 4, t16, t64, c));
+    }
+}
+
+// UpdateLozengeFromButterflys{H,L,R}, :654-738
+__device__ __forceinline__ void loz_from_bfly(const View& V, int i, int j, int t) {
+    const int F = V.g(BH + t, i, j);
+    const int full = sel((F & 3) == 3, 1) + sel((F & 12) == 12, 2) + sel((F & 48) == 48, 4) + sel((F & 192) == 192, 8);
+    const int c = (F & 256) == 256;
+    if (t == 0) {
+        V.s(EH, i, j, full);
+        V.s(EL, i, j, sel(c, 1) + sel((F & 48) == 48, 2) + sel((F & 192) == 64, 4) + sel((F & 192) == 128, 8));
+        V.s(ER, i, j, sel(c, 1) + sel((F & 48) == 16, 2) + sel((F & 48) == 32, 4) + sel((F & 192) == 192, 8));
+    } else if (t == 1) {
+        V.s(EL, i, j, full);
+        V.s(ER, i, j, sel((F & 3) == 3, 1) + sel((F & 12) == 4, 2) + sel((F & 12) == 8, 4) + sel(c, 8));
+        V.s(EH, i, j, sel((F & 3) == 1, 1) + sel((F & 3) == 2, 2) + sel((F & 12) == 12, 4) + sel(c, 8));
+    } else {
+        V.s(ER, i, j, full);
+        V.s(EH, i, j, sel((F & 3) == 1, 1) + sel((F & 3) == 2, 2) + sel(c, 4) + sel((F & 192) == 192, 8));
+        V.s(EL, i, j, sel((F & 3) == 3, 1) + sel(c, 2) + sel((F & 192) == 64, 4) + sel((F & 192) == 128, 8));
+    }
+}
+
+// ---- phase driver ----------------------------------------------------------------------
+
+// Sites of a region a phase may compute: at least `m` away from every region edge that cuts
+// the domain, and inside the reference's update range 0 < gi, gj < M - 1.
+struct Region {
+    int gi0, gj0, rows, cols, M;
+    bool cutT, cutB, cutL, cutR;
+};
+
+template <int NT, class F>
+__device__ __forceinline__ void for_sites(const Region& R, int m, F f) {
+    const int i0 = R.cutT ? m : 0, i1 = R.rows - (R.cutB ? m : 0);
+    const int j0 = R.cutL ? m : 0, j1 = R.cols - (R.cutR ? m : 0);
+    const int w = j1 - j0, n = (i1 - i0) * w;
+    for (int idx = threadIdx.x; idx < n; idx += NT) {
+        const int i = i0 + idx / w, j = j0 + idx % w;
+        const int gi = R.gi0 + i, gj = R.gj0 + j;
+        if (gi > 0 && gj > 0 && gi < R.M - 1 && gj < R.M - 1) f(i, j, gi, gj);
+    }
+}
+
+// The update kernels of one step after its Rotate (TriangleDimerTiler.cpp:217-338).
+template <int NT>
+__device__ void replay_updates(const View& V, const Region& R, const Move& mv) {
+    if (mv.type == 0) {
+        const int t = mv.r1, c = mv.r2;
+        for_sites<NT>(R, 1, [&](int i, int j, int gi, int gj) { loz_flipped(V, i, j, gi, gj, t, c); });
+        __syncthreads();
+        for_sites<NT>(R, 2, [&](int i, int j, int, int) { loz_update(V, i, j, t); });
+        __syncthreads();
+        for_sites<NT>(R, 3, [&](int i, int j, int, int) {
+            tri_u_from_loz(V, i, j);
+            bfly_from_loz(V, i, j, 0);
+            bfly_from_loz(V, i, j, 1);
+            bfly_from_loz(V, i, j, 2);
+        });
+        __syncthreads();
+        for_sites<NT>(R, 4, [&](int i, int j, int, int) { tri_update(V, i, j, 0); });
+    } else if (mv.type == 1) {
+        const int c = mv.r1;
+        const bool down = mv.r2 == 1;
+        for_sites<NT>(R, 1, [&](int i, int j, int gi, int gj) { tri_flipped(V, i, j, gi, gj, c, down); });
+        __syncthreads();
+        for_sites<NT>(R, 2, [&](int i, int j, int, int) { tri_update(V, i, j, down ? 1 : 0); });
+        __syncthreads();
+        for_sites<NT>(R, 3, [&](int i, int j, int, int) { loz_from_tri(V, i, j); });
+        __syncthreads();
+        for_sites<NT>(R, 4, [&](int i, int j, int, int) {
+            bfly_from_loz(V, i, j, 0);
+            bfly_from_loz(V, i, j, 1);
+            bfly_from_loz(V, i, j, 2);
+        });
+    } else {
+        const int t = mv.r1, p1 = mv.p1, p2 = mv.p2;
+        for_sites<NT>(R, 1, [&](int i, int j, int gi, int gj) { bfly_flipped1(V, i, j, gi, gj, t, p1, p2); });
+        __syncthreads();
+        for_sites<NT>(R, 2, [&](int i, int j, int gi, int gj) { bfly_flipped21(V, i, j, gi, gj, t, p1); });
+        __syncthreads();
+        for_sites<NT>(R, 3, [&](int i, int j, int gi, int gj) { bfly_flipped22(V, i, j, gi, gj, t, p1); });
+        __syncthreads();
+        // a site's lozenge states depend only on its own (just final) butterfly state
+        for_sites<NT>(R, 4, [&](int i, int j, int gi, int gj) {
+            bfly_flipped23(V, i, j, gi, gj, t, p1);
+            loz_from_bfly(V, i, j, t);
+        });
+        __syncthreads();
+        for_sites<NT>(R, 5, [&](int i, int j, int, int) {
+            if (t != 0) bfly_from_loz(V, i, j, 0);
+            if (t != 1) bfly_from_loz(V, i, j, 1);
+            if (t != 2) bfly_from_loz(V, i, j, 2);
+            tri_u_from_loz(V, i, j);
+        });
+        __syncthreads();
+        for_sites<NT>(R, 6, [&](int i, int j, int, int) { tri_update(V, i, j, 0); });
+    }
+}
+
+__device__ __forceinline__ void unpack_cell(const uint4 c, int v[8]) {
+    v[0] = c.x & 0xffff, v[1] = c.x >> 16, v[2] = c.y & 0xffff, v[3] = c.y >> 16;
+    v[4] = c.z & 0xffff, v[5] = c.z >> 16, v[6] = c.w & 0xffff, v[7] = c.w >> 16;
+}
+
+template <bool UNI>
+__device__ __forceinline__ void load_params(const TdArgs& a, size_t id, uint32_t& m1, uint32_t& m2, uint32_t& tm) {
+    if (UNI) {
+        m1 = a.m1, m2 = a.m2, tm = a.tm;
+    } else {
+        const uint4 p = a.params[id];
+        m1 = p.x, m2 = p.y, tm = p.z;
+    }
+}
+
+// ---- one step per launch, tiles + halo --------------------------------------------------
+template <int TR, int TC, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_td_step(TdArgs a, uint32_t move) {
+    constexpr int RR = TR + 2 * kHalo, RC = TC + 2 * kHalo;
+    __shared__ uint16_t lds[8 * RR * RC];
+    const int M = a.M;
+    const int tiles_c = (M + TC - 1) / TC;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int ti = tile / tiles_c, tj = tile % tiles_c;
+    const int oi = ti * TR, oj = tj * TC;  // owned tile origin
+    Region R;
+    R.M = M;
+    R.gi0 = max(0, oi - kHalo);
+    R.gj0 = max(0, oj - kHalo);
+    const int gi1 = min(M, oi + TR + kHalo), gj1 = min(M, oj + TC + kHalo);
+    R.rows = gi1 - R.gi0;
+    R.cols = gj1 - R.gj0;
+    R.cutT = R.gi0 > 0;
+    R.cutL = R.gj0 > 0;
+    R.cutB = gi1 < M;
+    R.cutR = gj1 < M;
+    const View V{lds, RC, RR * RC};
+    const Move mv = decode(move);
+    const int rp = rotate_plane(mv);
+    const int n = R.rows * R.cols;
+    const int oi1 = min(M, oi + TR), oj1 = min(M, oj + TC);
+    // load the region, draw every site's number, rotate
+    for (int idx = threadIdx.x; idx < n; idx += NT) {
+        const int i = idx / R.cols, j = idx % R.cols;
+        const int gi = R.gi0 + i, gj = R.gj0 + j;
+        const size_t id = (size_t)gi * M + gj;
+        int v[8];
+        unpack_cell(a.cells_in[id], v);
+        uint4 g = a.gen_in[id];
+        uint32_t m1, m2, tm;
+        load_params<UNI>(a, id, m1, m2, tm);
+        const uint32_t u = tm_draw_raw(g.x, g.y, g.z, g.w, m1, m2, tm);
+        if (gi >= oi && gi < oi1 && gj >= oj && gj < oj1) a.gen_out[id] = g;
+        v[rp] = rotate_value(mv, v[rp], gi, gj, u < 0x80000000u);
+#pragma unroll
+        for (int p = 0; p < 8; p++) V.s(p, i, j, v[p]);
+    }
+    __syncthreads();
+    replay_updates<NT>(V, R, mv);
+    __syncthreads();
+    // write the owned tile
+    const int ow = oj1 - oj, on = (oi1 - oi) * ow;
+    for (int idx = threadIdx.x; idx < on; idx += NT) {
+        const int gi = oi + idx / ow, gj = oj + idx % ow;
+        const int i = gi - R.gi0, j = gj - R.gj0;
+        uint4 c;
+        c.x = V.g(0, i, j) | (V.g(1, i, j) << 16);
+        c.y = V.g(2, i, j) | (V.g(3, i, j) << 16);
+        c.z = V.g(4, i, j) | (V.g(5, i, j) << 16);
+        c.w = V.g(6, i, j) | (V.g(7, i, j) << 16);
+        a.cells_out[(size_t)gi * M + gj] = c;
+    }
+}
+
+// ---- whole domain in one workgroup, every step in one launch ----------------------------
+constexpr int kSmallNT = 1024;
+constexpr int kSmallPer = (kSmallMax * kSmallMax + kSmallNT - 1) / kSmallNT;  // sites per thread
+
+template <bool UNI>
+__global__ __launch_bounds__(kSmallNT) void k_td_small(TdArgs a) {
+    __shared__ uint16_t lds[8 * kSmallMax * kSmallMax];
+    const int M = a.M, n = M * M;
+    const View V{lds, M, M * M};
+    Region R{0, 0, M, M, M, false, false, false, false};
+    uint4 g[kSmallPer];
+    uint32_t pm1[kSmallPer], pm2[kSmallPer], ptm[kSmallPer];
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int id = threadIdx.x + k * kSmallNT;
+        if (id < n) {
+            g[k] = a.gen_in[id];
+            load_params<UNI>(a, id, pm1[k], pm2[k], ptm[k]);
+            int v[8];
+            unpack_cell(a.cells_in[id], v);
+#pragma unroll
+            for (int p = 0; p < 8; p++) V.s(p, id / M, id % M, v[p]);
+        }
+    }
+    __syncthreads();
+    for (int64_t s = 0; s < a.nsteps; s++) {
+        const Move mv = decode(a.sched[s]);
+        const int rp = rotate_plane(mv);
+#pragma unroll
+        for (int k = 0; k < kSmallPer; k++) {
+            const int id = threadIdx.x + k * kSmallNT;
+            if (id < n) {
+                const uint32_t u = tm_draw_raw(g[k].x, g[k].y, g[k].z, g[k].w, pm1[k], pm2[k], ptm[k]);
+                const int gi = id / M, gj = id % M;
+                const int v = V.g(rp, gi, gj);
+                const int w = rotate_value(mv, v, gi, gj, u < 0x80000000u);
+                if (w != v) V.s(rp, gi, gj, w);
+            }
+        }
+        __syncthreads();
+        replay_updates<kSmallNT>(V, R, mv);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int id = threadIdx.x + k * kSmallNT;
+        if (id < n) {
+            a.gen_out[id] = g[k];
+            const int i = id / M, j = id % M;
+            uint4 c;
+            c.x = V.g(0, i, j) | (V.g(1, i, j) << 16);
+            c.y = V.g(2, i, j) | (V.g(3, i, j) << 16);
+            c.z = V.g(4, i, j) | (V.g(5, i, j) << 16);
+            c.w = V.g(6, i, j) | (V.g(7, i, j) << 16);
+            a.cells_out[id] = c;
+        }
+    }
+}
+
+// ---- derive the move states from a tiling, and merge them back --------------------------
+
+// TilingToTriangleFlips (TriangleDimerTiler.cpp:600-631) at (i, j) of the N/2 x N array
+__device__ int tri_state(const int32_t* t, int N, int i, int j) {
+    if (i < 1 || i >= N / 2 - 1 || j < 1 || j >= N - 1) return 0;
+    const int a = t[(2 * i + 1) * N + j], b = t[(2 * i + 1) * N + j + 1];
+    if (j % 2 == 0) {
+        const int c = t[2 * i * N + j + 1];
+        return ((a & 4) / 4 + (a & 8) / 4 + ((b & 8) / 8) * 3) + 4 * ((b & 4) / 4 + (a & 2) + ((b & 2) / 2) * 3) +
+               16 * ((c & 1) * 2 + (a & 1) + ((c & 2) / 2) * 3);
+    }
+    const int d = t[(2 * i + 2) * N + j];
+    return ((a & 1) + (a & 8) / 4 + ((b & 8) / 8) * 3) + 4 * ((b & 1) + (a & 2) + ((b & 2) / 2) * 3) +
+           16 * ((d & 8) / 4 + ((d & 4) / 4) * 3 + (a & 4) / 4);
+}
+
+// TilingToButterflyFlips (TriangleDimerTiler.cpp:633-680) at (i, j) of the N x N array
+__device__ int bfly_state(const int32_t* t, int N, int i, int j) {
+    if (i < 2 || i >= N - 2 || j < 2 || j >= N - 2) return 0;
+    const int x = i * N + j, s = t[x];
+    int f1, f4, f16, f64, c;
+    if (i % 2 == 0) {
+        if (j % 2 == 0) return 0;
+        const int u = t[x - N], ur = t[x - N + 1], d = t[x + N], dl = t[x + N - 1];
+        f1 = (u & 8) / 8 + (u & 1) * 2 + (s & 1) * 3;
+        f4 = (ur & 1) + (ur & 2) + ((s & 2) / 2) * 3;
+        f16 = (d & 2) / 2 + (d & 4) / 2 + ((s & 4) / 4) * 3;
+        f64 = (dl & 4) / 4 + (dl & 8) / 4 + ((s & 8) / 8) * 3;
+        c = (ur & 4) / 4;
+    } else if (j % 2 == 0) {
+        const int ur = t[x - N + 1], r = t[x + 1], dl = t[x + N - 1], l = t[x - 1];
+        f1 = (ur & 1) + (ur & 2) + (s & 1) * 3;
+        f4 = (r & 2) / 2 + (r & 4) / 2 + ((s & 2) / 2) * 3;
+        f16 = (dl & 4) / 4 + (dl & 8) / 4 + ((s & 4) / 4) * 3;
+        f64 = (l & 8) / 8 + (l & 1) * 2 + ((s & 8) / 8) * 3;
+        c = (r & 8) / 8;
+    } else {
+        const int u = t[x - N], r = t[x + 1], d = t[x + N], l = t[x - 1];
+        f1 = (u & 1) + (u & 2) + (s & 1) * 3;
+        f4 = (r & 1) + (r & 2) + ((s & 2) / 2) * 3;
+        f16 = (d & 4) / 4 + (d & 8) / 4 + ((s & 4) / 4) * 3;
+        f64 = (l & 4) / 4 + (l & 8) / 4 + ((s & 8) / 8) * 3;
+        c = (r & 8) / 8;
+    }
+    return f1 + 4 * f4 + 16 * f16 + 64 * f64 + 256 * c;
+}
+
+// Walk's host split (TriangleDimerTiler.cpp:104-129): site (a, b) takes eH from t[2a][2b+1],
+// eL from t[2a+1][2b], eR from t[2a+1][2b+1], tD/tU from triangle states (a, 2b)/(a, 2b+1),
+// bH/bL/bR from butterfly states at the eH/eL/eR positions.
+__global__ void k_td_derive(const int32_t* t, int N, uint4* cells) {
+    const int M = N / 2;
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= M * M) return;
+    const int a = id / M, b = id % M;
+    const int r0 = 2 * a, r1 = 2 * a + 1, c0 = 2 * b, c1 = 2 * b + 1;
+    const int eh = t[r0 * N + c1], el = t[r1 * N + c0], er = t[r1 * N + c1];
+    const int tu = tri_state(t, N, a, c1), tdn = tri_state(t, N, a, c0);
+    const int bh = bfly_state(t, N, r0, c1), bl = bfly_state(t, N, r1, c0), br = bfly_state(t, N, r1, c1);
+    cells[id] = make_uint4(eh | (el << 16), er | (tu << 16), tdn | (bh << 16), bl | (br << 16));
+}
+
+// Walk's merge (TriangleDimerTiler.cpp:437-451): only the lozenge-move states go back.
+__global__ void k_td_merge(const uint4* cells, int N, int32_t* t) {
+    const int M = N / 2;
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= M * M) return;
+    const int a = id / M, b = id % M;
+    const uint4 c = cells[id];
+    t[(2 * a) * N + 2 * b + 1] = c.x & 0xffff;
+    t[(2 * a + 1) * N + 2 * b] = c.x >> 16;
+    t[(2 * a + 1) * N + 2 * b + 1] = c.y & 0xffff;
+}
+
+// Tile configurations of the one-step kernel: {rows, cols, threads}.
+struct TdCfg {
+    int tr, tc, nt;
+};
+constexpr TdCfg kCfgs[] = {{32, 32, 256}, {64, 64, 1024}, {32, 64, 512}};
+constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
+
+template <int I, bool UNI>
+hipError_t launch_cfg(const TdArgs& a, uint32_t move, hipStream_t st) {
+    constexpr TdCfg c = kCfgs[I];
+    const int tiles = ((a.M + c.tr - 1) / c.tr) * ((a.M + c.tc - 1) / c.tc);
+    hipLaunchKernelGGL((k_td_step<c.tr, c.tc, c.nt, UNI>), dim3(tiles), dim3(c.nt), 0, st, a, move);
+    return hipGetLastError();
+}
+
+}  // namespace td
+
+int td_config_count() { return td::kNumCfgs; }
+
+bool td_config(int idx, int* tr, int* tc, int* nt) {
+    if (idx < 0 || idx >= td::kNumCfgs) return false;
+    *tr = td::kCfgs[idx].tr;
+    *tc = td::kCfgs[idx].tc;
+    *nt = td::kCfgs[idx].nt;
+    return true;
+}
+
+hipError_t td_launch_derive(const int32_t* t, int N, uint4* cells, hipStream_t st) {
+    const int n = (N / 2) * (N / 2);
+    hipLaunchKernelGGL(td::k_td_derive, dim3((n + 255) / 256), dim3(256), 0, st, t, N, cells);
+    return hipGetLastError();
+}
+
+hipError_t td_launch_merge(const uint4* cells, int N, int32_t* t, hipStream_t st) {
+    const int n = (N / 2) * (N / 2);
+    hipLaunchKernelGGL(td::k_td_merge, dim3((n + 255) / 256), dim3(256), 0, st, cells, N, t);
+    return hipGetLastError();
+}
+
+hipError_t td_launch_small(const TdArgs& a, bool uni, hipStream_t st) {
+    if (a.M > kSmallMax) return hipErrorInvalidValue;
+    if (uni) hipLaunchKernelGGL(td::k_td_small<true>, dim3(1), dim3(td::kSmallNT), 0, st, a);
+    else hipLaunchKernelGGL(td::k_td_small<false>, dim3(1), dim3(td::kSmallNT), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t td_launch_step(int cfg, const TdArgs& a, bool uni, uint32_t move, hipStream_t st) {
+    switch (cfg) {
+        case 0: return uni ? td::launch_cfg<0, true>(a, move, st) : td::launch_cfg<0, false>(a, move, st);
+        case 1: return uni ? td::launch_cfg<1, true>(a, move, st) : td::launch_cfg<1, false>(a, move, st);
+        case 2: return uni ? td::launch_cfg<2, true>(a, move, st) : td::launch_cfg<2, false>(a, move, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace rt

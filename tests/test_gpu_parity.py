@@ -200,3 +200,68 @@ def test_autotune_leaves_chain_untouched(gctx, golden):
     ch.close()
     o = O.lozenge_walk(t0, 120, 9, default_table(BIG), rule=1, q=0.999)
     assert np.array_equal(g, o)
+
+
+# ---- triangle dimers ------------------------------------------------------------------------
+N_TD = rt.tile_config_count(rt.RT_TRIDIMER)
+TD_PATHS = [(rt.RT_PATH_AUTO, -1)] + [(rt.RT_PATH_STEPWISE, i) for i in range(N_TD)]
+
+
+@pytest.mark.parametrize("path,cfg", TD_PATHS)
+def test_tridimer_walk_parity(gctx, golden, path, cfg):
+    """Small cones (one workgroup holds the whole move lattice on the AUTO path; one tiled
+    launch per step on the STEPWISE path) against the oracle."""
+    from tests.tilings import tridimer_covered, tridimer_legal
+    for key, steps in [("cone2_0", 40), ("cone6_0", 300), ("cone9_2", 333), ("cone22_0", 517)]:
+        t0 = golden[f"tridimer_{key}"]
+        say("tridimer", path, cfg, key, steps)
+        gctx.set_path(path, cfg)
+        try:
+            g = gctx.tridimer_walk(t0, steps, 4242)
+        finally:
+            gctx.set_path(rt.RT_PATH_AUTO, -1)
+        o = O.tridimer_walk(t0, steps, 4242, default_table(BIG))
+        assert np.array_equal(g, o), key
+        assert tridimer_legal(g, tridimer_covered(t0))
+
+
+@pytest.mark.parametrize("cfg", range(N_TD))
+def test_tridimer_large_cone_tiled(gctx, cfg):
+    """A cone whose move lattice spans several tiles of every configuration (M = 154): the
+    halo replay must reproduce the reference's kernel sequence across tile borders."""
+    from randomtilings_amd import statics as S
+    t0 = S.call("tridimer.IceCreamCone", args=[150, 0])
+    say("tridimer large", cfg)
+    gctx.set_path(rt.RT_PATH_STEPWISE, cfg)
+    try:
+        g = gctx.tridimer_walk(t0, 120, 99)
+    finally:
+        gctx.set_path(rt.RT_PATH_AUTO, -1)
+    o = O.tridimer_walk(t0, 120, 99, default_table(BIG))
+    assert np.array_equal(g, o)
+
+
+def test_tridimer_scrambled_params(sctx, golden):
+    t0 = golden["tridimer_cone14_0"]
+    for path, cfg in [(rt.RT_PATH_AUTO, -1), (rt.RT_PATH_STEPWISE, 0)]:
+        say("tridimer scrambled", path, cfg)
+        sctx.set_path(path, cfg)
+        try:
+            g = sctx.tridimer_walk(t0, 250, 8)
+        finally:
+            sctx.set_path(rt.RT_PATH_AUTO, -1)
+        assert np.array_equal(g, O.tridimer_walk(t0, 250, 8, scrambled_table(BIG, 11)))
+
+
+def test_tridimer_chain_runs_compose(gctx, golden):
+    t0 = golden["tridimer_cone10_0"]
+    for path in [rt.RT_PATH_AUTO, rt.RT_PATH_STEPWISE]:
+        say("tridimer compose", path)
+        ch = gctx.chain(rt.RT_TRIDIMER, 0, t0)
+        ch.set_path(path, -1)
+        ch.seed(31)
+        for s in [1, 9, 150, 40]:
+            ch.run(s)
+        g = ch.read()
+        ch.close()
+        assert np.array_equal(g, O.tridimer_walk(t0, 200, 31, default_table(BIG)))

@@ -101,3 +101,82 @@ def test_oracle_per_generator_params_change_the_walk(golden):
     a = O.domino_walk(t0, 500, 3, default_table(t0.size), rule=1)
     b = O.domino_walk(t0, 500, 3, scrambled_table(t0.size, 1), rule=1)
     assert not np.array_equal(a, b)
+
+
+def _matchings(cov):
+    """Number of perfect matchings of the covered vertices (brute force, tiny regions)."""
+    V = {tuple(v) for v in np.argwhere(cov)}
+    nb = {v: [w for w in [(v[0], v[1] + 1), (v[0] + 1, v[1]), (v[0] + 1, v[1] - 1),
+                          (v[0], v[1] - 1), (v[0] - 1, v[1]), (v[0] - 1, v[1] + 1)] if w in V] for v in V}
+
+    def count(left):
+        if not left:
+            return 1
+        v = min(left)
+        return sum(count(left - {v, w}) for w in nb[v] if w in left)
+
+    return count(frozenset(V))
+
+
+def test_tridimer_walk_keeps_a_dimer_cover(golden):
+    """Every step keeps a perfect matching of the same vertices (TriangleDimerTiler.cpp:86-483
+    with all three move types), and all three move types occur."""
+    from tests.tilings import tridimer_covered, tridimer_legal
+    for key in ["cone6_0", "cone9_2", "cone10_0"]:
+        t0 = golden[f"tridimer_{key}"]
+        cov = tridimer_covered(t0)
+        t = t0
+        for seed in range(3):
+            t, mv = O.tridimer_walk(t, 150, 77 + seed, default_table(t0.size), return_moves=True)
+            assert tridimer_legal(t, cov), (key, seed)
+            assert set(np.unique(mv & 3)) == {0, 1, 2}
+        assert not np.array_equal(t, t0)
+
+
+def test_tridimer_move_schedule_draw_order():
+    """Walk draws type, r1, r2 (+ p1, p2 for butterflies) from one minstd_rand0 through
+    uniform_int_distribution(0,2)/(0,1) (TriangleDimerTiler.cpp:211-222, :291-292)."""
+    t0 = np.zeros(16 * 16, np.int32)  # an empty region: the schedule alone is observable
+    _, mv = O.tridimer_walk(t0, 200, 12345, default_table(64), return_moves=True)
+    # minstd_rand0 (x <- 16807 x mod 2^31-1) restated independently of libstdc++
+    x = 12345 % 2147483647
+
+    def nxt():
+        nonlocal x
+        x = (x * 16807) % 2147483647
+        return x
+
+    def uni(hi):  # libstdc++ downscaling for a range of hi+1 values over minstd (max - min = 2^31 - 3)
+        urange = hi + 1
+        scaling = (2147483646 - 1) // urange
+        past = urange * scaling
+        while True:
+            r = nxt() - 1
+            if r < past:
+                return r // scaling
+    exp = []
+    for _ in range(200):
+        ty, r1, r2 = uni(2), uni(2), uni(1)
+        p1 = p2 = 0
+        if ty == 2:
+            p1, p2 = uni(2), uni(1)
+        exp.append(ty | r1 << 2 | r2 << 4 | p1 << 5 | p2 << 7)
+    assert mv.tolist() == exp
+
+
+def test_tridimer_uniform_law(golden):
+    """The cone of side 3 has exactly 6 dimer covers (brute force); the chain's moves are
+    symmetric involutions accepted with probability 1/2, so its law is uniform over them."""
+    t0 = golden["tridimer_cone3_0"]
+    from tests.tilings import tridimer_covered
+    assert _matchings(tridimer_covered(t0)) == 6
+    params = default_table(t0.size)
+    counts = {}
+    S = 1800
+    for s in range(S):
+        t = O.tridimer_walk(t0, 800, 7919 * s + 1, params)
+        counts[t.tobytes()] = counts.get(t.tobytes(), 0) + 1
+    assert len(counts) == 6
+    exp = S / 6
+    chi2 = sum((c - exp) ** 2 / exp for c in counts.values())
+    assert chi2 < 20.5  # chi-square, 5 dof, p = 0.001

@@ -101,3 +101,81 @@ def lozenge_legal(t, d=None) -> bool:
 def lozenge_cubes(t) -> int:
     """Count of vertices in the 'cube' state 38 (qlozengekernel.cl:25)."""
     return int(np.count_nonzero(_sq(t) == 38))
+
+
+# ---- Triangle dimers ---------------------------------------------------------------------
+# TriangleDimerTiler.cpp:488-540 (IceCreamCone) encodes the dimer indicators h (horizontal),
+# l (up-left) and r (up-right) of the triangular lattice into lozenge-move states:
+#   t[2a][2b+1]   = r[a-1][b] + 2 l[a-1][b+1] + 4 r[a][b] + 8 l[a][b]
+#   t[2a+1][2b]   = h[a][b] + 2 r[a][b] + 4 h[a+1][b-1] + 8 r[a][b-1]
+#   t[2a+1][2b+1] = h[a][b] + 2 l[a][b+1] + 4 h[a+1][b] + 8 l[a][b]
+# so every lattice edge appears in four states.  Vertex (a,b) meets h[a][b], h[a][b-1],
+# l[a][b], l[a-1][b], r[a-1][b] and r[a][b-1] (the six edges TilingToVertices,
+# TriangleDimerTiler.cpp:711-735, inspects).
+
+def tridimer_edges(t):
+    """Decode (h, l, r) from a tiling and check that all four copies of every edge agree.
+    Returns (h, l, r, consistent)."""
+    t = _sq(t).astype(np.int64)
+    N = t.shape[0]
+    E = N // 2
+    h = np.zeros((E + 1, E + 1), np.int64)
+    l = np.zeros_like(h)
+    r = np.zeros_like(h)
+    seen = {}
+
+    def put(arr, name, a, b, v):
+        if not (0 <= a <= E and 0 <= b <= E):
+            return v == 0
+        key = (name, a, b)
+        if key in seen:
+            return seen[key] == v
+        seen[key] = v
+        arr[a, b] = v
+        return True
+
+    ok = True
+    for i in range(N):
+        for j in range(N):
+            s = int(t[i, j])
+            a, b = i // 2, j // 2
+            bits = [(s >> k) & 1 for k in range(4)]
+            if i % 2 == 0 and j % 2 == 1:
+                cells = [(r, "r", a - 1, b), (l, "l", a - 1, b + 1), (r, "r", a, b), (l, "l", a, b)]
+            elif i % 2 == 1 and j % 2 == 0:
+                cells = [(h, "h", a, b), (r, "r", a, b), (h, "h", a + 1, b - 1), (r, "r", a, b - 1)]
+            elif i % 2 == 1:
+                cells = [(h, "h", a, b), (l, "l", a, b + 1), (h, "h", a + 1, b), (l, "l", a, b)]
+            else:
+                ok &= s == 0
+                continue
+            for (arr, name, x, y), v in zip(cells, bits):
+                ok &= put(arr, name, x, y, v)
+    return h, l, r, bool(ok)
+
+
+def tridimer_vertex_degree(h, l, r):
+    """Number of dimers at every vertex."""
+    E = h.shape[0]
+    z = np.zeros((1, E), h.dtype)
+    zc = np.zeros((E, 1), h.dtype)
+    up = lambda a: np.vstack([z, a[:-1]])          # a[i-1][j]
+    left = lambda a: np.hstack([zc, a[:, :-1]])    # a[i][j-1]
+    return h + left(h) + l + up(l) + up(r) + left(r)
+
+
+def tridimer_legal(t, covered=None) -> bool:
+    """Consistent encoding, a matching (no vertex meets two dimers) and, when `covered` (the
+    start tiling's covered vertices) is given, the same set of covered vertices."""
+    h, l, r, ok = tridimer_edges(t)
+    if not ok:
+        return False
+    deg = tridimer_vertex_degree(h, l, r)
+    if deg.max() > 1:
+        return False
+    return covered is None or np.array_equal(deg == 1, covered)
+
+
+def tridimer_covered(t):
+    h, l, r, _ = tridimer_edges(t)
+    return tridimer_vertex_degree(h, l, r) == 1
