@@ -33,6 +33,7 @@ extern "C" {
 #define RT_ERR_IO (-4)      /* parameter file missing or malformed */
 #define RT_ERR_DEVICE (-5)  /* device-side error flag (e.g. a bounded grid-barrier wait expired) */
 #define RT_ERR_UNTILEABLE (-6) /* domain has no tiling (max/min construction failed) */
+#define RT_ERR_NOT_COALESCED (-7) /* coupling from the past did not coalesce within the bound */
 
 /* models */
 #define RT_DOMINO 0
@@ -55,10 +56,19 @@ extern "C" {
                            reference's Walk(t, steps, seed) derives it per model            */
 #define RT_SEED_EPOCH 1 /* epoch of Walk(t, vector steps, vector seeds): DominoTiler.cpp:79-82 */
 
+/* coupling-from-the-past epoch schedules (rt_domino_cftp) */
+#define RT_CFTP_REFERENCE 0 /* Examples/AztecDiamondCftpEx.cpp:42-68: start with one epoch of
+                               first_steps; each restart prepends an epoch of twice the last
+                               epoch's length (so 2*first_steps each time)                   */
+#define RT_CFTP_DOUBLING 1  /* each restart prepends an epoch as long as all current epochs
+                               together, doubling the total (Propp-Wilson)                   */
+
 /* execution paths */
 #define RT_PATH_AUTO 0       /* persistent temporal-blocked kernel where available        */
 #define RT_PATH_PERSISTENT 1 /* persistent LDS-tiled kernel, generators resident in VGPRs */
 #define RT_PATH_STEPWISE 2   /* one Rotate + Update launch pair per step (reference shape) */
+#define RT_PATH_BLOCKED 3    /* LDS-tiled kernel, one launch per block of h steps: no
+                                co-residency needed (large grids, shared devices)        */
 
 typedef struct rt_ctx rt_ctx;
 typedef struct rt_chain rt_chain;
@@ -142,6 +152,18 @@ int rt_recttri_walk(rt_ctx* ctx, const float* weights16, int32_t* tiling, int32_
  * lattice N/2 <= 64), RT_PATH_STEPWISE = one tiled launch per step (tile_config selects the
  * tile), RT_PATH_AUTO = the former when it fits. */
 int rt_tridimer_walk(rt_ctx* ctx, int32_t* tiling, int32_t N, int64_t steps, int64_t seed);
+
+/* Coupling from the past for the monotone domino rules (RT_DOMINO_CFTP, RT_DOMINO_WEIGHTED),
+ * as Examples/AztecDiamondCftpEx.cpp:42-68 runs it over DominoTiler::Walk(t, steps, seeds):
+ * epoch seeds come from std::mt19937(seed); every restart replays all epochs, oldest first,
+ * on copies of tmax and tmin (the maximal and minimal tilings of the domain) and stops when
+ * the two agree everywhere.  Both chains run as one coupled device chain (one random number
+ * per site and step drives both) and the comparison is a device reduction.  On success
+ * `out` holds the sample, *total_steps the steps of the final replay and *restarts the number
+ * of restarts (either may be NULL).  RT_ERR_NOT_COALESCED after max_restarts restarts. */
+int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* tmax, const int32_t* tmin,
+                   int32_t N, int64_t seed, int mode, int64_t first_steps, int32_t max_restarts, int32_t* out,
+                   int64_t* total_steps, int32_t* restarts);
 
 /* ---- device-resident chains (benchmarks, CFTP, batching) --------------------------- */
 

@@ -53,6 +53,9 @@ def lib() -> C.CDLL:
         L.or_tridimer_walk.argtypes = [_I32P, C.c_int, C.c_long, C.c_long, _U32P, C.c_int64, C.POINTER(C.c_uint8)]
         L.or_tridimer_walk.restype = C.c_int
         L.or_tridimer_flips.argtypes = [_I32P, C.c_int, _I32P, _I32P]
+        L.or_domino_cftp.argtypes = [C.c_int, _DP, _I32P, _I32P, C.c_int, C.c_long, C.c_int, C.c_long, C.c_int,
+                                     _U32P, C.c_int64, _I32P, _LP, C.POINTER(C.c_int)]
+        L.or_domino_cftp.restype = C.c_int
         _or = L
     return _or
 
@@ -210,3 +213,23 @@ def tridimer_flips(t):
     bf = np.zeros(N * N, np.int32)
     lib().or_tridimer_flips(t.ctypes.data_as(_I32P), N, tri.ctypes.data_as(_I32P), bf.ctypes.data_as(_I32P))
     return tri, bf
+
+
+def domino_cftp(tmax, tmin, seed: int, params, rule: int = 1, p12=None, mode: int = 0, first_steps: int = 128,
+                max_restarts: int = 64):
+    """Coupling from the past restated (AztecDiamondCftpEx.cpp:42-68); returns (sample,
+    steps of the final replay, restarts)."""
+    tmax = np.ascontiguousarray(tmax, dtype=np.int32)
+    tmin = np.ascontiguousarray(tmin, dtype=np.int32)
+    N = int(round(np.sqrt(tmax.size)))
+    p, pp, n = _params(params)
+    pr = None if p12 is None else np.ascontiguousarray(p12, dtype=np.float64)
+    out = np.zeros_like(tmax)
+    tot = C.c_long()
+    rs = C.c_int()
+    rc = lib().or_domino_cftp(rule, None if pr is None else pr.ctypes.data_as(_DP), tmax.ctypes.data_as(_I32P),
+                              tmin.ctypes.data_as(_I32P), N, seed, mode, first_steps, max_restarts, pp, n,
+                              out.ctypes.data_as(_I32P), C.byref(tot), C.byref(rs))
+    if rc:
+        raise ValueError(f"or_domino_cftp rc={rc}")
+    return out, tot.value, rs.value

@@ -874,6 +874,41 @@ int or_domino_walk_epochs(int rule, const double* p12, int32_t* t, int N, const 
     return 0;
 }
 
+// Coupling from the past as Examples/AztecDiamondCftpEx.cpp:42-68 runs it: epoch seeds from
+// std::mt19937(seed); each restart walks copies of tmax and tmin through all epochs (oldest
+// first) and compares them; mode 0 prepends an epoch of twice the last epoch's length
+// (:67), mode 1 one as long as all epochs together.  Returns 0 and the sample, or 1 when
+// max_restarts restarts did not coalesce.
+int or_domino_cftp(int rule, const double* p12, const int32_t* tmax, const int32_t* tmin, int N, long seed, int mode,
+                   long first_steps, int max_restarts, const uint32_t* params, int64_t nparams, int32_t* out,
+                   long* total_steps, int* restarts) {
+    std::mt19937 mt_rand((std::mt19937::result_type)seed);
+    std::vector<long> steps{first_steps}, seeds{(long)mt_rand()};
+    const size_t NN = (size_t)N * N;
+    std::vector<int32_t> top(NN), bottom(NN);
+    for (int r = 0; r <= max_restarts; r++) {
+        std::memcpy(top.data(), tmax, NN * 4);
+        std::memcpy(bottom.data(), tmin, NN * 4);
+        int rc = or_domino_walk_epochs(rule, p12, bottom.data(), N, steps.data(), seeds.data(), (int)steps.size(),
+                                       params, nparams);
+        if (!rc) rc = or_domino_walk_epochs(rule, p12, top.data(), N, steps.data(), seeds.data(), (int)steps.size(),
+                                            params, nparams);
+        if (rc) return rc;
+        long total = 0;
+        for (long s : steps) total += s;
+        if (top == bottom) {
+            std::memcpy(out, top.data(), NN * 4);
+            if (total_steps) *total_steps = total;
+            if (restarts) *restarts = r;
+            return 0;
+        }
+        const long len = (mode == 0) ? 2 * steps.back() : total;
+        steps.insert(steps.begin(), len);
+        seeds.insert(seeds.begin(), (long)mt_rand());
+    }
+    return 1;
+}
+
 // ---- Lozenge -------------------------------------------------------------------------
 // LozengeTiler::Walk(tiling&, int steps, long seed), LozengeTiler.cpp:18-88
 int or_lozenge_walk(int rule, double q, int32_t* t, int N, long steps, long seed, const uint32_t* params,

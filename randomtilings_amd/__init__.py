@@ -18,13 +18,15 @@ LIB_DIR = PKG / "lib"
 
 # status codes (rt_hip.h)
 RT_OK, RT_ERR_ARG, RT_ERR_PARAMS, RT_ERR_HIP, RT_ERR_IO, RT_ERR_DEVICE, RT_ERR_UNTILEABLE = 0, -1, -2, -3, -4, -5, -6
+RT_ERR_NOT_COALESCED = -7
+RT_CFTP_REFERENCE, RT_CFTP_DOUBLING = 0, 1
 # models
 RT_DOMINO, RT_LOZENGE, RT_RECTTRI, RT_TRIDIMER = 0, 1, 2, 3
 # rules
 RT_DOMINO_LAZY, RT_DOMINO_CFTP, RT_DOMINO_WEIGHTED = 0, 1, 2
 RT_LOZENGE_UNIFORM, RT_LOZENGE_QREF, RT_LOZENGE_QHEAT = 0, 1, 2
 RT_SEED_WALK, RT_SEED_EPOCH = 0, 1
-RT_PATH_AUTO, RT_PATH_PERSISTENT, RT_PATH_STEPWISE = 0, 1, 2
+RT_PATH_AUTO, RT_PATH_PERSISTENT, RT_PATH_STEPWISE, RT_PATH_BLOCKED = 0, 1, 2, 3
 
 # Every symbol include/rt_hip.h declares, with (restype, argtypes).
 _P = C.c_void_p
@@ -53,6 +55,8 @@ SIGNATURES = {
     "rt_lozenge_walk": (C.c_int, [_P, C.c_int, C.c_double, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_recttri_walk": (C.c_int, [_P, _FP, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_tridimer_walk": (C.c_int, [_P, _I32P, C.c_int32, C.c_int64, C.c_int64]),
+    "rt_domino_cftp": (C.c_int, [_P, C.c_int, _DP, _I32P, _I32P, C.c_int32, C.c_int64, C.c_int, C.c_int64, C.c_int32,
+                                 _I32P, _I64P, _I32P]),
     "rt_chain_create": (C.c_int, [_P, C.c_int, C.c_int, _DP, C.c_int32, _I32P, C.c_int32, C.POINTER(_P)]),
     "rt_chain_destroy": (C.c_int, [_P]),
     "rt_chain_set_path": (C.c_int, [_P, C.c_int, C.c_int32]),
@@ -221,6 +225,21 @@ class Context:
         N = int(round(np.sqrt(t.size)))
         _check(self._lib.rt_tridimer_walk(self.h, _i32(t), N, steps, seed))
         return t
+
+    def domino_cftp(self, tmax: np.ndarray, tmin: np.ndarray, seed: int, rule: int = RT_DOMINO_CFTP, p12=None,
+                    mode: int = RT_CFTP_REFERENCE, first_steps: int = 128, max_restarts: int = 64):
+        """Coupling from the past (rt_domino_cftp): (sample, steps of the final replay, restarts)."""
+        a = np.ascontiguousarray(tmax, dtype=np.int32)
+        b = np.ascontiguousarray(tmin, dtype=np.int32)
+        N = int(round(np.sqrt(a.size)))
+        rp = None if p12 is None else np.ascontiguousarray(p12, dtype=np.float64)
+        out = np.zeros_like(a)
+        tot = C.c_int64()
+        rs = C.c_int32()
+        _check(self._lib.rt_domino_cftp(self.h, rule, None if rp is None else rp.ctypes.data_as(_DP), _i32(a), _i32(b),
+                                        N, seed, mode, first_steps, max_restarts, _i32(out), C.byref(tot),
+                                        C.byref(rs)))
+        return out, tot.value, rs.value
 
     def chain(self, model: int, rule: int, tiling: np.ndarray, rparams=None) -> "Chain":
         return Chain(self, model, rule, tiling, rparams)

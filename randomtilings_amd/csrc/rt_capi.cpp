@@ -151,7 +151,7 @@ static int td_run(rt_chain* ch, int64_t steps) {
     a.m2 = c->um2;
     a.tm = c->utm;
     a.M = ch->W;
-    const bool small = ch->W <= kSmallMax && ch->path != RT_PATH_STEPWISE;
+    const bool small = ch->W <= kSmallMax && ch->path != RT_PATH_STEPWISE && ch->path != RT_PATH_BLOCKED;
     if (ch->path == RT_PATH_PERSISTENT && !small)
         return fail(RT_ERR_ARG, "the single-workgroup triangle-dimer kernel holds at most %d x %d sites", kSmallMax,
                     kSmallMax);
@@ -349,7 +349,7 @@ int rt_ctx_sync(rt_ctx* c) {
 }
 
 int rt_ctx_set_path(rt_ctx* c, int path, int32_t cfg) {
-    if (!c || path < RT_PATH_AUTO || path > RT_PATH_STEPWISE) return fail(RT_ERR_ARG, "rt_ctx_set_path: bad args");
+    if (!c || path < RT_PATH_AUTO || path > RT_PATH_BLOCKED) return fail(RT_ERR_ARG, "rt_ctx_set_path: bad args");
     c->default_path = path;
     c->default_cfg = cfg;
     return RT_OK;
@@ -456,7 +456,7 @@ static void split(const rt_chain* ch, const int32_t* t, std::vector<uint8_t>& ou
     for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++) {
             int c, col;
-            if (ch->model == RT_DOMINO) {
+            if (ch->model == RT_DOMINO || ch->model == kDominoPair) {
                 c = (i + j) % 2;  // DominoTiler.cpp:20-21
                 col = j / 2;
             } else if (ch->model == RT_LOZENGE) {
@@ -468,7 +468,10 @@ static void split(const rt_chain* ch, const int32_t* t, std::vector<uint8_t>& ou
             }
             const size_t at = c * CS + (size_t)i * W + col;
             if (ch->esize == 1) out[at] = (uint8_t)t[(size_t)i * N + j];
-            else std::memcpy(&out[at * 4], &t[(size_t)i * N + j], 4);
+            else if (ch->esize == 2) {
+                const uint16_t v = (uint16_t)t[(size_t)i * N + j];
+                std::memcpy(&out[at * 2], &v, 2);
+            } else std::memcpy(&out[at * 4], &t[(size_t)i * N + j], 4);
         }
 }
 
@@ -478,7 +481,7 @@ static void merge(const rt_chain* ch, const std::vector<uint8_t>& in, int32_t* t
     for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++) {
             int c, col;
-            if (ch->model == RT_DOMINO) {
+            if (ch->model == RT_DOMINO || ch->model == kDominoPair) {
                 c = (i + j) % 2;
                 col = j / 2;
             } else if (ch->model == RT_LOZENGE) {
@@ -490,7 +493,11 @@ static void merge(const rt_chain* ch, const std::vector<uint8_t>& in, int32_t* t
             }
             const size_t at = c * CS + (size_t)i * W + col;
             if (ch->esize == 1) t[(size_t)i * N + j] = in[at];
-            else std::memcpy(&t[(size_t)i * N + j], &in[at * 4], 4);
+            else if (ch->esize == 2) {
+                uint16_t v;
+                std::memcpy(&v, &in[at * 2], 2);
+                t[(size_t)i * N + j] = v;
+            } else std::memcpy(&t[(size_t)i * N + j], &in[at * 4], 4);
         }
 }
 
@@ -523,7 +530,7 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
     ch->N = N;
     ch->path = ctx->default_path;
     ch->cfg = ctx->default_cfg;
-    if (model == RT_DOMINO) {
+    if (model == RT_DOMINO || model == kDominoPair) {
         // DominoTiler.h:34-37: 2M x 2M, twice zero-padded.  The reference only warns on odd N
         // (DominoTiler.cpp:12) and then indexes past its arrays; odd N is rejected here.
         if (N < 6 || N % 2) {
@@ -532,7 +539,7 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
         }
         ch->NC = 2;
         ch->W = N / 2;
-        ch->esize = 1;
+        ch->esize = model == kDominoPair ? 2 : 1;
         ch->G = (int64_t)(N - 2) * (ch->W - 2);
         if (rule == RT_DOMINO_WEIGHTED && (!rparams || nrparams < 4)) {
             delete ch;
@@ -592,6 +599,12 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
                 delete ch;
                 return fail(RT_ERR_ARG, "domino vertex state %d out of the 8-bit range", tiling[i]);
             }
+    } else if (model == kDominoPair) {
+        for (int i = 0; i < N * N; i++)
+            if (tiling[i] < 0 || tiling[i] > 0xffff) {
+                delete ch;
+                return fail(RT_ERR_ARG, "coupled domino state %d out of range", tiling[i]);
+            }
     }
     ch->P = (int)((((size_t)ch->W * ch->esize + 15) / 16) * 16 / ch->esize);
     hipError_t e = hipSetDevice(ctx->device);
@@ -622,7 +635,7 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
 }
 
 int rt_chain_set_path(rt_chain* ch, int path, int32_t cfg) {
-    if (!ch || path < RT_PATH_AUTO || path > RT_PATH_STEPWISE) return fail(RT_ERR_ARG, "rt_chain_set_path: bad args");
+    if (!ch || path < RT_PATH_AUTO || path > RT_PATH_BLOCKED) return fail(RT_ERR_ARG, "rt_chain_set_path: bad args");
     const int ncfg = ch->model == RT_TRIDIMER ? td_config_count() : persist_config_count(ch->model);
     if (cfg >= ncfg) return fail(RT_ERR_ARG, "tile config %d out of range", cfg);
     ch->path = path;
@@ -639,7 +652,7 @@ int rt_chain_seed(rt_chain* ch, int mode, int64_t seed) {
                     (long long)c->nparams, (long long)ch->G);
     uint32_t tmseed;
     const long lseed = (long)seed;
-    if (ch->model == RT_DOMINO) {
+    if (ch->model == RT_DOMINO || ch->model == kDominoPair) {
         ch->mt.seed(lseed);
         // Walk: TinyMT seeded with the first mt19937 output (DominoTiler.cpp:31-32);
         // epoch: with the epoch seed itself (:81-82).  Both pass through a 32-bit kernel arg.
@@ -659,43 +672,47 @@ int rt_chain_seed(rt_chain* ch, int mode, int64_t seed) {
 }
 
 
-static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c) {
+// Tile configuration for the LDS-resident paths.  *resident is true when the whole tile grid
+// fits on the device at once (one launch runs many blocks of h steps, tiles exchanging
+// borders through flags); otherwise the chain runs "blocked": one launch per block of h
+// steps, the launch boundary being the exchange.
+static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* resident) {
     const int n = persist_config_count(ch->model);
     const bool uni = ch->ctx->uniform;
-    auto fits = [&](int idx) {
+    auto grid = [&](int idx, bool* res) {
         int tr, tc, h, nt;
         if (!persist_config(ch->model, idx, &tr, &tc, &h, &nt)) return false;
         const int trn = (ch->N + tr - 1) / tr, tcn = (ch->W + tc - 1) / tc;
         const int cap = persist_capacity(ch->model, idx, uni, ch->ctx->device);
-        if (trn * tcn > cap || trn * tcn > kMaxBlocks) return false;
+        if (cap <= 0) return false;
+        *res = trn * tcn <= cap && trn * tcn <= kMaxBlocks;
         *nblocks = trn * tcn;
         *tiles_c = tcn;
         return true;
     };
-    if (ch->cfg >= 0) return fits(ch->cfg) ? ch->cfg : -1;
+    if (ch->cfg >= 0) return grid(ch->cfg, resident) ? ch->cfg : -1;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(std::make_tuple(ch->model, ch->N, uni));
-        if (it != g_tuned.end() && fits(it->second)) return it->second;
+        if (it != g_tuned.end() && grid(it->second, resident)) return it->second;
     }
-    // default: the configuration with the fewest redundant halo positions per owned one
-    // among those whose tile grid fits in one wave of resident workgroups
+    // default: the configuration with the fewest redundant halo positions per owned one,
+    // preferring those whose tile grid is resident
     int best = -1;
     double best_cost = 1e30;
     for (int i = 0; i < n; i++) {
-        int nb, tcn;
-        if (!fits(i)) continue;
+        bool res = false;
+        if (!grid(i, &res)) continue;
         int tr, tc, h, nt;
         persist_config(ch->model, i, &tr, &tc, &h, &nt);
-        (void)nb;
         const double redund = (double)(tr + 2 * h) * (tc + 2 * h) / ((double)tr * tc);
-        const double cost = redund + 2.0 / h;  // halo compute + barrier cost per step
+        const double cost = redund + 2.0 / h + (res ? 0.0 : 100.0);  // halo compute + barrier cost per step
         if (cost < best_cost) {
             best_cost = cost;
             best = i;
         }
     }
-    if (best >= 0) fits(best);
+    if (best >= 0) grid(best, resident);
     return best;
 }
 
@@ -742,7 +759,8 @@ int rt_chain_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
         t->path = RT_PATH_PERSISTENT;
         t->cfg = i;
         int nb = 0, tcn = 0;
-        if (choose_persist_cfg(t, &nb, &tcn) != i) continue;  // does not fit
+        bool res = false;
+        if (choose_persist_cfg(t, &nb, &tcn, &res) != i) continue;  // does not fit
         rc = rt_chain_run(t, probe_steps / 4 + 1);  // warm-up
         if (rc == RT_OK) rc = rt_chain_run(t, probe_steps);
         double ms = 0;
@@ -784,7 +802,7 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
     }
     // host colour schedule, drawn exactly as the reference's Walk loop draws it
     for (int64_t s = 0; s < steps; s++) {
-        if (ch->model == RT_DOMINO) {
+        if (ch->model == RT_DOMINO || ch->model == kDominoPair) {
             const int r = ch->mt() % 2;  // DominoTiler.cpp:35-36: r==1 rotates B (class 0)
             ch->hsched[s] = (r == 1) ? 0 : 1;
         } else {
@@ -796,8 +814,10 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
     if (steps == 0) return RT_OK;
 
     int nblocks = 0, tiles_c = 0, cfg = -1;
-    if (ch->path != RT_PATH_STEPWISE) cfg = choose_persist_cfg(ch, &nblocks, &tiles_c);
-    if (ch->path == RT_PATH_PERSISTENT && cfg < 0)
+    bool resident = false;
+    if (ch->path != RT_PATH_STEPWISE) cfg = choose_persist_cfg(ch, &nblocks, &tiles_c, &resident);
+    if (ch->path == RT_PATH_BLOCKED) resident = false;
+    if ((ch->path == RT_PATH_PERSISTENT || ch->path == RT_PATH_BLOCKED) && cfg < 0)
         return fail(RT_ERR_ARG, "no persistent tile configuration fits this chain on the device");
 
     if (cfg < 0) {
@@ -835,8 +855,9 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         int tr, tc, h, nt;
         persist_config(ch->model, cfg, &tr, &tc, &h, &nt);
         HIPCHK(hipEventRecord(ch->ev0, c->stream));
-        // one launch per chunk keeps single kernels short; chunk is a multiple of h
-        const int64_t chunk = (int64_t)h * 2048;
+        // one launch per chunk keeps single kernels short; chunk is a multiple of h.  A grid
+        // that is not resident runs one block of h steps per launch.
+        const int64_t chunk = resident ? (int64_t)h * 2048 : (int64_t)h;
         for (int64_t s0 = 0; s0 < steps; s0 += chunk) {
             const int64_t ns = std::min<int64_t>(chunk, steps - s0);
             PersistArgs a{};
@@ -962,6 +983,83 @@ static int walk(rt_ctx* ctx, int model, int rule, const double* rp, int nrp, int
 int rt_domino_walk(rt_ctx* ctx, int rule, const double* rparams, int32_t* tiling, int32_t N, int64_t steps,
                    int64_t seed) {
     return walk(ctx, RT_DOMINO, rule, rparams, rparams ? 4 : 0, tiling, N, &steps, &seed, 1, RT_SEED_WALK);
+}
+
+int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* tmax, const int32_t* tmin,
+                   int32_t N, int64_t seed, int mode, int64_t first_steps, int32_t max_restarts, int32_t* out,
+                   int64_t* total_steps, int32_t* restarts) {
+    if (!ctx || !tmax || !tmin || !out) return fail(RT_ERR_ARG, "rt_domino_cftp: null argument");
+    if (rule != RT_DOMINO_CFTP && rule != RT_DOMINO_WEIGHTED)
+        return fail(RT_ERR_ARG, "coupling from the past needs a monotone rule (RT_DOMINO_CFTP or RT_DOMINO_WEIGHTED)");
+    if (mode != RT_CFTP_REFERENCE && mode != RT_CFTP_DOUBLING) return fail(RT_ERR_ARG, "unknown CFTP schedule %d", mode);
+    if (first_steps <= 0 || max_restarts < 0) return fail(RT_ERR_ARG, "rt_domino_cftp: bad step bounds");
+    if (N < 6 || N % 2) return fail(RT_ERR_ARG, "domino tiling side %d must be even and >= 6", N);
+    const size_t NN = (size_t)N * N;
+    std::vector<int32_t> pair(NN);
+    for (size_t i = 0; i < NN; i++) {
+        if (tmax[i] < 0 || tmax[i] > 255 || tmin[i] < 0 || tmin[i] > 255)
+            return fail(RT_ERR_ARG, "domino vertex state out of the 8-bit range");
+        pair[i] = tmax[i] | (tmin[i] << 8);
+    }
+    rt_chain* ch = nullptr;
+    int rc = rt_chain_create(ctx, kDominoPair, rule, rparams, rparams ? 4 : 0, pair.data(), N, &ch);
+    if (rc) return rc;
+    const size_t bytes = (size_t)N * ch->P * ch->NC * ch->esize;
+    void* init = nullptr;
+    uint32_t* count = nullptr;
+    hipError_t e = hipMalloc(&init, bytes);
+    if (e == hipSuccess) e = hipMalloc(&count, 64);
+    if (e == hipSuccess) e = hipMemcpy(init, ch->cls[ch->cur], bytes, hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) {
+        if (init) hipFree(init);
+        if (count) hipFree(count);
+        rt_chain_destroy(ch);
+        return fail(RT_ERR_HIP, "rt_domino_cftp setup: %s", hipGetErrorString(e));
+    }
+    // epochs, oldest first (AztecDiamondCftpEx.cpp:47-48, :67)
+    std::mt19937 mt((std::mt19937::result_type)seed);
+    std::vector<std::pair<int64_t, int64_t>> epochs{{first_steps, (int64_t)mt()}};
+    rc = RT_ERR_NOT_COALESCED;
+    int r = 0;
+    for (; r <= max_restarts; r++) {
+        int st = RT_OK;
+        if (hipMemcpyAsync(ch->cls[ch->cur], init, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            st = fail(RT_ERR_HIP, "rt_domino_cftp: reset failed");
+        int64_t total = 0;
+        for (size_t k = 0; k < epochs.size() && st == RT_OK; k++) {
+            st = rt_chain_seed(ch, RT_SEED_EPOCH, epochs[k].second);
+            if (st == RT_OK) st = rt_chain_run(ch, epochs[k].first);
+            total += epochs[k].first;
+        }
+        uint32_t differ = 1;
+        if (st == RT_OK && launch_pair_differs((const uint16_t*)ch->cls[ch->cur], (int64_t)(bytes / 2), count,
+                                               ctx->stream) != hipSuccess)
+            st = fail(RT_ERR_HIP, "rt_domino_cftp: coupling test launch failed");
+        if (st == RT_OK) st = rt_chain_sync(ch);
+        if (st == RT_OK && hipMemcpy(&differ, count, sizeof(differ), hipMemcpyDeviceToHost) != hipSuccess)
+            st = fail(RT_ERR_HIP, "rt_domino_cftp: coupling test read failed");
+        if (st != RT_OK) {
+            rc = st;
+            break;
+        }
+        if (differ == 0) {
+            std::vector<int32_t> both(NN);
+            rc = rt_chain_read(ch, both.data());
+            if (rc == RT_OK) {
+                for (size_t i = 0; i < NN; i++) out[i] = both[i] & 0xff;
+                if (total_steps) *total_steps = total;
+                if (restarts) *restarts = r;
+            }
+            break;
+        }
+        const int64_t len = (mode == RT_CFTP_REFERENCE) ? 2 * epochs.back().first : total;
+        epochs.insert(epochs.begin(), {len, (int64_t)mt()});
+    }
+    if (rc == RT_ERR_NOT_COALESCED) fail(rc, "no coalescence after %d restarts", max_restarts);
+    hipFree(init);
+    hipFree(count);
+    rt_chain_destroy(ch);
+    return rc;
 }
 
 int rt_domino_walk_epochs(rt_ctx* ctx, int rule, const double* rparams, int32_t* tiling, int32_t N,

@@ -9,6 +9,9 @@
 
 namespace rt {
 
+// Internal model id of the coupled domino pair (the two chains of coupling from the past).
+constexpr int kDominoPair = 4;
+
 struct StepwiseArgs {
     RuleParams rp;
     void* cls;  // NC contiguous class arrays of N rows of pitch P elements
@@ -41,6 +44,7 @@ struct PersistArgs {
 hipError_t launch_init_gens(int64_t G, uint32_t seed, const uint4* params, uint32_t m1, uint32_t m2, uint32_t tm,
                             uint4* gens, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t st);
+hipError_t launch_pair_differs(const uint16_t* a, int64_t n, uint32_t* count, hipStream_t st);
 hipError_t launch_stepwise(int model, const StepwiseArgs& a, hipStream_t st);
 
 int persist_config_count(int model);

@@ -19,6 +19,8 @@
 #include "rt_loz4.hpp"
 #include "rt_models.hpp"
 
+#include <algorithm>
+
 namespace rt {
 
 // ---------------------------------------------------------------------------------
@@ -56,6 +58,25 @@ __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
 
 hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+// Coupling test of a coupled domino pair: counts the positions whose two chains differ.
+__global__ void k_pair_differs(const uint16_t* __restrict__ a, int64_t n, uint32_t* __restrict__ count) {
+    uint32_t local = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint16_t v = a[i];
+        local += (v & 0xff) != (v >> 8);
+    }
+    // wave-level sum, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(count, local);
+}
+
+hipError_t launch_pair_differs(const uint16_t* a, int64_t n, uint32_t* count, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, count, 1, 0u);
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_pair_differs, dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(256), 0, st, a, n, count);
     return hipGetLastError();
 }
 
@@ -114,6 +135,7 @@ hipError_t launch_stepwise(int model, const StepwiseArgs& a, hipStream_t st) {
         case RT_DOMINO: return launch_stepwise_t<DominoModel>(a, st);
         case RT_LOZENGE: return launch_stepwise_t<LozengeModel>(a, st);
         case RT_RECTTRI: return launch_stepwise_t<RectTriModel>(a, st);
+        case kDominoPair: return launch_stepwise_t<DominoPairModel>(a, st);
     }
     return hipErrorInvalidValue;
 }
@@ -321,6 +343,7 @@ static const Cfg* cfg_table(int model, int* n) {
         case RT_DOMINO: *n = sizeof(kDominoCfgs) / sizeof(Cfg); return kDominoCfgs;
         case RT_LOZENGE: *n = sizeof(kLozengeCfgs) / sizeof(Cfg); return kLozengeCfgs;
         case RT_RECTTRI: *n = sizeof(kRectTriCfgs) / sizeof(Cfg); return kRectTriCfgs;
+        case kDominoPair: *n = sizeof(kDominoCfgs) / sizeof(Cfg); return kDominoCfgs;
     }
     *n = 0;
     return nullptr;
@@ -370,6 +393,11 @@ static const void* persist_kernel(int model, int idx, bool uni) {
         case RT_RECTTRI: {
 #define X(a, b, c, d, e) RT_PICK(RectTriModel, a, b, c, d, e)
             RT_RECTTRI_CFGS(X)
+#undef X
+        } break;
+        case kDominoPair: {
+#define X(a, b, c, d, e) RT_PICK(DominoPairModel, a, b, c, d, e)
+            RT_DOMINO_CFGS(X)
 #undef X
         } break;
     }

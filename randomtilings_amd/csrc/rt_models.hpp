@@ -65,6 +65,43 @@ struct DominoModel {
     }
 };
 
+// Two domino chains driven by the same random numbers (the top and bottom chains of
+// coupling from the past, Examples/AztecDiamondCftpEx.cpp:42-68): each position holds the
+// upper chain's state in the low byte and the lower chain's in the high byte.  One TinyMT
+// draw decides both flips, and the update's bit moves never cross a byte, so it runs on
+// both chains at once.
+struct DominoPairModel {
+    static constexpr int NC = 2;
+    using T = uint16_t;
+    __device__ __forceinline__ static bool has_gen(int gi, int gj, int N, int W) {
+        return DominoModel::has_gen(gi, gj, N, W);
+    }
+    __device__ __forceinline__ static int gen_id(int gi, int gj, int N, int W) {
+        return DominoModel::gen_id(gi, gj, N, W);
+    }
+    __device__ __forceinline__ static bool updatable(int gi, int gj, int N, int W) {
+        return DominoModel::updatable(gi, gj, N, W);
+    }
+    using StepRule = DominoModel::StepRule;
+    __device__ __forceinline__ static StepRule step_rule(const RuleParams& rp, int c) {
+        return DominoModel::step_rule(rp, c);
+    }
+    __device__ __forceinline__ static T flip(const StepRule& r, int gi, T s, uint32_t k) {
+        const uint8_t lo = DominoModel::flip(r, gi, (uint8_t)(s & 0xff), k);
+        const uint8_t hi = DominoModel::flip(r, gi, (uint8_t)(s >> 8), k);
+        return (T)(lo | (hi << 8));
+    }
+    template <class P>
+    __device__ __forceinline__ static void update(int c, int gi, P* base, int cs, int idx, int pitch) {
+        const int t = 1 - c;
+        const P* src = base + c * cs;
+        P* dst = base + (1 - c) * cs;
+        const int offL = (gi + t + 1) & 1, offR = (gi + t) & 1;
+        dst[idx] = (P)(((src[idx - pitch] & 0x0202) >> 1) | ((src[idx + pitch] & 0x0101) << 1) |
+                       ((src[idx - offL] & 0x0808) >> 1) | ((src[idx + offR] & 0x0404) << 1));
+    }
+};
+
 struct LozengeModel {
     static constexpr int NC = 3;
     using T = uint8_t;

@@ -104,6 +104,21 @@ void DominoTiler::WalkWeighted(tiling& t, long steps, long seed, const double p1
                  "DominoTiler::WalkWeighted");
 }
 
+tiling DominoTiler::CFTP(const tiling& tMax, const tiling& tMin, long seed, long first_steps, bool doubling,
+                         int max_restarts, const double* p12, long* total_steps) {
+    const int N = rtapi::side(tMax.size());
+    if (tMin.size() != tMax.size()) throw std::runtime_error("DominoTiler::CFTP: tMax and tMin differ in size");
+    tiling out(tMax.size(), 0);
+    int64_t total = 0;
+    int32_t restarts = 0;
+    rtapi::check(rt_domino_cftp(ctx.get(), p12 ? RT_DOMINO_WEIGHTED : rule, p12, tMax.data(), tMin.data(), N, seed,
+                                doubling ? RT_CFTP_DOUBLING : RT_CFTP_REFERENCE, first_steps, max_restarts, out.data(),
+                                &total, &restarts),
+                 "DominoTiler::CFTP");
+    if (total_steps) *total_steps = (long)total;
+    return out;
+}
+
 // ---- domains ----------------------------------------------------------------------------
 
 domain DominoTiler::Rectangle(int M, int N) {

@@ -55,6 +55,15 @@ class DominoTiler {
     // the vertices with i+j even.  Uniform p12 = 1/2 is the dominokernelCFTP.cl rule.
     void WalkWeighted(tiling &t, long steps, long seed, const double p12[4]);
 
+    // Coupling from the past: an exact sample of the chain's stationary law, drawn the way
+    // Examples/AztecDiamondCftpEx.cpp does with Walk (epoch seeds from std::mt19937(seed),
+    // epochs of first_steps and then 2*first_steps prepended per restart; `doubling` doubles
+    // the total instead).  tMax/tMin are the domain's maximal and minimal tilings.  The
+    // tiler's rule must be monotone (dominokernelCFTP.cl); p12 selects the weighted rule.
+    // Throws std::runtime_error when max_restarts restarts do not coalesce.
+    tiling CFTP(const tiling &tMax, const tiling &tMin, long seed, long first_steps = 128, bool doubling = false,
+                int max_restarts = 64, const double *p12 = nullptr, long *total_steps = nullptr);
+
     // The rule this tiler runs (RT_DOMINO_LAZY or RT_DOMINO_CFTP of rt_hip.h).
     int Rule() const { return rule; }
     rt_ctx *Handle() const { return ctx.get(); }

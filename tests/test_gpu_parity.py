@@ -265,3 +265,45 @@ def test_tridimer_chain_runs_compose(gctx, golden):
         g = ch.read()
         ch.close()
         assert np.array_equal(g, O.tridimer_walk(t0, 200, 31, default_table(BIG)))
+
+
+# ---- coupling from the past -------------------------------------------------------------------
+@pytest.mark.parametrize("mode", [rt.RT_CFTP_REFERENCE, rt.RT_CFTP_DOUBLING])
+@pytest.mark.parametrize("rule,p12", [(rt.RT_DOMINO_CFTP, None), (rt.RT_DOMINO_WEIGHTED, [0.8, 0.2, 0.2, 0.8])])
+def test_domino_cftp_parity(gctx, golden, mode, rule, p12):
+    """rt_domino_cftp (coupled device chains, device coupling test) returns the oracle's
+    sample, final replay length and restart count."""
+    for n, seed in [(2, 345903), (5, 345903), (16, 11), (32, 7)]:
+        tmax, tmin = golden[f"domino_aztec{n}_max"], golden[f"domino_aztec{n}_min"]
+        say("cftp", mode, rule, n, seed)
+        g = gctx.domino_cftp(tmax, tmin, seed, rule=rule, p12=p12, mode=mode, max_restarts=40)
+        o = O.domino_cftp(tmax, tmin, seed, default_table(BIG), rule=rule, p12=p12, mode=mode, max_restarts=40)
+        assert np.array_equal(g[0], o[0]) and g[1:] == o[1:], (n, g[1:], o[1:])
+
+
+def test_domino_cftp_bound(gctx, golden):
+    tmax, tmin = golden["domino_aztec16_max"], golden["domino_aztec16_min"]
+    with pytest.raises(rt.RTError) as e:
+        gctx.domino_cftp(tmax, tmin, 11, max_restarts=0, first_steps=4)
+    assert e.value.code == rt.RT_ERR_NOT_COALESCED
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_domino_blocked_path(gctx, cfg):
+    """The blocked path (one launch per block of h steps, the launch boundary being the
+    exchange; what a tile grid too large to be resident runs) on an Aztec diamond of order
+    700 (~2000 tiles of 16 x 32)."""
+    from randomtilings_amd import statics as S
+    d = S.call("domino.AztecDiamond", args=[700])
+    t0 = S.call("domino.MinTiling", d)
+    say("domino blocked", cfg)
+    ch = gctx.chain(rt.RT_DOMINO, rt.RT_DOMINO_CFTP, t0)
+    ch.set_path(rt.RT_PATH_BLOCKED, cfg)
+    ch.seed(5)
+    ch.run(37)
+    launches = ch.last_launches()
+    g = ch.read()
+    ch.close()
+    h = rt.tile_config(rt.RT_DOMINO, cfg)[2]
+    assert launches == (37 + h - 1) // h
+    assert np.array_equal(g, O.domino_walk(t0, 37, 5, default_table(BIG), rule=1))

@@ -180,3 +180,62 @@ def test_tridimer_uniform_law(golden):
     exp = S / 6
     chi2 = sum((c - exp) ** 2 / exp for c in counts.values())
     assert chi2 < 20.5  # chi-square, 5 dof, p = 0.001
+
+
+def test_cftp_schedules(golden):
+    """Final replay lengths: 128 + 256 r for the reference schedule
+    (AztecDiamondCftpEx.cpp:47-67), 128 * 2^r when doubling."""
+    tmax, tmin = golden["domino_aztec16_max"], golden["domino_aztec16_min"]
+    p = default_table(tmax.size)
+    _, tot, r = O.domino_cftp(tmax, tmin, 345903, p, mode=0)
+    assert r > 0 and tot == 128 + 256 * r
+    _, tot, r = O.domino_cftp(tmax, tmin, 345903, p, mode=1)
+    assert r > 0 and tot == 128 * 2 ** r
+
+
+def test_cftp_uniform_law(golden):
+    """Coupling from the past with dominokernelCFTP.cl's rule samples the 8 tilings of the
+    order-2 Aztec diamond uniformly."""
+    tmax, tmin = golden["domino_aztec2_max"], golden["domino_aztec2_min"]
+    p = default_table(tmax.size)
+    counts = {}
+    S = 2400
+    for s in range(S):
+        t, _, _ = O.domino_cftp(tmax, tmin, 1000003 * s + 7, p)
+        counts[t.tobytes()] = counts.get(t.tobytes(), 0) + 1
+    assert len(counts) == 8
+    exp = S / 8
+    assert sum((c - exp) ** 2 / exp for c in counts.values()) < 24.3  # 7 dof, p = 0.001
+
+
+def test_cftp_weighted_law(golden):
+    """The weighted heat-bath (P(12) = p_v per vertex class and row parity) has stationary
+    law mu(T) ~ prod_v (p_v / (1 - p_v))^((-1)^(i+j) h_T(v) / 4), h the domino height
+    function: one flip at v changes only h(v), by 4 in the direction set by the
+    checkerboard colour of v.  CFTP samples on the order-2 Aztec diamond follow it."""
+    from randomtilings_amd import statics as St
+    n = 2
+    tmax, tmin, d = (golden[f"domino_aztec{n}_{k}"] for k in ("max", "min", "domain"))
+    p12 = [0.8, 0.2, 0.3, 0.6]
+    p = default_table(tmax.size)
+    counts, tiles = {}, {}
+    S = 4000
+    for s in range(S):
+        t, _, _ = O.domino_cftp(tmax, tmin, 1000003 * s + 7, p, rule=2, p12=p12)
+        counts[t.tobytes()] = counts.get(t.tobytes(), 0) + 1
+        tiles[t.tobytes()] = t
+    assert len(counts) == 8
+    N = int(round(np.sqrt(tmax.size)))
+    inf = np.iinfo(np.int32).max
+    keys = list(counts)
+    logw = []
+    for k in keys:
+        h = St.call("domino.TilingToHeightfunc", tiles[k], d).reshape(N, N)
+        ii, jj = np.nonzero(h != inf)
+        pv = np.array([p12[((i + j) % 2) * 2 + (i & 1)] for i, j in zip(ii, jj)])
+        logw.append(np.sum((-1.0) ** (ii + jj) * h[ii, jj] / 4 * np.log(pv / (1 - pv))))
+    w = np.exp(np.array(logw) - max(logw))
+    w /= w.sum()
+    obs = np.array([counts[k] for k in keys])
+    chi2 = float(((obs - S * w) ** 2 / (S * w)).sum())
+    assert chi2 < 24.3, chi2  # 7 dof, p = 0.001
