@@ -59,6 +59,8 @@ struct rt_ctx {
     int default_path = RT_PATH_AUTO;
     int default_cfg = -1;
     hipEvent_t marks[16] = {};
+    int live_chains = 0;   // chains created on this context and not yet destroyed
+    bool closing = false;  // rt_destroy was called while chains were live
 };
 
 struct rt_chain {
@@ -92,6 +94,7 @@ struct rt_chain {
     std::vector<uint8_t> hsched;
     int64_t last_launches = 0;
     int32_t last_cfg = -1;
+    bool counted = false;  // counted in ctx->live_chains (chains handed out by rt_chain_create)
 };
 
 // Tile configurations chosen by rt_chain_autotune, keyed by (model, side, uniform params).
@@ -311,8 +314,7 @@ int rt_create(int device, rt_ctx** out) {
     return RT_OK;
 }
 
-int rt_destroy(rt_ctx* c) {
-    if (!c) return RT_OK;
+static void ctx_free(rt_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (hipEvent_t& e : c->marks)
@@ -320,6 +322,17 @@ int rt_destroy(rt_ctx* c) {
     if (c->params) hipFree(c->params);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
+}
+
+// A context outlives its chains: destroying it while chains are live defers the release to
+// the destruction of the last one.
+int rt_destroy(rt_ctx* c) {
+    if (!c) return RT_OK;
+    if (c->live_chains > 0) {
+        c->closing = true;
+        return RT_OK;
+    }
+    ctx_free(c);
     return RT_OK;
 }
 
@@ -516,7 +529,10 @@ int rt_chain_destroy(rt_chain* ch) {
     if (ch->ev0) hipEventDestroy(ch->ev0);
     if (ch->ev1) hipEventDestroy(ch->ev1);
     if (ch->ev_sched) hipEventDestroy(ch->ev_sched);
+    rt_ctx* c = ch->ctx;
+    const bool counted = ch->counted;
     delete ch;
+    if (counted && --c->live_chains == 0 && c->closing) ctx_free(c);
     return RT_OK;
 }
 
@@ -581,6 +597,8 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
             rt_chain_destroy(ch);
             return rc;
         }
+        ch->counted = true;
+        ctx->live_chains++;
         *out = ch;
         return RT_OK;
     } else {
@@ -630,6 +648,8 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
         rt_chain_destroy(ch);
         return fail(RT_ERR_HIP, "chain upload: %s", hipGetErrorString(e));
     }
+    ch->counted = true;
+    ctx->live_chains++;
     *out = ch;
     return RT_OK;
 }

@@ -269,15 +269,20 @@ def test_tridimer_chain_runs_compose(gctx, golden):
 
 # ---- coupling from the past -------------------------------------------------------------------
 @pytest.mark.parametrize("mode", [rt.RT_CFTP_REFERENCE, rt.RT_CFTP_DOUBLING])
-@pytest.mark.parametrize("rule,p12", [(rt.RT_DOMINO_CFTP, None), (rt.RT_DOMINO_WEIGHTED, [0.8, 0.2, 0.2, 0.8])])
+@pytest.mark.parametrize("rule,p12", [(rt.RT_DOMINO_CFTP, None), (rt.RT_DOMINO_WEIGHTED, [0.6, 0.4, 0.45, 0.55])])
 def test_domino_cftp_parity(gctx, golden, mode, rule, p12):
     """rt_domino_cftp (coupled device chains, device coupling test) returns the oracle's
-    sample, final replay length and restart count."""
-    for n, seed in [(2, 345903), (5, 345903), (16, 11), (32, 7)]:
+    sample, final replay length and restart count (the cases coalesce within 17 restarts of
+    the reference schedule and 8 doublings)."""
+    cases = [(2, 345903), (5, 345903), (16, 11)]
+    if mode == rt.RT_CFTP_DOUBLING:  # the reference schedule grows only linearly per restart
+        cases.append((32, 7))
+    bound = 24 if mode == rt.RT_CFTP_REFERENCE else 9
+    for n, seed in cases:
         tmax, tmin = golden[f"domino_aztec{n}_max"], golden[f"domino_aztec{n}_min"]
         say("cftp", mode, rule, n, seed)
-        g = gctx.domino_cftp(tmax, tmin, seed, rule=rule, p12=p12, mode=mode, max_restarts=40)
-        o = O.domino_cftp(tmax, tmin, seed, default_table(BIG), rule=rule, p12=p12, mode=mode, max_restarts=40)
+        g = gctx.domino_cftp(tmax, tmin, seed, rule=rule, p12=p12, mode=mode, max_restarts=bound)
+        o = O.domino_cftp(tmax, tmin, seed, default_table(BIG), rule=rule, p12=p12, mode=mode, max_restarts=bound)
         assert np.array_equal(g[0], o[0]) and g[1:] == o[1:], (n, g[1:], o[1:])
 
 
@@ -298,12 +303,29 @@ def test_domino_blocked_path(gctx, cfg):
     t0 = S.call("domino.MinTiling", d)
     say("domino blocked", cfg)
     ch = gctx.chain(rt.RT_DOMINO, rt.RT_DOMINO_CFTP, t0)
-    ch.set_path(rt.RT_PATH_BLOCKED, cfg)
-    ch.seed(5)
-    ch.run(37)
-    launches = ch.last_launches()
-    g = ch.read()
-    ch.close()
+    try:
+        ch.set_path(rt.RT_PATH_BLOCKED, cfg)
+        ch.seed(5)
+        ch.run(37)
+        launches = ch.last_launches
+        g = ch.read()
+    finally:
+        ch.close()
     h = rt.tile_config(rt.RT_DOMINO, cfg)[2]
     assert launches == (37 + h - 1) // h
     assert np.array_equal(g, O.domino_walk(t0, 37, 5, default_table(BIG), rule=1))
+
+
+def test_context_outlives_its_chains(golden):
+    """Destroying a context while a chain is live defers the release to the chain's
+    destruction; the chain keeps working meanwhile."""
+    t0 = golden["lozenge_hex10_2_min"]
+    c = rt.Context(0)
+    c.load_params(default_table(BIG))
+    ch = c.chain(rt.RT_LOZENGE, rt.RT_LOZENGE_QREF, t0, [0.9])
+    c.close()
+    ch.seed(3)
+    ch.run(77)
+    g = ch.read()
+    ch.close()
+    assert np.array_equal(g, O.lozenge_walk(t0, 77, 3, default_table(BIG), rule=1, q=0.9))
