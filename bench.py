@@ -47,6 +47,12 @@ def main() -> int:
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (to scale HBM byte counters)")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c4", "tridimer"],
+                    help="c2 (default): lozenge hexagon 1000; c1: domino Aztec diamond 128, 10^5 steps; "
+                         "c4: coupling from the past on an Aztec diamond (--aztec); tridimer: triangle-dimer cone")
+    ap.add_argument("--aztec", type=int, default=256, help="Aztec diamond order of the c4 workload")
+    ap.add_argument("--cone", type=int, default=1000, help="cone side of the tridimer workload")
+    ap.add_argument("--check", action="store_true", help="c1: also compare the final tiling with the oracle")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -69,6 +75,9 @@ def main() -> int:
     import randomtilings_amd as rt
     from randomtilings_amd import statics
     from randomtilings_amd.params import default_table
+
+    if args.workload != "c2":
+        return {"c1": bench_c1, "c4": bench_c4, "tridimer": bench_tridimer}[args.workload](args)
 
     d = statics.call("lozenge.AlmostHexDomain", args=[args.hex, 0])
     t0 = statics.call("lozenge.MinTiling", d)
@@ -176,6 +185,140 @@ def main() -> int:
         "cpu_baseline": cpu,
         "gpu_kernel_ms_per_chain_step": gpu_ms / (args.steps * S),
     }
+    print(json.dumps(line), flush=True)
+    ch.close()
+    ctx.close()
+    return 0
+
+
+def _line(metric, value, unit, args, ms_per_step, dtype, workload, extra, higher=True):
+    return {
+        "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype, "data": "synthetic (deterministic domain, TinyMT default parameter table)",
+        "config": dict({"workload": workload, "parallelism": "single GPU"}, **extra),
+    }
+
+
+def bench_c1(args) -> int:
+    """Domino tilings of the Aztec diamond of order 128 (tiling side 262), dominokernelCFTP.cl's
+    rule, minimal start, 10^5 chain steps per bench step; latency-bound (the 67 KB state lives
+    in LDS), so the figure of merit is chain steps per second."""
+    import randomtilings_amd as rt
+    from randomtilings_amd import statics
+    from randomtilings_amd.params import default_table
+
+    d = statics.call("domino.AztecDiamond", args=[128])
+    t0 = statics.call("domino.MinTiling", d)
+    N = int(round(np.sqrt(t0.size)))
+    ctx = rt.Context(0)
+    ctx.load_params(default_table(t0.size // 2))
+    ch = ctx.chain(rt.RT_DOMINO, rt.RT_DOMINO_CFTP, t0)
+    ch.autotune(4096)
+    S = 100000
+    ch.seed(args.seed)
+    for _ in range(args.warmup):
+        ch.run(S)
+    ch.sync()
+    if args.check:
+        ch.seed(args.seed)
+    ctx.sync()
+    ctx.mark(0)
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        ch.run(S)
+    ctx.mark(1)
+    ch.sync()
+    wall = time.perf_counter() - w0
+    gpu_ms = ctx.elapsed_ms(0, 1)
+    extra = {"aztec_order": 128, "tiling_side": N, "chain_steps_per_step": S,
+             "attempts_per_chain_step": ch.attempts_per_step,
+             "flip_attempts_per_s": ch.attempts_per_step * S * args.steps / wall,
+             "gpu_us_per_chain_step": gpu_ms * 1e3 / (S * args.steps),
+             "tile_config": ch.tile_configs()[ch.last_config] if ch.last_config >= 0 else "stepwise"}
+    if args.check:
+        import oracle
+        g = ch.read()
+        o = oracle.domino_walk(t0, S * args.steps, args.seed, default_table(t0.size // 2), rule=1)
+        extra["bit_identical_to_oracle"] = bool(np.array_equal(g, o))
+    line = _line("chain steps/s (domino Aztec diamond n=128, CFTP rule)", S * args.steps / wall, "steps/s", args,
+                 wall * 1e3 / args.steps, "u8", "domino Aztec diamond n=128, dominokernelCFTP.cl rule, minimal start, "
+                 "10^5 chain steps per bench step", extra)
+    print(json.dumps(line), flush=True)
+    ch.close()
+    ctx.close()
+    return 0
+
+
+def bench_c4(args) -> int:
+    """Coupling from the past on the Aztec diamond of order --aztec, uniform rule, doubling
+    schedule: wall time to coalescence of the coupled chain (one launch sequence per restart,
+    device coupling test)."""
+    import randomtilings_amd as rt
+    from randomtilings_amd import statics
+    from randomtilings_amd.params import default_table
+
+    n = args.aztec
+    d = statics.call("domino.AztecDiamond", args=[n])
+    tmax, tmin = statics.call("domino.MaxTiling", d), statics.call("domino.MinTiling", d)
+    ctx = rt.Context(0)
+    ctx.load_params(default_table(tmax.size // 2))
+    times, steps, restarts = [], [], []
+    for k in range(args.warmup + args.steps):
+        w0 = time.perf_counter()
+        out, tot, r = ctx.domino_cftp(tmax, tmin, args.seed + k, mode=rt.RT_CFTP_DOUBLING, first_steps=1024,
+                                      max_restarts=30)
+        if k >= args.warmup:
+            times.append(time.perf_counter() - w0)
+            steps.append(tot)
+            restarts.append(r)
+    mean = float(np.mean(times))
+    executed = [sum(1024 * 2 ** i for i in range(r + 1)) for r in restarts]  # every replay's steps
+    extra = {"aztec_order": n, "schedule": "doubling from 1024 steps", "final_replay_steps": steps,
+             "restarts": restarts, "coupled_steps_executed_per_sample": executed,
+             "chain_attempts_per_step": 2 * (tmax.size // 2), "samples": args.steps}
+    line = _line(f"seconds to coalescence (CFTP, domino Aztec diamond n={n})", mean, "s", args, mean * 1e3,
+                 "u8x2", f"CFTP domino Aztec diamond n={n}, dominokernelCFTP.cl rule, coupled max/min chains",
+                 extra, higher=False)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+def bench_tridimer(args) -> int:
+    """Triangle-dimer chain on IceCreamCone(--cone, 0): flip attempts per second of the
+    one-launch-per-step tiled kernel (all three move types, reference schedule)."""
+    import randomtilings_amd as rt
+    from randomtilings_amd import statics
+    from randomtilings_amd.params import default_table
+
+    t0 = statics.call("tridimer.IceCreamCone", args=[args.cone, 0])
+    N = int(round(np.sqrt(t0.size)))
+    ctx = rt.Context(0)
+    ctx.load_params(default_table((N // 2) ** 2))
+    ch = ctx.chain(rt.RT_TRIDIMER, 0, t0)
+    ch.autotune(64)
+    ch.seed(args.seed)
+    S = 2000
+    for _ in range(args.warmup):
+        ch.run(S)
+    ch.sync()
+    ctx.sync()
+    ctx.mark(0)
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        ch.run(S)
+    ctx.mark(1)
+    ch.sync()
+    wall = time.perf_counter() - w0
+    gpu_ms = ctx.elapsed_ms(0, 1)
+    att = ch.attempts_per_step
+    extra = {"cone_side": args.cone, "tiling_side": N, "move_lattice": N // 2, "chain_steps_per_step": S,
+             "attempts_per_chain_step": att, "gpu_us_per_chain_step": gpu_ms * 1e3 / (S * args.steps),
+             "state_bytes": ch.state_bytes, "tile_config": ch.last_config}
+    line = _line("flip attempts/sec (triangle dimers)", att * S * args.steps / wall, "flip attempts/s", args,
+                 wall * 1e3 / args.steps, "u16", f"triangle-dimer IceCreamCone({args.cone},0), reference move schedule",
+                 extra)
     print(json.dumps(line), flush=True)
     ch.close()
     ctx.close()

@@ -61,15 +61,16 @@ hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t st) {
     return hipGetLastError();
 }
 
-// Coupling test of a coupled domino pair: counts the positions whose two chains differ.
+// Coupling test of a coupled domino pair: counts the positions whose two chains differ, one
+// wave64 ballot per 64 positions and one atomic per wave.
 __global__ void k_pair_differs(const uint16_t* __restrict__ a, int64_t n, uint32_t* __restrict__ count) {
     uint32_t local = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint16_t v = a[i];
-        local += (v & 0xff) != (v >> 8);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int64_t i = base + threadIdx.x;
+        const bool differ = i < n && ((a[i] & 0xff) != (a[i] >> 8));
+        local += (uint32_t)__popcll(__ballot(differ));
     }
-    // wave-level sum, one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off);
     if ((threadIdx.x & 63) == 0 && local) atomicAdd(count, local);
 }
 
