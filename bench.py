@@ -47,7 +47,7 @@ def main() -> int:
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (to scale HBM byte counters)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c4", "tridimer"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c4", "tridimer", "closure"],
                     help="c2 (default): lozenge hexagon 1000; c1: domino Aztec diamond 128, 10^5 steps; "
                          "c4: coupling from the past on an Aztec diamond (--aztec); tridimer: triangle-dimer cone")
     ap.add_argument("--aztec", type=int, default=256, help="Aztec diamond order of the c4 workload")
@@ -77,7 +77,7 @@ def main() -> int:
     from randomtilings_amd.params import default_table
 
     if args.workload != "c2":
-        return {"c1": bench_c1, "c4": bench_c4, "tridimer": bench_tridimer}[args.workload](args)
+        return {"c1": bench_c1, "c4": bench_c4, "tridimer": bench_tridimer, "closure": bench_closure}[args.workload](args)
 
     d = statics.call("lozenge.AlmostHexDomain", args=[args.hex, 0])
     t0 = statics.call("lozenge.MinTiling", d)
@@ -321,6 +321,35 @@ def bench_tridimer(args) -> int:
                  extra)
     print(json.dumps(line), flush=True)
     ch.close()
+    ctx.close()
+    return 0
+
+
+def bench_closure(args) -> int:
+    """Maximal tiling of the a=b=c=--hex hexagon (N = 2007 at 1000) built on the device
+    (host boundary walk, device closure, device read-off) against the host construction."""
+    import randomtilings_amd as rt
+    from randomtilings_amd import statics
+
+    d = statics.call("lozenge.AlmostHexDomain", args=[args.hex, 0])
+    ctx = rt.Context(0)
+    for _ in range(args.warmup):
+        ctx.extremal_tiling(rt.RT_LOZENGE, d, True)
+    times = []
+    for _ in range(args.steps):
+        w0 = time.perf_counter()
+        t, _ = ctx.extremal_tiling(rt.RT_LOZENGE, d, True)
+        times.append(time.perf_counter() - w0)
+    w0 = time.perf_counter()
+    ht = statics.call("lozenge.MaxTiling", d)
+    host_s = time.perf_counter() - w0
+    mean = float(np.mean(times))
+    N = int(round(np.sqrt(t.size)))
+    extra = {"hexagon": args.hex, "tiling_side": N, "host_construction_s": host_s,
+             "identical_to_host": bool(np.array_equal(t, ht))}
+    line = _line("seconds per maximal tiling (device closure)", mean, "s", args, mean * 1e3, "i32",
+                 f"lozenge MaxTiling of AlmostHexDomain({args.hex},0)", extra, higher=False)
+    print(json.dumps(line), flush=True)
     ctx.close()
     return 0
 

@@ -165,6 +165,28 @@ int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* 
                    int32_t N, int64_t seed, int mode, int64_t first_steps, int32_t max_restarts, int32_t* out,
                    int64_t* total_steps, int32_t* restarts);
 
+/* ---- height functions ----------------------------------------------------------------- */
+
+/* MaxTiling (dir = +1) / MinTiling (dir = -1) on the device — DominoTiler.cpp:214-400,
+ * LozengeTiler.cpp:140-318.  domain: RT_DOMINO an M x M array of squares, RT_LOZENGE an
+ * M x 2M array of triangles; tiling (and heights, may be NULL): (M+1) x (M+1).  The boundary
+ * walk runs on the host, the closure of the boundary heights over the domain's directed unit
+ * edges on the GPU, and the tiling is read off the heights on the GPU.  Returns
+ * RT_ERR_UNTILEABLE with an all-zero tiling when the region has no tiling (the reference
+ * prints a message and returns a partial height vector). */
+int rt_extremal_tiling(rt_ctx* ctx, int model, const int32_t* domain, int32_t M, int dir, int32_t* tiling,
+                       int32_t* heights);
+
+/* Closure of a height field over the domain's directed unit edges, in place: dir = +1 gives
+ * the largest field below `heights` with h(v) <= h(u) + 1 along every "+1" edge u -> v
+ * (dir = -1 the smallest above it along the "-1" edges).  INT32_MAX marks unset vertices.
+ * Vertices with a nonzero `fixed` entry (NULL: none) may not move: RT_ERR_UNTILEABLE if they
+ * would.  Projects a predicted height field (rounded to the lattice's residue classes) onto
+ * admissible ones.  *launches (may be NULL) receives the relaxation launches used.  No
+ * counterpart in the reference (its construction runs on the host). */
+int rt_height_closure(rt_ctx* ctx, int model, const int32_t* domain, int32_t M, int dir, int32_t* heights,
+                      const uint8_t* fixed, int32_t* launches);
+
 /* ---- device-resident chains (benchmarks, CFTP, batching) --------------------------- */
 
 /* Upload a tiling and split it into colour classes on the device.  rparams as in the

@@ -55,6 +55,8 @@ SIGNATURES = {
     "rt_lozenge_walk": (C.c_int, [_P, C.c_int, C.c_double, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_recttri_walk": (C.c_int, [_P, _FP, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_tridimer_walk": (C.c_int, [_P, _I32P, C.c_int32, C.c_int64, C.c_int64]),
+    "rt_extremal_tiling": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, C.c_int, _I32P, _I32P]),
+    "rt_height_closure": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, C.c_int, _I32P, C.POINTER(C.c_uint8), _I32P]),
     "rt_domino_cftp": (C.c_int, [_P, C.c_int, _DP, _I32P, _I32P, C.c_int32, C.c_int64, C.c_int, C.c_int64, C.c_int32,
                                  _I32P, _I64P, _I32P]),
     "rt_chain_create": (C.c_int, [_P, C.c_int, C.c_int, _DP, C.c_int32, _I32P, C.c_int32, C.POINTER(_P)]),
@@ -240,6 +242,27 @@ class Context:
                                         N, seed, mode, first_steps, max_restarts, _i32(out), C.byref(tot),
                                         C.byref(rs)))
         return out, tot.value, rs.value
+
+    def extremal_tiling(self, model: int, domain: np.ndarray, maximal: bool = True):
+        """(tiling, heights) of the maximal / minimal tiling, built on the device."""
+        d = np.ascontiguousarray(domain, dtype=np.int32)
+        M = int(round(np.sqrt(d.size / (2 if model == RT_LOZENGE else 1))))
+        t = np.zeros((M + 1) ** 2, np.int32)
+        h = np.zeros((M + 1) ** 2, np.int32)
+        _check(self._lib.rt_extremal_tiling(self.h, model, _i32(d), M, 1 if maximal else -1, _i32(t), _i32(h)))
+        return t, h
+
+    def height_closure(self, model: int, domain: np.ndarray, heights: np.ndarray, direction: int = 1, fixed=None):
+        """(closed heights, relaxation launches) of rt_height_closure."""
+        d = np.ascontiguousarray(domain, dtype=np.int32)
+        M = int(round(np.sqrt(d.size / (2 if model == RT_LOZENGE else 1))))
+        h = np.ascontiguousarray(heights, dtype=np.int32).copy()
+        fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        nl = C.c_int32()
+        _check(self._lib.rt_height_closure(self.h, model, _i32(d), M, direction, _i32(h),
+                                           None if fx is None else fx.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                           C.byref(nl)))
+        return h, nl.value
 
     def chain(self, model: int, rule: int, tiling: np.ndarray, rparams=None) -> "Chain":
         return Chain(self, model, rule, tiling, rparams)

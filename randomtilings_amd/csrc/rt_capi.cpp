@@ -1082,6 +1082,61 @@ int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* 
     return rc;
 }
 
+// ---- height functions on the device -----------------------------------------------------
+
+static int domain_of(int model, const int32_t* domain, int32_t M, std::vector<int>& d) {
+    if (model != RT_DOMINO && model != RT_LOZENGE)
+        return fail(RT_ERR_ARG, "height functions are defined for RT_DOMINO and RT_LOZENGE domains");
+    if (!domain || M < 2) return fail(RT_ERR_ARG, "bad domain");
+    const size_t n = (size_t)M * (model == RT_LOZENGE ? 2 * M : M);
+    d.assign(domain, domain + n);
+    return RT_OK;
+}
+
+int rt_height_closure(rt_ctx* ctx, int model, const int32_t* domain, int32_t M, int dir, int32_t* heights,
+                      const uint8_t* fixed, int32_t* launches) {
+    if (!ctx || !heights || (dir != 1 && dir != -1)) return fail(RT_ERR_ARG, "rt_height_closure: bad arguments");
+    std::vector<int> d;
+    int rc = domain_of(model, domain, M, d);
+    if (rc) return rc;
+    const size_t n = (size_t)(M + 1) * (M + 1);
+    std::vector<int32_t> h(heights, heights + n);
+    std::vector<uint8_t> fx(n, 0);
+    if (fixed)
+        for (size_t k = 0; k < n; k++) fx[k] = fixed[k] != 0;
+    std::string err;
+    int nl = 0;
+    rc = height_closure_device(ctx->device, ctx->stream, model, d, dir, h, fx, nullptr, &nl, err);
+    if (launches) *launches = nl;
+    if (rc == RT_ERR_UNTILEABLE) return fail(rc, "the closure moves a fixed vertex: no admissible height function");
+    if (rc) return fail(rc, "rt_height_closure: %s", err.c_str());
+    std::memcpy(heights, h.data(), n * sizeof(int32_t));
+    return RT_OK;
+}
+
+int rt_extremal_tiling(rt_ctx* ctx, int model, const int32_t* domain, int32_t M, int dir, int32_t* tiling,
+                       int32_t* heights) {
+    if (!ctx || !tiling || (dir != 1 && dir != -1)) return fail(RT_ERR_ARG, "rt_extremal_tiling: bad arguments");
+    std::vector<int> d;
+    int rc = domain_of(model, domain, M, d);
+    if (rc) return rc;
+    const size_t n = (size_t)(M + 1) * (M + 1);
+    std::memset(tiling, 0, n * sizeof(int32_t));
+    std::vector<int32_t> h;
+    std::vector<uint8_t> fx;
+    if (!boundary_heights_host(model, d, h, fx))
+        return fail(RT_ERR_UNTILEABLE, "the region is not tileable (its boundary does not close)");
+    std::vector<int32_t> t;
+    std::string err;
+    int nl = 0;
+    rc = height_closure_device(ctx->device, ctx->stream, model, d, dir, h, fx, &t, &nl, err);
+    if (rc == RT_ERR_UNTILEABLE) return fail(rc, "the region is not tileable (the closure moves a boundary height)");
+    if (rc) return fail(rc, "rt_extremal_tiling: %s", err.c_str());
+    std::memcpy(tiling, t.data(), n * sizeof(int32_t));
+    if (heights) std::memcpy(heights, h.data(), n * sizeof(int32_t));
+    return RT_OK;
+}
+
 int rt_domino_walk_epochs(rt_ctx* ctx, int rule, const double* rparams, int32_t* tiling, int32_t N,
                           const int64_t* steps, const int64_t* seeds, int32_t K) {
     if (K < 0 || (K > 0 && (!steps || !seeds))) return fail(RT_ERR_ARG, "rt_domino_walk_epochs: bad epochs");

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+#include <vector>
+
 #include "../../include/rt_hip.h"
 #include "rt_device.hpp"
 
@@ -46,6 +49,13 @@ hipError_t launch_init_gens(int64_t G, uint32_t seed, const uint4* params, uint3
 hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t st);
 hipError_t launch_pair_differs(const uint16_t* a, int64_t n, uint32_t* count, hipStream_t st);
 hipError_t launch_stepwise(int model, const StepwiseArgs& a, hipStream_t st);
+
+// Height functions (rt_heights.hip).  heights: initial values (INT_MAX = unset), replaced by
+// the closure; fixed: vertices that may not move.  With `tiling`, also reads the tiling off.
+int height_closure_device(int device, hipStream_t s, int model, const std::vector<int>& d, int dir,
+                          std::vector<int32_t>& heights, const std::vector<uint8_t>& fixed, std::vector<int32_t>* tiling,
+                          int* launches, std::string& err);
+bool boundary_heights_host(int model, const std::vector<int>& d, std::vector<int32_t>& h, std::vector<uint8_t>& fixed);
 
 int persist_config_count(int model);
 bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt);

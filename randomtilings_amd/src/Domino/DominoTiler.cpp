@@ -11,37 +11,14 @@
 #include <stdexcept>
 
 #include "../common/heights.hpp"
+#include "../common/lattice.hpp"
 #include "../common/rtapi.hpp"
 #include "rt_hip.h"
 
 namespace {
 
-// edge to (i+1,j), (i-1,j), (i,j+1), (i,j-1) carries bit 2, 1, 8, 4
-constexpr int kBit[4] = {2, 1, 8, 4};
-
-struct DominoGeom {
-    static constexpr int DEG = 4;
-    static constexpr int di[4] = {1, -1, 0, 0};
-    static constexpr int dj[4] = {0, 0, 1, -1};
-    const domain& d;
-    int N, M;  // vertices per side, faces per side
-    DominoGeom(const domain& dd, int n) : d(dd), N(n), M(n - 1) {}
-    bool face(int fi, int fj) const { return fi >= 0 && fj >= 0 && fi < M && fj < M && d[(size_t)fi * M + fj] != 0; }
-    // the two squares beside the edge from vertex (i,j) in direction nb
-    void faces(int i, int j, int nb, bool& a, bool& b) const {
-        switch (nb) {
-            case 0: a = face(i, j - 1); b = face(i, j); break;          // down
-            case 1: a = face(i - 1, j - 1); b = face(i - 1, j); break;  // up
-            case 2: a = face(i - 1, j); b = face(i, j); break;          // right
-            default: a = face(i - 1, j - 1); b = face(i, j - 1); break; // left
-        }
-    }
-    // Crossing a black-on-left edge raises the height by one: from a vertex with i+j even the
-    // horizontal steps go up and the vertical ones down; the other way round when i+j is odd.
-    int delta(int i, int j, int nb) const { return (nb > 1 ? 1 : -1) * (((i + j) & 1) == 0 ? 1 : -1); }
-};
-constexpr int DominoGeom::di[4];
-constexpr int DominoGeom::dj[4];
+using rtlat::DominoGeom;
+constexpr const int* kBit = DominoGeom::bit;
 
 tiling thurston(const domain& d, int dir, const char* name) {
     const std::vector<int> vertices = DominoTiler::DomainToVertices(d);
@@ -221,24 +198,7 @@ domain DominoTiler::TilingToDomain(const tiling& t) {
     return d;
 }
 
-std::vector<int> DominoTiler::DomainToVertices(const domain& d) {
-    const int M = rtapi::side(d.size());
-    const int N = M + 1;
-    auto at = [&](int fi, int fj) {
-        fi = std::min(std::max(fi, 0), M - 1);
-        fj = std::min(std::max(fj, 0), M - 1);
-        return d[(size_t)fi * M + fj] != 0;
-    };
-    std::vector<int> v((size_t)N * N, 0);
-    for (int i = 0; i < N; ++i)
-        for (int j = 0; j < N; ++j) {
-            const bool a = at(i - 1, j - 1), b = at(i - 1, j), c = at(i, j - 1), e = at(i, j);
-            const bool border = (i == 0 || j == 0 || i == M || j == M);
-            if (!border && a && b && c && e) v[(size_t)i * N + j] = 1;
-            else if (a || b || c || e) v[(size_t)i * N + j] = 2;
-        }
-    return v;
-}
+std::vector<int> DominoTiler::DomainToVertices(const domain& d) { return rtlat::domino_vertices(d); }
 
 // ---- SVG --------------------------------------------------------------------------------
 

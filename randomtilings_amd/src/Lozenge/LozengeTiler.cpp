@@ -9,43 +9,16 @@
 #include <stdexcept>
 
 #include "../common/heights.hpp"
+#include "../common/lattice.hpp"
 #include "../common/rtapi.hpp"
 #include "rt_hip.h"
 
 namespace {
 
-// neighbour k of vertex (i,j): offset, the height change along an uncrossed edge, and the
-// state bit of a crossed edge
-constexpr int kDi[6] = {0, -1, -1, 0, 1, 1};
-constexpr int kDj[6] = {1, 1, 0, -1, -1, 0};
-constexpr int kDh[6] = {-1, 1, -1, 1, -1, 1};
-constexpr int kBit[6] = {8, 2, 1, 4, 16, 32};
-// the two triangles beside edge k, as (row offset, column offset from 2j) in the M x 2M domain
-constexpr int kFace[6][2][2] = {{{0, 0}, {-1, 1}},   {{-1, 1}, {-1, 0}}, {{-1, 0}, {-1, -1}},
-                                {{-1, -1}, {0, -2}}, {{0, -2}, {0, -1}}, {{0, -1}, {0, 0}}};
-
-// Flat index into the M x 2M domain, the way the reference addresses it: a column offset
-// of -1/-2 at j = 0 reaches the end of the previous row.  Out-of-array reads count as empty.
-inline bool face_flat(const domain& d, int M, long r, long c) {
-    const long k = r * 2 * M + c;
-    return k >= 0 && k < (long)d.size() && d[(size_t)k] != 0;
-}
-
-struct LozGeom {
-    static constexpr int DEG = 6;
-    static constexpr int di[6] = {0, -1, -1, 0, 1, 1};
-    static constexpr int dj[6] = {1, 1, 0, -1, -1, 0};
-    const domain& d;
-    int N, M;
-    LozGeom(const domain& dd, int n) : d(dd), N(n), M(n - 1) {}
-    void faces(int i, int j, int nb, bool& a, bool& b) const {
-        a = face_flat(d, M, i + kFace[nb][0][0], 2L * j + kFace[nb][0][1]);
-        b = face_flat(d, M, i + kFace[nb][1][0], 2L * j + kFace[nb][1][1]);
-    }
-    int delta(int, int, int nb) const { return kDh[nb]; }
-};
-constexpr int LozGeom::di[6];
-constexpr int LozGeom::dj[6];
+using rtlat::LozGeom;
+constexpr const int* kDi = LozGeom::di;
+constexpr const int* kDj = LozGeom::dj;
+constexpr const int* kBit = LozGeom::bit;
 
 tiling thurston(const domain& d, int dir, const char* name) {
     const std::vector<int> vertices = LozengeTiler::DomainToVertices(d);
@@ -120,21 +93,7 @@ domain LozengeTiler::AlmostHexDomain(int N, int M) {
     return d;
 }
 
-std::vector<int> LozengeTiler::DomainToVertices(const domain& d) {
-    const int M = (int)std::lround(std::sqrt(d.size() / 2.0));
-    const int N = M + 1;
-    std::vector<int> v((size_t)N * N, 0);
-    auto f = [&](int r, int c) { return d[(size_t)r * 2 * M + c] != 0; };
-    for (int i = 1; i < M; ++i)
-        for (int j = 1; j < M; ++j) {
-            // the six triangles around vertex (i,j)
-            const bool a = f(i, 2 * j), b = f(i, 2 * j - 1), c = f(i, 2 * j - 2);
-            const bool e = f(i - 1, 2 * j), g = f(i - 1, 2 * j + 1), h = f(i - 1, 2 * j - 1);
-            if (a && b && c && e && g && h) v[(size_t)i * N + j] = 1;
-            else if (a || b || c || e || g || h) v[(size_t)i * N + j] = 2;
-        }
-    return v;
-}
+std::vector<int> LozengeTiler::DomainToVertices(const domain& d) { return rtlat::lozenge_vertices(d); }
 
 domain LozengeTiler::TilingToDomain(const tiling& t) {
     const int N = rtapi::side(t.size());
