@@ -72,6 +72,7 @@ extern "C" {
 
 typedef struct rt_ctx rt_ctx;
 typedef struct rt_chain rt_chain;
+typedef struct rt_batch rt_batch;
 
 /* Thread-local description of the last error on this thread. */
 const char* rt_last_error(void);
@@ -233,6 +234,26 @@ int64_t rt_chain_state_bytes(rt_chain* ch);
  * used (-1 for the stepwise path). */
 int64_t rt_chain_last_launches(rt_chain* ch);
 int32_t rt_chain_last_config(rt_chain* ch);
+
+/* ---- batches of independent chains (no counterpart in the reference: its callers loop
+ * over tilers) ---------------------------------------------------------------------------
+ * A batch runs many seeded chains of one context together: the domino, lozenge and
+ * rectangle-triangle chains of each model advance in lockstep, one blocked launch per
+ * block of h steps covering the tiles of every chain of the model (a chain-index table
+ * maps blocks to chains); with use_graph the launch of one block and its step-counter
+ * update are captured once as a HIP graph and replayed.  Triangle-dimer chains run their
+ * own launches.  After rt_batch_run(b, k) every chain is exactly where rt_chain_run(ch, k)
+ * would have left it.  Chains stay owned by the caller and must outlive the batch. */
+int rt_batch_create(rt_ctx* ctx, rt_batch** out);
+int rt_batch_destroy(rt_batch* b);
+int rt_batch_add(rt_batch* b, rt_chain* ch);
+int rt_batch_run(rt_batch* b, int64_t steps, int use_graph);
+int rt_batch_sync(rt_batch* b);
+/* GPU milliseconds of the most recent rt_batch_run (HIP events on the context stream). */
+int rt_batch_elapsed_ms(rt_batch* b, double* ms);
+/* Kernel launches (graph replays included) and graphs instantiated by the last run. */
+int64_t rt_batch_last_launches(rt_batch* b);
+int32_t rt_batch_last_graphs(rt_batch* b);
 
 /* Persistent tile configurations compiled for `model` (no device needed): count, and the
  * rows, columns, halo width and threads per workgroup of configuration idx. */

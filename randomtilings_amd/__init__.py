@@ -76,6 +76,14 @@ SIGNATURES = {
     "rt_chain_last_config": (C.c_int32, [_P]),
     "rt_chain_tile_configs": (C.c_int32, [_P]),
     "rt_chain_tile_config_info": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
+    "rt_batch_create": (C.c_int, [_P, C.POINTER(_P)]),
+    "rt_batch_destroy": (C.c_int, [_P]),
+    "rt_batch_add": (C.c_int, [_P, _P]),
+    "rt_batch_run": (C.c_int, [_P, C.c_int64, C.c_int]),
+    "rt_batch_sync": (C.c_int, [_P]),
+    "rt_batch_elapsed_ms": (C.c_int, [_P, _DP]),
+    "rt_batch_last_launches": (C.c_int64, [_P]),
+    "rt_batch_last_graphs": (C.c_int32, [_P]),
 }
 
 _lib = None
@@ -343,3 +351,49 @@ class Chain:
             _check(self._lib.rt_chain_tile_config_info(self.h, i, *[C.byref(x) for x in v]))
             out.append(tuple(x.value for x in v))
         return out
+
+
+class Batch:
+    """Many chains of one context run together (rt_batch_*)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        h = _P()
+        _check(self._lib.rt_batch_create(ctx.h, C.byref(h)))
+        self.h = h
+        self.chains = []
+
+    def add(self, ch: Chain) -> None:
+        _check(self._lib.rt_batch_add(self.h, ch.h))
+        self.chains.append(ch)
+
+    def run(self, steps: int, graph: bool = True) -> None:
+        _check(self._lib.rt_batch_run(self.h, steps, 1 if graph else 0))
+
+    def sync(self) -> None:
+        _check(self._lib.rt_batch_sync(self.h))
+
+    def elapsed_ms(self) -> float:
+        v = C.c_double()
+        _check(self._lib.rt_batch_elapsed_ms(self.h, C.byref(v)))
+        return v.value
+
+    @property
+    def last_launches(self) -> int:
+        return int(self._lib.rt_batch_last_launches(self.h))
+
+    @property
+    def last_graphs(self) -> int:
+        return int(self._lib.rt_batch_last_graphs(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            self._lib.rt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -101,6 +101,19 @@ struct rt_chain {
 static std::mutex g_tune_mu;
 static std::map<std::tuple<int, int, bool>, int> g_tuned;
 
+// Host colour schedule of the planar models, drawn exactly as the reference's Walk loop
+// draws it.
+static void draw_schedule(rt_chain* ch, int64_t steps, uint8_t* out) {
+    for (int64_t s = 0; s < steps; s++) {
+        if (ch->model == RT_DOMINO || ch->model == kDominoPair) {
+            const int r = ch->mt() % 2;  // DominoTiler.cpp:35-36: r==1 rotates B (class 0)
+            out[s] = (r == 1) ? 0 : 1;
+        } else {
+            out[s] = (uint8_t)ch->d3(ch->lcg);  // LozengeTiler.cpp:50-56, RectTriangleTiler.cpp:47-50
+        }
+    }
+}
+
 // ---- triangle dimers (TriangleDimerTiler::Walk, TriangleDimerTiler.cpp:86-483) ----------
 // The chain state is M x M cells of eight 16-bit move states (M = N/2), derived on the
 // device from the uploaded tiling; the host draws each step's move exactly as Walk does.
@@ -820,15 +833,7 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         ch->last_launches = 0;
         return steps == 0 ? RT_OK : td_run(ch, steps);
     }
-    // host colour schedule, drawn exactly as the reference's Walk loop draws it
-    for (int64_t s = 0; s < steps; s++) {
-        if (ch->model == RT_DOMINO || ch->model == kDominoPair) {
-            const int r = ch->mt() % 2;  // DominoTiler.cpp:35-36: r==1 rotates B (class 0)
-            ch->hsched[s] = (r == 1) ? 0 : 1;
-        } else {
-            ch->hsched[s] = (uint8_t)ch->d3(ch->lcg);
-        }
-    }
+    draw_schedule(ch, steps, ch->hsched.data());
     ch->timed = false;
     ch->last_launches = 0;
     if (steps == 0) return RT_OK;
@@ -983,6 +988,259 @@ int rt_chain_tile_config_info(rt_chain* ch, int32_t idx, int32_t* rows, int32_t*
     if (!ch) return fail(RT_ERR_ARG, "null argument");
     return rt_tile_config(ch->model, idx, rows, cols, halo, threads);
 }
+
+// ---- batches ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct rt_batch {
+    rt_ctx* ctx = nullptr;
+    std::vector<rt_chain*> chains;
+    uint8_t* dsched = nullptr;
+    size_t dsched_cap = 0;
+    BatchEntry* dent = nullptr;
+    size_t dent_cap = 0;
+    int64_t* ctr = nullptr;  // [0] step counter of the current group, [1..] scratch
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_up = nullptr;
+    bool up_pending = false;
+    std::vector<uint8_t> hsched;
+    std::vector<BatchEntry> hent;
+    std::vector<rt_chain*> hent_chain;  // the chain of each entry
+    int64_t launches = 0;
+    int graphs = 0;
+    bool timed = false;
+};
+
+extern "C" {
+
+int rt_batch_create(rt_ctx* ctx, rt_batch** out) {
+    if (!ctx || !out) return fail(RT_ERR_ARG, "rt_batch_create: null argument");
+    rt_batch* b = new rt_batch();
+    b->ctx = ctx;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipMalloc(&b->ctr, 64);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_up, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        rt_batch_destroy(b);
+        return fail(RT_ERR_HIP, "rt_batch_create: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return RT_OK;
+}
+
+int rt_batch_destroy(rt_batch* b) {
+    if (!b) return RT_OK;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    if (b->dsched) hipFree(b->dsched);
+    if (b->dent) hipFree(b->dent);
+    if (b->ctr) hipFree(b->ctr);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->ev_up) hipEventDestroy(b->ev_up);
+    delete b;
+    return RT_OK;
+}
+
+int rt_batch_add(rt_batch* b, rt_chain* ch) {
+    if (!b || !ch) return fail(RT_ERR_ARG, "rt_batch_add: null argument");
+    if (ch->ctx != b->ctx) return fail(RT_ERR_ARG, "rt_batch_add: the chain belongs to another context");
+    if (std::find(b->chains.begin(), b->chains.end(), ch) != b->chains.end())
+        return fail(RT_ERR_ARG, "rt_batch_add: chain already in the batch");
+    b->chains.push_back(ch);
+    return RT_OK;
+}
+
+int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
+    if (!b || steps < 0) return fail(RT_ERR_ARG, "rt_batch_run: bad arguments");
+    rt_ctx* c = b->ctx;
+    for (rt_chain* ch : b->chains)
+        if (!ch->seeded) return fail(RT_ERR_ARG, "rt_batch_run: every chain must be seeded");
+    HIPCHK(hipSetDevice(c->device));
+    if (b->up_pending) {
+        HIPCHK(hipEventSynchronize(b->ev_up));
+        b->up_pending = false;
+    }
+    b->launches = 0;
+    b->graphs = 0;
+    b->timed = false;
+    if (steps == 0 || b->chains.empty()) return RT_OK;
+    // planar chains are grouped by model and run together; triangle-dimer chains run their
+    // own launches
+    const int models[] = {RT_DOMINO, RT_LOZENGE, RT_RECTTRI, kDominoPair};
+    // per-chain schedule segments: 16-byte aligned with 16 bytes of slack, as a chain's own
+    // schedule buffer (kernels fetch a block's colours as 8-byte words)
+    const size_t seg = (((size_t)steps + 16) + 15) / 16 * 16;
+    std::vector<rt_chain*> planar;
+    for (rt_chain* ch : b->chains)
+        if (ch->model != RT_TRIDIMER) planar.push_back(ch);
+    b->hsched.assign(seg * planar.size(), 0);
+    for (size_t k = 0; k < planar.size(); k++) draw_schedule(planar[k], steps, b->hsched.data() + k * seg);
+    if (b->dsched_cap < b->hsched.size()) {
+        if (b->dsched) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipFree(b->dsched));
+            b->dsched = nullptr;
+        }
+        HIPCHK(hipMalloc(&b->dsched, b->hsched.size()));
+        b->dsched_cap = b->hsched.size();
+    }
+    struct Group {
+        int model, cfg, h;
+        size_t first, count;
+        int blocks;
+    };
+    std::vector<Group> groups;
+    b->hent.clear();
+    b->hent_chain.clear();
+    for (int m : models) {
+        Group g{m, -1, 0, b->hent.size(), 0, 0};
+        rt_chain* largest = nullptr;
+        for (rt_chain* ch : planar)
+            if (ch->model == m && (!largest || ch->G > largest->G)) largest = ch;
+        if (!largest) continue;
+        int nb = 0, tcn = 0;
+        bool res = false;
+        g.cfg = choose_persist_cfg(largest, &nb, &tcn, &res);
+        if (g.cfg < 0) return fail(RT_ERR_ARG, "rt_batch_run: no tile configuration for model %d", m);
+        int tr, tc, h, nt;
+        persist_config(m, g.cfg, &tr, &tc, &h, &nt);
+        g.h = h;
+        for (size_t k = 0; k < planar.size(); k++) {
+            rt_chain* ch = planar[k];
+            if (ch->model != m) continue;
+            BatchEntry e{};
+            e.rp = ch->rp;
+            e.cls[0] = ch->cls[0];
+            e.cls[1] = ch->cls[1];
+            e.gens[0] = ch->gens[0];
+            e.gens[1] = ch->gens[1];
+            e.sched = b->dsched + k * seg;
+            e.N = ch->N;
+            e.W = ch->W;
+            e.P = ch->P;
+            e.tiles_c = (ch->W + tc - 1) / tc;
+            e.buf0 = ch->cur;
+            e.g0 = ch->gcur;
+            e.tile0 = g.blocks;
+            e.ntiles = ((ch->N + tr - 1) / tr) * e.tiles_c;
+            g.blocks += e.ntiles;
+            b->hent.push_back(e);
+            b->hent_chain.push_back(ch);
+            g.count++;
+        }
+        groups.push_back(g);
+    }
+    if (b->dent_cap < b->hent.size()) {
+        if (b->dent) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipFree(b->dent));
+            b->dent = nullptr;
+        }
+        HIPCHK(hipMalloc(&b->dent, sizeof(BatchEntry) * std::max<size_t>(b->hent.size(), 1)));
+        b->dent_cap = b->hent.size();
+    }
+    if (!planar.empty()) {
+        HIPCHK(hipMemcpyAsync(b->dsched, b->hsched.data(), b->hsched.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b->dent, b->hent.data(), sizeof(BatchEntry) * b->hent.size(), hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipEventRecord(b->ev_up, c->stream));
+        b->up_pending = true;
+    }
+    HIPCHK(hipEventRecord(b->ev0, c->stream));
+    for (const Group& g : groups) {
+        PersistArgs a{};
+        a.params = c->uniform ? nullptr : c->params;
+        a.m1 = c->um1;
+        a.m2 = c->um2;
+        a.tm = c->utm;
+        a.batch = b->dent + g.first;
+        a.nbatch = (int)g.count;
+        a.block_h = g.h;
+        a.step_ctr = b->ctr;
+        a.err = (int*)(b->ctr + 1);
+        a.spin_limit = 1;
+        const int64_t full = steps / g.h, rest = steps % g.h;
+        HIPCHK(launch_fill_u32((uint32_t*)b->ctr, 4, 0u, c->stream));
+        a.nsteps = g.h;
+        if (use_graph && full > 0) {
+            // one round (h steps of every chain of the group, then the step counter) captured
+            // once and replayed
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e1 = launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream);
+            hipError_t e2 = launch_add_i64(b->ctr, g.h, c->stream);
+            hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
+            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+                if (graph) hipGraphDestroy(graph);
+                return fail(RT_ERR_HIP, "rt_batch_run: graph capture failed");
+            }
+            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            for (int64_t r = 0; r < full && e == hipSuccess; r++) e = hipGraphLaunch(exec, c->stream);
+            if (exec) hipGraphExecDestroy(exec);
+            hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(RT_ERR_HIP, "rt_batch_run: graph replay: %s", hipGetErrorString(e));
+            b->launches += full;
+            b->graphs++;
+        } else {
+            for (int64_t r = 0; r < full; r++) {
+                HIPCHK(launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream));
+                HIPCHK(launch_add_i64(b->ctr, g.h, c->stream));
+            }
+            b->launches += full;
+        }
+        if (rest > 0) {
+            a.nsteps = rest;
+            HIPCHK(launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream));
+            b->launches++;
+        }
+        const int64_t rounds = full + (rest > 0 ? 1 : 0);
+        for (size_t k = g.first; k < g.first + g.count; k++) {
+            // each chain of the group advanced by `rounds` copy flips
+            rt_chain* ch = b->hent_chain[k];
+            if (rounds & 1) {
+                ch->cur ^= 1;
+                ch->gcur ^= 1;
+            }
+            ch->last_launches = rounds;
+            ch->last_cfg = g.cfg;
+            ch->timed = false;
+        }
+    }
+    for (rt_chain* ch : b->chains)
+        if (ch->model == RT_TRIDIMER) {
+            const int rc = rt_chain_run(ch, steps);
+            if (rc) return rc;
+            b->launches += ch->last_launches;
+        }
+    HIPCHK(hipEventRecord(b->ev1, c->stream));
+    b->timed = true;
+    return RT_OK;
+}
+
+int rt_batch_sync(rt_batch* b) {
+    if (!b) return fail(RT_ERR_ARG, "rt_batch_sync: null batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    return RT_OK;
+}
+
+int rt_batch_elapsed_ms(rt_batch* b, double* ms) {
+    if (!b || !ms) return fail(RT_ERR_ARG, "rt_batch_elapsed_ms: null argument");
+    *ms = 0.0;
+    if (!b->timed) return RT_OK;
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, b->ev0, b->ev1));
+    *ms = f;
+    return RT_OK;
+}
+
+int64_t rt_batch_last_launches(rt_batch* b) { return b ? b->launches : 0; }
+int32_t rt_batch_last_graphs(rt_batch* b) { return b ? b->graphs : 0; }
 
 // ---- Walk entry points ----------------------------------------------------------------
 

@@ -26,6 +26,17 @@ struct StepwiseArgs {
     int64_t nsteps;
 };
 
+// One chain of a batched launch: the blocks [tile0, tile0 + ntiles) of the grid run its tiles.
+struct BatchEntry {
+    RuleParams rp;
+    void* cls[2];
+    uint4* gens[2];
+    const uint8_t* sched;  // the chain's colour schedule for the whole batched run
+    int N, W, P, tiles_c;
+    int buf0, g0;  // copies holding the state / generators when the run starts
+    int tile0, ntiles;
+};
+
 struct PersistArgs {
     RuleParams rp;
     void* cls[2];  // two copies of the NC contiguous class arrays
@@ -42,7 +53,41 @@ struct PersistArgs {
     uint32_t epoch0;  // block epochs completed before this launch
     int* err;
     uint32_t spin_limit;
+    // batched runs (blocked path): every block looks up its chain; the steps done so far in
+    // the run are read from step_ctr (a device counter, so one captured round replays as is)
+    const BatchEntry* batch;
+    int nbatch;
+    int block_h;
+    const int64_t* step_ctr;
 };
+
+// Resolve a batched block: the chain owning blockIdx.x, with the copies and schedule offset
+// of this round.
+__device__ __forceinline__ void resolve_batch(PersistArgs& a, int& tile, int& ntiles) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = a.nbatch - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.batch[mid].tile0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchEntry& e = a.batch[lo];
+    const int64_t s0 = *a.step_ctr;
+    const int par = (int)((s0 / a.block_h) & 1);
+    a.rp = e.rp;
+    a.cls[0] = e.cls[0];
+    a.cls[1] = e.cls[1];
+    a.buf = e.buf0 ^ par;
+    a.gen_in = e.gens[e.g0 ^ par];
+    a.gen_out = e.gens[e.g0 ^ par ^ 1];
+    a.sched = e.sched + s0;
+    a.N = e.N;
+    a.W = e.W;
+    a.P = e.P;
+    a.tiles_c = e.tiles_c;
+    tile = b - e.tile0;
+    ntiles = e.ntiles;
+}
 
 hipError_t launch_init_gens(int64_t G, uint32_t seed, const uint4* params, uint32_t m1, uint32_t m2, uint32_t tm,
                             uint4* gens, hipStream_t st);
@@ -61,5 +106,6 @@ int persist_config_count(int model);
 bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt);
 int persist_capacity(int model, int idx, bool uni, int device);
 hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st);
+hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t st);
 
 }  // namespace rt

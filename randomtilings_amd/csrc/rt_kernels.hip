@@ -56,6 +56,15 @@ __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
+__global__ void k_add_i64(int64_t* p, int64_t v) {
+    if (threadIdx.x == 0) *p += v;
+}
+
+hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_i64, dim3(1), dim3(64), 0, st, p, v);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
     return hipGetLastError();
@@ -149,7 +158,7 @@ hipError_t launch_stepwise(int model, const StepwiseArgs& a, hipStream_t st) {
 // across XCDs without writing back or invalidating whole L2s), rows being 16-byte aligned.
 
 template <class M, int TR, int TC, int H, int NT, bool UNI>
-__global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
+__device__ __forceinline__ void persist_body(const PersistArgs& a, const int tile, const int ntiles) {
     using T = typename M::T;
     constexpr int NC = M::NC;
     constexpr int ER = TR + 2 * H, EC = TC + 2 * H;
@@ -166,8 +175,7 @@ __global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
     const int N = a.N, W = a.W;
     const int PW = a.P * ES / 4;             // dwords per global row
     const size_t CSW = (size_t)N * PW;       // dwords per class
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tiles_r = gridDim.x / a.tiles_c;
+    const int tiles_r = ntiles / a.tiles_c;
     const int tr = tile / a.tiles_c, tc = tile - (tile / a.tiles_c) * a.tiles_c;
     const int r0 = tr * TR - H, c0 = tc * TC - H;  // global coordinates of LDS (0, 0)
     const int c0w = c0 * ES / 4;                   // ... in dwords (c0 is a whole dword)
@@ -313,6 +321,20 @@ __global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
     }
 }
 
+template <class M, int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
+    persist_body<M, TR, TC, H, NT, UNI>(a, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+}
+
+// batched launches (blocked path): every block runs one tile of one chain of the batch
+template <class M, int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist_batch(PersistArgs a_in) {
+    PersistArgs a = a_in;
+    int tile, ntiles;
+    resolve_batch(a, tile, ntiles);
+    persist_body<M, TR, TC, H, NT, UNI>(a, tile, ntiles);
+}
+
 // tile configurations per model: {TR, TC, H, NT, packed}; packed selects the quad kernel
 struct Cfg {
     int tr, tc, h, nt, packed;
@@ -368,18 +390,24 @@ bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt) {
 }
 
 template <class M, int TR, int TC, int H, int NT, int PACKED>
-static const void* kernel_ptr(bool uni) {
+static const void* kernel_ptr(bool uni, bool batch) {
     if constexpr (PACKED != 0) {
+        if (batch)
+            return uni ? (const void*)k_persist_loz4_batch<TR, TC, H, NT, true>
+                       : (const void*)k_persist_loz4_batch<TR, TC, H, NT, false>;
         return uni ? (const void*)k_persist_loz4<TR, TC, H, NT, true> : (const void*)k_persist_loz4<TR, TC, H, NT, false>;
     } else {
+        if (batch)
+            return uni ? (const void*)k_persist_batch<M, TR, TC, H, NT, true>
+                       : (const void*)k_persist_batch<M, TR, TC, H, NT, false>;
         return uni ? (const void*)k_persist<M, TR, TC, H, NT, true> : (const void*)k_persist<M, TR, TC, H, NT, false>;
     }
 }
 
-static const void* persist_kernel(int model, int idx, bool uni) {
+static const void* persist_kernel(int model, int idx, bool uni, bool batch = false) {
     int i = 0;
 #define RT_PICK(M_, a, b, c, d, e)                              \
-    if (i++ == idx) return kernel_ptr<M_, a, b, c, d, e>(uni);
+    if (i++ == idx) return kernel_ptr<M_, a, b, c, d, e>(uni, batch);
     switch (model) {
         case RT_DOMINO: {
 #define X(a, b, c, d, e) RT_PICK(DominoModel, a, b, c, d, e)
@@ -419,7 +447,7 @@ int persist_capacity(int model, int idx, bool uni, int device) {
 }
 
 hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st) {
-    const void* k = persist_kernel(model, idx, uni);
+    const void* k = persist_kernel(model, idx, uni, a.batch != nullptr);
     int tr, tc, h, nt;
     if (!k || !persist_config(model, idx, &tr, &tc, &h, &nt)) return hipErrorInvalidValue;
     void* args[] = {(void*)&a};

@@ -30,7 +30,7 @@ __device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, int sh) {
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>
-__global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
+__device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, const int ntiles) {
     constexpr int NC = 3;
     constexpr int ER = TR + 2 * H, EC = TC + 2 * H;
     static_assert(TC % 4 == 0 && H % 4 == 0, "quads must tile the rows");
@@ -44,8 +44,7 @@ __global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
     const int N = a.N, W = a.W;
     const int PW = a.P / 4;
     const size_t CSW = (size_t)N * PW;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tiles_r = gridDim.x / a.tiles_c;
+    const int tiles_r = ntiles / a.tiles_c;
     const int tr = tile / a.tiles_c, tc = tile - (tile / a.tiles_c) * a.tiles_c;
     const int r0 = tr * TR - H, c0 = tc * TC - H;
     const int c0w = c0 / 4;
@@ -259,6 +258,19 @@ __global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
             a.gen_out[id] = make_uint4(s0[g][b], s1[g][b], s2[g][b], s3[g][b]);
         }
     }
+}
+
+template <int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
+    loz4_body<TR, TC, H, NT, UNI>(a, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+}
+
+template <int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist_loz4_batch(PersistArgs a_in) {
+    PersistArgs a = a_in;
+    int tile, ntiles;
+    resolve_batch(a, tile, ntiles);
+    loz4_body<TR, TC, H, NT, UNI>(a, tile, ntiles);
 }
 
 }  // namespace rt

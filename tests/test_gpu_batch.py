@@ -1,0 +1,92 @@
+"""Batched runs (rt_batch_*): many chains of different models, sizes and rules advanced
+together — one blocked launch per block of h steps per model, optionally replayed from a
+captured HIP graph — end exactly where individual rt_chain_run calls end, run after run."""
+import numpy as np
+import pytest
+
+import oracle as O
+import randomtilings_amd as rt
+from randomtilings_amd import statics as S
+from randomtilings_amd.params import default_table
+
+pytestmark = pytest.mark.gpu
+BIG = 1 << 20
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def bctx():
+    c = rt.Context(0)
+    c.load_params(default_table(BIG))
+    yield c
+    c.close()
+
+
+def _specs(golden):
+    specs = []
+    for n in (3, 16, 64, 130):
+        d = S.call("domino.AztecDiamond", args=[n])
+        specs.append((rt.RT_DOMINO, rt.RT_DOMINO_CFTP, S.call("domino.MinTiling", d), None))
+    specs.append((rt.RT_DOMINO, rt.RT_DOMINO_WEIGHTED, golden["domino_aztec32_max"], [0.6, 0.4, 0.45, 0.55]))
+    specs.append((rt.RT_DOMINO, rt.RT_DOMINO_LAZY, golden["domino_rect10x10_min"], None))
+    for key, q in (("hex10_2_min", 0.9), ("hex50_0_min", 0.999), ("hex100_0_max", 1.2)):
+        specs.append((rt.RT_LOZENGE, rt.RT_LOZENGE_QREF, golden[f"lozenge_{key}"], [q]))
+    specs.append((rt.RT_LOZENGE, rt.RT_LOZENGE_UNIFORM, golden["lozenge_hex20_0_max"], [0.99]))
+    for key in ("minHex10", "slopeHex20"):
+        specs.append((rt.RT_RECTTRI, 0, golden[f"recttri_{key}"], None))
+    specs.append((rt.RT_TRIDIMER, 0, golden["tridimer_cone22_0"], None))
+    return specs
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_equals_individual_runs(bctx, golden, graph):
+    specs = _specs(golden)
+    solo, batched = [], []
+    b = rt.Batch(bctx)
+    try:
+        for k, (m, rule, t0, rp) in enumerate(specs):
+            a = bctx.chain(m, rule, t0, rp)
+            c = bctx.chain(m, rule, t0, rp)
+            a.seed(1000 + k)
+            c.seed(1000 + k)
+            solo.append(a)
+            batched.append(c)
+            b.add(c)
+        for steps in (37, 200, 1, 64):
+            say("batch", graph, steps)
+            b.run(steps, graph)
+            for a in solo:
+                a.run(steps)
+            b.sync()
+            for k, (a, c) in enumerate(zip(solo, batched)):
+                assert np.array_equal(a.read(), c.read()), (k, steps)
+        assert b.last_launches > 0
+        if graph:
+            assert b.last_graphs >= 3  # one per planar model
+    finally:
+        b.close()
+        for ch in solo + batched:
+            ch.close()
+
+
+def test_batch_matches_oracle(bctx, golden):
+    t0 = golden["lozenge_hex50_0_min"]
+    c = bctx.chain(rt.RT_LOZENGE, rt.RT_LOZENGE_QREF, t0, [0.999])
+    d = bctx.chain(rt.RT_DOMINO, rt.RT_DOMINO_CFTP, golden["domino_aztec16_min"])
+    b = rt.Batch(bctx)
+    try:
+        c.seed(5)
+        d.seed(6)
+        b.add(c)
+        b.add(d)
+        b.run(150)
+        b.run(91)
+        assert np.array_equal(c.read(), O.lozenge_walk(t0, 241, 5, default_table(BIG), rule=1, q=0.999))
+        assert np.array_equal(d.read(), O.domino_walk(golden["domino_aztec16_min"], 241, 6, default_table(BIG), rule=1))
+    finally:
+        b.close()
+        c.close()
+        d.close()
