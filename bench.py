@@ -47,12 +47,15 @@ def main() -> int:
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (to scale HBM byte counters)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c4", "tridimer", "closure"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "tridimer", "closure"],
                     help="c2 (default): lozenge hexagon 1000; c1: domino Aztec diamond 128, 10^5 steps; "
                          "c4: coupling from the past on an Aztec diamond (--aztec); tridimer: triangle-dimer cone")
     ap.add_argument("--aztec", type=int, default=256, help="Aztec diamond order of the c4 workload")
     ap.add_argument("--cone", type=int, default=1000, help="cone side of the tridimer workload")
     ap.add_argument("--check", action="store_true", help="c1: also compare the final tiling with the oracle")
+    ap.add_argument("--domains", type=int, default=64, help="c3: number of domains")
+    ap.add_argument("--sides", type=int, nargs=2, default=[64, 512], help="c3: domain side range")
+    ap.add_argument("--domain-steps", type=int, default=1000, help="c3: chain steps per domain")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -77,7 +80,8 @@ def main() -> int:
     from randomtilings_amd.params import default_table
 
     if args.workload != "c2":
-        return {"c1": bench_c1, "c4": bench_c4, "tridimer": bench_tridimer, "closure": bench_closure}[args.workload](args)
+        return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "tridimer": bench_tridimer,
+                "closure": bench_closure}[args.workload](args)
 
     d = statics.call("lozenge.AlmostHexDomain", args=[args.hex, 0])
     t0 = statics.call("lozenge.MinTiling", d)
@@ -321,6 +325,100 @@ def bench_tridimer(args) -> int:
                  extra)
     print(json.dumps(line), flush=True)
     ch.close()
+    ctx.close()
+    return 0
+
+
+def bench_c3(args) -> int:
+    """Batched plain chains: --domains domains drawn from --seed with sides uniform in --sides,
+    cycling Aztec diamonds, domino rectangles with random staircase notches (tileability
+    checked by the device max/min construction), triangle-dimer cones and rectangle-triangle
+    hexagons; every chain runs --domain-steps steps.  One batched run (blocked launches per
+    model, HIP-graph replay) against one rt_chain_run per domain, on identical chains."""
+    import randomtilings_amd as rt
+    from randomtilings_amd import statics
+    from randomtilings_amd.params import default_table
+
+    rng = np.random.default_rng(args.seed)
+    lo, hi = args.sides
+    ctx = rt.Context(0)
+    specs = []
+    for k in range(args.domains):
+        kind = ("aztec", "notched", "tridimer", "recttri")[k % 4]
+        side = int(rng.integers(lo, hi + 1))
+        if kind == "aztec":
+            d = statics.call("domino.AztecDiamond", args=[max(1, side // 2 - 3)])
+            specs.append((rt.RT_DOMINO, rt.RT_DOMINO_CFTP, ctx.extremal_tiling(rt.RT_DOMINO, d, False)[0], kind))
+        elif kind == "notched":
+            while True:
+                m = side - 6
+                d = statics.call("domino.Rectangle", args=[m, m])
+                sd = int(round(np.sqrt(d.size)))
+                d = d.reshape(sd, sd).copy()
+                cut = 0
+                for i in range(m):
+                    step = int(rng.integers(0, 3))
+                    cut = min(m - 1, cut + step) if i < m // 2 else max(0, cut - step)
+                    d[2 + i, 2:2 + 2 * (cut // 2)] = 0
+                try:
+                    t0 = ctx.extremal_tiling(rt.RT_DOMINO, d.ravel(), False)[0]
+                    break
+                except rt.RTError:
+                    side = max(lo, side - 1)
+            specs.append((rt.RT_DOMINO, rt.RT_DOMINO_CFTP, t0, kind))
+        elif kind == "tridimer":
+            n = max(2, side // 2 - 4)
+            while ((n + 1) * (n + 2) // 2) % 2:
+                n += 1
+            specs.append((rt.RT_TRIDIMER, 0, statics.call("tridimer.IceCreamCone", args=[n, 0]), kind))
+        else:
+            specs.append((rt.RT_RECTTRI, 0, statics.call("recttri.maxHex", args=[max(1, (side - 7) // 2)]), kind))
+    gmax = max(int(t.size) for _, _, t, _ in specs)
+    ctx.load_params(default_table(gmax))
+    steps = args.domain_steps
+
+    def make():
+        chs = [ctx.chain(m, r, t) for m, r, t, _ in specs]
+        for k, ch in enumerate(chs):
+            ch.seed(args.seed + k)
+        return chs
+
+    solo = make()
+    ctx.sync()
+    w0 = time.perf_counter()
+    for ch in solo:
+        ch.run(steps)
+    ctx.sync()
+    solo_s = time.perf_counter() - w0
+    batched = make()
+    b = rt.Batch(ctx)
+    for ch in batched:
+        b.add(ch)
+    ctx.sync()
+    w0 = time.perf_counter()
+    b.run(steps, True)
+    b.sync()
+    batch_s = time.perf_counter() - w0
+    att = sum(ch.attempts_per_step for ch in batched) * steps
+    same = all(np.array_equal(a.read(), c.read()) for a, c in zip(solo, batched))
+    kinds = {}
+    for (_, _, t, kind), ch in zip(specs, batched):
+        kinds.setdefault(kind, [0, 0])
+        kinds[kind][0] += 1
+        kinds[kind][1] += ch.attempts_per_step
+    extra = {"domains": args.domains, "sides": [lo, hi], "steps_per_domain": steps,
+             "attempts_per_step_all_domains": att // steps, "per_kind_domains_attempts": kinds,
+             "batch_seconds": batch_s, "one_run_per_domain_seconds": solo_s,
+             "one_run_per_domain_flips_per_s": att / solo_s, "batch_launches": b.last_launches,
+             "batch_graphs": b.last_graphs, "bit_identical_batch_vs_individual": bool(same)}
+    line = _line("aggregate flip attempts/s (batched domains)", att / batch_s, "flip attempts/s", args,
+                 batch_s * 1e3, "u8/i32/u16",
+                 f"{args.domains} domains, sides U[{lo},{hi}], Aztec/notched-rectangle dominoes, triangle-dimer cones, "
+                 f"rect-triangle hexagons, {steps} steps each", extra)
+    print(json.dumps(line), flush=True)
+    b.close()
+    for ch in solo + batched:
+        ch.close()
     ctx.close()
     return 0
 

@@ -241,9 +241,9 @@ int32_t rt_chain_last_config(rt_chain* ch);
  * rectangle-triangle chains of each model advance in lockstep, one blocked launch per
  * block of h steps covering the tiles of every chain of the model (a chain-index table
  * maps blocks to chains); with use_graph the launch of one block and its step-counter
- * update are captured once as a HIP graph and replayed.  Triangle-dimer chains run their
- * own launches.  After rt_batch_run(b, k) every chain is exactly where rt_chain_run(ch, k)
- * would have left it.  Chains stay owned by the caller and must outlive the batch. */
+ * update are captured once as a HIP graph and replayed.  Triangle-dimer chains advance
+ * together, one tiled launch per step (replayed the same way).  After rt_batch_run(b, k)
+ * every chain is exactly where rt_chain_run(ch, k) would have left it.  Chains stay owned by the caller and must outlive the batch. */
 int rt_batch_create(rt_ctx* ctx, rt_batch** out);
 int rt_batch_destroy(rt_batch* b);
 int rt_batch_add(rt_batch* b, rt_chain* ch);

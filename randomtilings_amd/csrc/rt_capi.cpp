@@ -149,9 +149,8 @@ static int td_create(rt_chain* ch, const int32_t* tiling) {
     return RT_OK;
 }
 
-static int td_run(rt_chain* ch, int64_t steps) {
-    rt_ctx* c = ch->ctx;
-    // TriangleDimerTiler.cpp:217-222, :291-292: type, r1, r2, then p1, p2 for butterflies
+// TriangleDimerTiler.cpp:217-222, :291-292: type, r1, r2, then p1, p2 for butterflies
+static void draw_td_moves(rt_chain* ch, int64_t steps, uint8_t* out) {
     for (int64_t s = 0; s < steps; s++) {
         const int type = ch->d3(ch->lcg), r1 = ch->d3(ch->lcg), r2 = ch->d2(ch->lcg);
         int p1 = 0, p2 = 0;
@@ -159,8 +158,13 @@ static int td_run(rt_chain* ch, int64_t steps) {
             p1 = ch->d3(ch->lcg);
             p2 = ch->d2(ch->lcg);
         }
-        ch->hsched[s] = td_move(type, r1, r2, p1, p2);
+        out[s] = td_move(type, r1, r2, p1, p2);
     }
+}
+
+static int td_run(rt_chain* ch, int64_t steps) {
+    rt_ctx* c = ch->ctx;
+    draw_td_moves(ch, steps, ch->hsched.data());
     TdArgs a{};
     a.params = c->uniform ? nullptr : c->params;
     a.m1 = c->um1;
@@ -1000,6 +1004,8 @@ struct rt_batch {
     size_t dsched_cap = 0;
     BatchEntry* dent = nullptr;
     size_t dent_cap = 0;
+    TdBatchEntry* dtd = nullptr;  // triangle-dimer chains
+    size_t dtd_cap = 0;
     int64_t* ctr = nullptr;  // [0] step counter of the current group, [1..] scratch
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_up = nullptr;
     bool up_pending = false;
@@ -1036,6 +1042,7 @@ int rt_batch_destroy(rt_batch* b) {
     hipStreamSynchronize(b->ctx->stream);
     if (b->dsched) hipFree(b->dsched);
     if (b->dent) hipFree(b->dent);
+    if (b->dtd) hipFree(b->dtd);
     if (b->ctr) hipFree(b->ctr);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
@@ -1073,11 +1080,11 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
     // per-chain schedule segments: 16-byte aligned with 16 bytes of slack, as a chain's own
     // schedule buffer (kernels fetch a block's colours as 8-byte words)
     const size_t seg = (((size_t)steps + 16) + 15) / 16 * 16;
-    std::vector<rt_chain*> planar;
-    for (rt_chain* ch : b->chains)
-        if (ch->model != RT_TRIDIMER) planar.push_back(ch);
-    b->hsched.assign(seg * planar.size(), 0);
+    std::vector<rt_chain*> planar, tdc;
+    for (rt_chain* ch : b->chains) (ch->model == RT_TRIDIMER ? tdc : planar).push_back(ch);
+    b->hsched.assign(seg * (planar.size() + tdc.size()), 0);
     for (size_t k = 0; k < planar.size(); k++) draw_schedule(planar[k], steps, b->hsched.data() + k * seg);
+    for (size_t k = 0; k < tdc.size(); k++) draw_td_moves(tdc[k], steps, b->hsched.data() + (planar.size() + k) * seg);
     if (b->dsched_cap < b->hsched.size()) {
         if (b->dsched) {
             HIPCHK(hipStreamSynchronize(c->stream));
@@ -1142,13 +1149,54 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         HIPCHK(hipMalloc(&b->dent, sizeof(BatchEntry) * std::max<size_t>(b->hent.size(), 1)));
         b->dent_cap = b->hent.size();
     }
-    if (!planar.empty()) {
-        HIPCHK(hipMemcpyAsync(b->dsched, b->hsched.data(), b->hsched.size(), hipMemcpyHostToDevice, c->stream));
+    // triangle-dimer entries: one tiled launch per step covers every chain
+    int td_cfg = 0, td_blocks = 0;
+    std::vector<TdBatchEntry> htd;
+    if (!tdc.empty()) {
+        rt_chain* largest = tdc[0];
+        for (rt_chain* ch : tdc)
+            if (ch->G > largest->G) largest = ch;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned.find(std::make_tuple((int)RT_TRIDIMER, largest->N, c->uniform));
+            if (it != g_tuned.end() && it->second >= 0) td_cfg = it->second;
+        }
+        int tr, tc, nt;
+        td_config(td_cfg, &tr, &tc, &nt);
+        for (size_t k = 0; k < tdc.size(); k++) {
+            rt_chain* ch = tdc[k];
+            TdBatchEntry e{};
+            e.cells[0] = (uint4*)ch->cls[0];
+            e.cells[1] = (uint4*)ch->cls[1];
+            e.gens[0] = ch->gens[0];
+            e.gens[1] = ch->gens[1];
+            e.sched = b->dsched + (planar.size() + k) * seg;
+            e.M = ch->W;
+            e.buf0 = ch->cur;
+            e.g0 = ch->gcur;
+            e.tile0 = td_blocks;
+            e.ntiles = ((ch->W + tr - 1) / tr) * ((ch->W + tc - 1) / tc);
+            td_blocks += e.ntiles;
+            htd.push_back(e);
+        }
+        if (b->dtd_cap < htd.size()) {
+            if (b->dtd) {
+                HIPCHK(hipStreamSynchronize(c->stream));
+                HIPCHK(hipFree(b->dtd));
+                b->dtd = nullptr;
+            }
+            HIPCHK(hipMalloc(&b->dtd, sizeof(TdBatchEntry) * htd.size()));
+            b->dtd_cap = htd.size();
+        }
+    }
+    HIPCHK(hipMemcpyAsync(b->dsched, b->hsched.data(), b->hsched.size(), hipMemcpyHostToDevice, c->stream));
+    if (!b->hent.empty())
         HIPCHK(hipMemcpyAsync(b->dent, b->hent.data(), sizeof(BatchEntry) * b->hent.size(), hipMemcpyHostToDevice,
                               c->stream));
-        HIPCHK(hipEventRecord(b->ev_up, c->stream));
-        b->up_pending = true;
-    }
+    if (!htd.empty())
+        HIPCHK(hipMemcpyAsync(b->dtd, htd.data(), sizeof(TdBatchEntry) * htd.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(b->ev_up, c->stream));
+    b->up_pending = true;
     HIPCHK(hipEventRecord(b->ev0, c->stream));
     for (const Group& g : groups) {
         PersistArgs a{};
@@ -1210,12 +1258,49 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
             ch->timed = false;
         }
     }
-    for (rt_chain* ch : b->chains)
-        if (ch->model == RT_TRIDIMER) {
-            const int rc = rt_chain_run(ch, steps);
-            if (rc) return rc;
-            b->launches += ch->last_launches;
+    if (!htd.empty()) {
+        TdArgs a{};
+        a.params = c->uniform ? nullptr : c->params;
+        a.m1 = c->um1;
+        a.m2 = c->um2;
+        a.tm = c->utm;
+        HIPCHK(launch_fill_u32((uint32_t*)b->ctr, 4, 0u, c->stream));
+        if (use_graph) {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e1 = td_launch_step_batch(td_cfg, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr,
+                                                 c->stream);
+            hipError_t e2 = launch_add_i64(b->ctr, 1, c->stream);
+            hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
+            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+                if (graph) hipGraphDestroy(graph);
+                return fail(RT_ERR_HIP, "rt_batch_run: graph capture failed");
+            }
+            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            for (int64_t r = 0; r < steps && e == hipSuccess; r++) e = hipGraphLaunch(exec, c->stream);
+            if (exec) hipGraphExecDestroy(exec);
+            hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(RT_ERR_HIP, "rt_batch_run: graph replay: %s", hipGetErrorString(e));
+            b->graphs++;
+        } else {
+            for (int64_t r = 0; r < steps; r++) {
+                HIPCHK(td_launch_step_batch(td_cfg, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr,
+                                            c->stream));
+                HIPCHK(launch_add_i64(b->ctr, 1, c->stream));
+            }
         }
+        b->launches += steps;
+        for (rt_chain* ch : tdc) {
+            if (steps & 1) {
+                ch->cur ^= 1;
+                ch->gcur ^= 1;
+            }
+            ch->last_launches = steps;
+            ch->last_cfg = td_cfg;
+            ch->timed = false;
+        }
+    }
     HIPCHK(hipEventRecord(b->ev1, c->stream));
     b->timed = true;
     return RT_OK;

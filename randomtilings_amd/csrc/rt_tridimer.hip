@@ -393,12 +393,11 @@ __device__ __forceinline__ void load_params(const TdArgs& a, size_t id, uint32_t
 
 // ---- one step per launch, tiles + halo --------------------------------------------------
 template <int TR, int TC, int NT, bool UNI>
-__global__ __launch_bounds__(NT) void k_td_step(TdArgs a, uint32_t move) {
+__device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t move, const int tile) {
     constexpr int RR = TR + 2 * kHalo, RC = TC + 2 * kHalo;
     __shared__ uint16_t lds[8 * RR * RC];
     const int M = a.M;
     const int tiles_c = (M + TC - 1) / TC;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ti = tile / tiles_c, tj = tile % tiles_c;
     const int oi = ti * TR, oj = tj * TC;  // owned tile origin
     Region R;
@@ -448,6 +447,35 @@ __global__ __launch_bounds__(NT) void k_td_step(TdArgs a, uint32_t move) {
         c.w = V.g(6, i, j) | (V.g(7, i, j) << 16);
         a.cells_out[(size_t)gi * M + gj] = c;
     }
+}
+
+template <int TR, int TC, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_td_step(TdArgs a, uint32_t move) {
+    td_step_body<TR, TC, NT, UNI>(a, move, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Batched: every block runs one tile of one chain; the step index comes from a device
+// counter (one captured launch replays as is), and with it the chain's move and copies.
+template <int TR, int TC, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_td_step_batch(TdArgs a0, const TdBatchEntry* __restrict__ batch, int nbatch,
+                                                      const int64_t* __restrict__ step_ctr) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = nbatch - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (batch[mid].tile0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const TdBatchEntry& e = batch[lo];
+    const int64_t s = *step_ctr;
+    const int par = (int)(s & 1);
+    TdArgs a = a0;
+    a.M = e.M;
+    a.cells_in = e.cells[e.buf0 ^ par];
+    a.cells_out = e.cells[e.buf0 ^ par ^ 1];
+    a.gen_in = e.gens[e.g0 ^ par];
+    a.gen_out = e.gens[e.g0 ^ par ^ 1];
+    td_step_body<TR, TC, NT, UNI>(a, e.sched[s], b - e.tile0);
 }
 
 // ---- whole domain in one workgroup, every step in one launch ----------------------------
@@ -598,6 +626,15 @@ hipError_t launch_cfg(const TdArgs& a, uint32_t move, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <int I, bool UNI>
+hipError_t launch_batch_cfg(const TdArgs& a, const TdBatchEntry* batch, int nbatch, int blocks,
+                            const int64_t* step_ctr, hipStream_t st) {
+    constexpr TdCfg c = kCfgs[I];
+    hipLaunchKernelGGL((k_td_step_batch<c.tr, c.tc, c.nt, UNI>), dim3(blocks), dim3(c.nt), 0, st, a, batch, nbatch,
+                       step_ctr);
+    return hipGetLastError();
+}
+
 }  // namespace td
 
 int td_config_count() { return td::kNumCfgs; }
@@ -627,6 +664,20 @@ hipError_t td_launch_small(const TdArgs& a, bool uni, hipStream_t st) {
     if (uni) hipLaunchKernelGGL(td::k_td_small<true>, dim3(1), dim3(td::kSmallNT), 0, st, a);
     else hipLaunchKernelGGL(td::k_td_small<false>, dim3(1), dim3(td::kSmallNT), 0, st, a);
     return hipGetLastError();
+}
+
+
+hipError_t td_launch_step_batch(int cfg, const TdArgs& a, bool uni, const TdBatchEntry* batch, int nbatch, int blocks,
+                                const int64_t* step_ctr, hipStream_t st) {
+    switch (cfg) {
+        case 0: return uni ? td::launch_batch_cfg<0, true>(a, batch, nbatch, blocks, step_ctr, st)
+                           : td::launch_batch_cfg<0, false>(a, batch, nbatch, blocks, step_ctr, st);
+        case 1: return uni ? td::launch_batch_cfg<1, true>(a, batch, nbatch, blocks, step_ctr, st)
+                           : td::launch_batch_cfg<1, false>(a, batch, nbatch, blocks, step_ctr, st);
+        case 2: return uni ? td::launch_batch_cfg<2, true>(a, batch, nbatch, blocks, step_ctr, st)
+                           : td::launch_batch_cfg<2, false>(a, batch, nbatch, blocks, step_ctr, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t td_launch_step(int cfg, const TdArgs& a, bool uni, uint32_t move, hipStream_t st) {

@@ -25,6 +25,16 @@ struct TdArgs {
     int64_t nsteps;
 };
 
+// One triangle-dimer chain of a batched launch (blocks [tile0, tile0 + ntiles)).
+struct TdBatchEntry {
+    uint4* cells[2];
+    uint4* gens[2];
+    const uint8_t* sched;  // move codes of the whole batched run
+    int M;
+    int buf0, g0;
+    int tile0, ntiles;
+};
+
 // Move code of one reference step: type | r1 << 2 | r2 << 4 | p1 << 5 | p2 << 7.
 inline uint8_t td_move(int type, int r1, int r2, int p1, int p2) {
     return (uint8_t)(type | r1 << 2 | r2 << 4 | p1 << 5 | p2 << 7);
@@ -36,5 +46,7 @@ hipError_t td_launch_derive(const int32_t* t, int N, uint4* cells, hipStream_t s
 hipError_t td_launch_merge(const uint4* cells, int N, int32_t* t, hipStream_t st);
 hipError_t td_launch_small(const TdArgs& a, bool uni, hipStream_t st);
 hipError_t td_launch_step(int cfg, const TdArgs& a, bool uni, uint32_t move, hipStream_t st);
+hipError_t td_launch_step_batch(int cfg, const TdArgs& a, bool uni, const TdBatchEntry* batch, int nbatch, int blocks,
+                                const int64_t* step_ctr, hipStream_t st);
 
 }  // namespace rt
