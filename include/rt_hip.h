@@ -73,6 +73,7 @@ extern "C" {
 typedef struct rt_ctx rt_ctx;
 typedef struct rt_chain rt_chain;
 typedef struct rt_batch rt_batch;
+typedef struct rt_unet rt_unet;
 
 /* Thread-local description of the last error on this thread. */
 const char* rt_last_error(void);
@@ -254,6 +255,32 @@ int rt_batch_elapsed_ms(rt_batch* b, double* ms);
 /* Kernel launches (graph replays included) and graphs instantiated by the last run. */
 int64_t rt_batch_last_launches(rt_batch* b);
 int32_t rt_batch_last_graphs(rt_batch* b);
+
+/* ---- learned warm start (no counterpart in the reference) --------------------------------
+ * A U-Net trained and run on the GPU by this library's kernels: 3 x 3 convolutions as bf16
+ * matrix-core products (v_mfma_f32_32x32x8_bf16, fp32 accumulation), ReLU, 2 x 2 average
+ * pooling, nearest upsampling, additive skips; `levels` levels of widths base * 2^l, in_ch
+ * input channels, one output channel.  Images are NHWC fp32 host arrays (B x H x W x in_ch
+ * in, B x H x W out), H and W divisible by 2^(levels-1).  Parameters form one flat fp32
+ * vector in layer order (encoder levels 0..L-1 two convs each, decoder levels L-2..0 an
+ * up-conv and a conv each, the output conv; each conv W [9 cin x cout] with
+ * W[(tap*cin + ci)*cout + co], tap = 3*dy + dx, then its bias). */
+int rt_unet_create(rt_ctx* ctx, int levels, int base, int in_ch, int64_t seed, rt_unet** out);
+int rt_unet_destroy(rt_unet* net);
+int64_t rt_unet_param_count(rt_unet* net);
+int rt_unet_get_params(rt_unet* net, float* out);
+int rt_unet_set_params(rt_unet* net, const float* in);
+int rt_unet_forward(rt_unet* net, const float* x, int B, int H, int W, float* y);
+/* Masked mean squared error over pixels with a nonzero mask, and (grad != NULL) its gradient
+ * with respect to every parameter. */
+int rt_unet_loss_grad(rt_unet* net, const float* x, const float* target, const float* mask, int B, int H, int W,
+                      float* loss, float* grad);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on a batch; *loss before the step. */
+int rt_unet_train_step(rt_unet* net, const float* x, const float* target, const float* mask, int B, int H, int W,
+                       float lr, float* loss);
+/* C = A B through the bf16 matrix-core kernel (A: M x K, B: K x N row-major fp32, rounded to
+ * bf16; C fp32): the product on its own, for tests. */
+int rt_nn_gemm(rt_ctx* ctx, int M, int N, int K, const float* A, const float* B, float* C);
 
 /* Persistent tile configurations compiled for `model` (no device needed): count, and the
  * rows, columns, halo width and threads per workgroup of configuration idx. */

@@ -76,6 +76,15 @@ SIGNATURES = {
     "rt_chain_last_config": (C.c_int32, [_P]),
     "rt_chain_tile_configs": (C.c_int32, [_P]),
     "rt_chain_tile_config_info": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
+    "rt_unet_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int64, C.POINTER(_P)]),
+    "rt_unet_destroy": (C.c_int, [_P]),
+    "rt_unet_param_count": (C.c_int64, [_P]),
+    "rt_unet_get_params": (C.c_int, [_P, _FP]),
+    "rt_unet_set_params": (C.c_int, [_P, _FP]),
+    "rt_unet_forward": (C.c_int, [_P, _FP, C.c_int, C.c_int, C.c_int, _FP]),
+    "rt_unet_loss_grad": (C.c_int, [_P, _FP, _FP, _FP, C.c_int, C.c_int, C.c_int, _FP, _FP]),
+    "rt_unet_train_step": (C.c_int, [_P, _FP, _FP, _FP, C.c_int, C.c_int, C.c_int, C.c_float, _FP]),
+    "rt_nn_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _FP, _FP, _FP]),
     "rt_batch_create": (C.c_int, [_P, C.POINTER(_P)]),
     "rt_batch_destroy": (C.c_int, [_P]),
     "rt_batch_add": (C.c_int, [_P, _P]),
@@ -390,6 +399,81 @@ class Batch:
     def close(self) -> None:
         if self.h:
             self._lib.rt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def nn_gemm(ctx: Context, A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """A @ B through the bf16 matrix-core kernel (inputs rounded to bf16, fp32 result)."""
+    A, B = _f32(A), _f32(B)
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2
+    Cm = np.zeros((M, N), np.float32)
+    _check(ctx._lib.rt_nn_gemm(ctx.h, M, N, K, A.ctypes.data_as(_FP), B.ctypes.data_as(_FP), Cm.ctypes.data_as(_FP)))
+    return Cm
+
+
+class UNet:
+    """The warm-start U-Net (rt_unet_*): NHWC float32 images, one output channel."""
+
+    def __init__(self, ctx: Context, levels: int = 4, base: int = 32, in_ch: int = 4, seed: int = 0):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.levels, self.base, self.in_ch = levels, base, in_ch
+        h = _P()
+        _check(self._lib.rt_unet_create(ctx.h, levels, base, in_ch, seed, C.byref(h)))
+        self.h = h
+
+    @property
+    def param_count(self) -> int:
+        return int(self._lib.rt_unet_param_count(self.h))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.param_count, np.float32)
+        _check(self._lib.rt_unet_get_params(self.h, p.ctypes.data_as(_FP)))
+        return p
+
+    def set_params(self, p) -> None:
+        p = _f32(p)
+        _check(self._lib.rt_unet_set_params(self.h, p.ctypes.data_as(_FP)))
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = _f32(x)
+        B, H, W, _ = x.shape
+        y = np.zeros((B, H, W), np.float32)
+        _check(self._lib.rt_unet_forward(self.h, x.ctypes.data_as(_FP), B, H, W, y.ctypes.data_as(_FP)))
+        return y
+
+    def loss_grad(self, x, target, mask, grad: bool = True):
+        x, t, m = _f32(x), _f32(target), _f32(mask)
+        B, H, W, _ = x.shape
+        loss = C.c_float()
+        g = np.zeros(self.param_count, np.float32) if grad else None
+        _check(self._lib.rt_unet_loss_grad(self.h, x.ctypes.data_as(_FP), t.ctypes.data_as(_FP), m.ctypes.data_as(_FP),
+                                           B, H, W, C.byref(loss), None if g is None else g.ctypes.data_as(_FP)))
+        return loss.value, g
+
+    def train_step(self, x, target, mask, lr: float = 1e-3) -> float:
+        x, t, m = _f32(x), _f32(target), _f32(mask)
+        B, H, W, _ = x.shape
+        loss = C.c_float()
+        _check(self._lib.rt_unet_train_step(self.h, x.ctypes.data_as(_FP), t.ctypes.data_as(_FP),
+                                            m.ctypes.data_as(_FP), B, H, W, lr, C.byref(loss)))
+        return loss.value
+
+    def close(self) -> None:
+        if self.h:
+            self._lib.rt_unet_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -294,6 +294,15 @@ static int td_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
     return RT_OK;
 }
 
+namespace rt {
+int ctx_device(rt_ctx* c) { return c->device; }
+hipStream_t ctx_stream(rt_ctx* c) { return c->stream; }
+int set_error(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace rt
+
 extern "C" {
 
 const char* rt_last_error(void) { return g_err.c_str(); }

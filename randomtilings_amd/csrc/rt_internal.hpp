@@ -12,6 +12,11 @@
 
 namespace rt {
 
+// Context accessors and the thread-local error message (rt_last_error) for the other units.
+int ctx_device(rt_ctx* c);
+hipStream_t ctx_stream(rt_ctx* c);
+int set_error(int code, const std::string& msg);
+
 // Internal model id of the coupled domino pair (the two chains of coupling from the past).
 constexpr int kDominoPair = 4;
 
