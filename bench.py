@@ -47,7 +47,7 @@ def main() -> int:
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (to scale HBM byte counters)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "tridimer", "closure"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "tridimer", "closure"],
                     help="c2 (default): lozenge hexagon 1000; c1: domino Aztec diamond 128, 10^5 steps; "
                          "c4: coupling from the past on an Aztec diamond (--aztec); tridimer: triangle-dimer cone")
     ap.add_argument("--aztec", type=int, default=256, help="Aztec diamond order of the c4 workload")
@@ -56,6 +56,11 @@ def main() -> int:
     ap.add_argument("--domains", type=int, default=64, help="c3: number of domains")
     ap.add_argument("--sides", type=int, nargs=2, default=[64, 512], help="c3: domain side range")
     ap.add_argument("--domain-steps", type=int, default=1000, help="c3: chain steps per domain")
+    ap.add_argument("--canvas", type=int, default=64, help="c5: largest vertex grid side (training canvas)")
+    ap.add_argument("--train-domains", type=int, default=48, help="c5: training domains")
+    ap.add_argument("--eq-steps", type=int, default=30000, help="c5: chain steps per training sample")
+    ap.add_argument("--epochs", type=int, default=30, help="c5: training epochs")
+    ap.add_argument("--horizon", type=int, default=40000, help="c5: chain steps of each evaluation trajectory")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -80,7 +85,7 @@ def main() -> int:
     from randomtilings_amd.params import default_table
 
     if args.workload != "c2":
-        return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "tridimer": bench_tridimer,
+        return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "tridimer": bench_tridimer,
                 "closure": bench_closure}[args.workload](args)
 
     d = statics.call("lozenge.AlmostHexDomain", args=[args.hex, 0])
@@ -419,6 +424,55 @@ def bench_c3(args) -> int:
     b.close()
     for ch in solo + batched:
         ch.close()
+    ctx.close()
+    return 0
+
+
+def bench_c5(args) -> int:
+    """Learned warm start: train the U-Net (4 levels, 32 base channels, bf16 matrix cores) on
+    chain-equilibrated height functions of random domino domains (Aztec diamonds and notched
+    squares fitting a --canvas vertex grid), then compare chain steps to equilibrium (mean
+    interior height within 1% of the max-min range of its long-run value) from the projected
+    prediction and from the minimal tiling on held-out domains."""
+    import randomtilings_amd as rt
+    from randomtilings_amd import warmstart as ws
+    from randomtilings_amd.params import default_table
+
+    rng = np.random.default_rng(args.seed)
+    ctx = rt.Context(0)
+    ctx.load_params(default_table(args.canvas * args.canvas))
+    t0 = time.perf_counter()
+    doms = [ws.Domain(ctx, d) for _, d in ws.domain_bank(rng, args.train_domains, args.canvas)]
+    samples = ws.equilibrium_samples(ctx, doms, args.eq_steps, 8, args.seed)
+    X, T, M = ws.build_dataset(ctx, doms, samples, args.canvas)
+    data_s = time.perf_counter() - t0
+    net = rt.UNet(ctx, 4, 32, ws.IN_CH, seed=args.seed)
+    t0 = time.perf_counter()
+    losses = ws.train(net, X, T, M, args.epochs, 16, 2e-3, rng)
+    train_s = time.perf_counter() - t0
+    every = 100
+    results = []
+    held = [ws.Domain(ctx, d) for _, d in ws.domain_bank(np.random.default_rng(args.seed + 99), 4, args.canvas)]
+    for k, dom in enumerate(held):
+        tw = ws.warm_tiling(ctx, net, dom, args.canvas)
+        plain = ws.trajectory(ctx, dom, dom.tmin, args.horizon, every, 77 + k)
+        warm = ws.trajectory(ctx, dom, tw, args.horizon, every, 77 + k)
+        eq = 0.5 * (plain[len(plain) // 2:].mean() + warm[len(warm) // 2:].mean())
+        sp = ws.steps_to_equilibrium(plain, eq, every)
+        sw = ws.steps_to_equilibrium(warm, eq, every)
+        results.append({"N": dom.N, "plain_steps": sp, "warm_steps": sw, "eq_fraction": eq,
+                        "warm_start_fraction": float(warm[0]), "plain_start_fraction": float(plain[0])})
+    ratio = float(np.mean([r["plain_steps"] for r in results]) / max(1.0, np.mean([r["warm_steps"] for r in results])))
+    extra = {"canvas": args.canvas, "train_domains": args.train_domains, "train_samples": int(len(X)),
+             "eq_steps_per_sample": args.eq_steps, "unet": "4 levels x 32 base channels, bf16 MFMA, fp32 accumulate",
+             "params": net.param_count, "epochs": args.epochs, "final_loss": float(np.mean(losses[-10:])),
+             "first_loss": float(losses[0]), "data_seconds": data_s, "train_seconds": train_s,
+             "held_out": results, "steps_plain_over_warm": ratio}
+    line = _line("sweeps-to-equilibrium, plain / warm (domino, learned warm start)", ratio, "x", args,
+                 train_s * 1e3, "bf16/fp32", "warm start from the projected U-Net prediction vs the minimal tiling",
+                 extra)
+    print(json.dumps(line), flush=True)
+    net.close()
     ctx.close()
     return 0
 
