@@ -239,3 +239,41 @@ def test_cftp_weighted_law(golden):
     obs = np.array([counts[k] for k in keys])
     chi2 = float(((obs - S * w) ** 2 / (S * w)).sum())
     assert chi2 < 24.3, chi2  # 7 dof, p = 0.001
+
+
+def _recttri_law_chi2(t0, w16, samples, steps):
+    """Chi-square of the oracle's final states against mu(T) ~ prod_faces w(face).  The flip
+    (recttrianglekernel.cl:25-118) proposes symmetrically (each of the five all-triangle /
+    single-rectangle states of a hexagon goes to each other one with probability 0.9/4; the
+    six rectangle pairs toggle with probability 0.9) and accepts iff the six-digit weight
+    ratio >= a uniform draw, i.e. Metropolis for the product of the weights of the six faces
+    around the vertex.  Every face's digit is copied unchanged into the two other vertices
+    that see it (updateTiles1/2), so prod_v prod_digits w = prod_faces w^3."""
+    p = default_table(t0.size)
+    counts, tiles = {}, {}
+    for s in range(samples):
+        t = O.recttri_walk(t0, steps, 7919 * s + 1, p, weights16=w16)
+        counts[t.tobytes()] = counts.get(t.tobytes(), 0) + 1
+        tiles[t.tobytes()] = t
+    keys = list(counts)
+    lw = np.log(np.where(np.arange(16) == 0, 1.0, np.asarray(w16, np.float64)))
+    logw = np.array([lw[np.stack([(tiles[k] >> (4 * q)) & 15 for q in range(6)])].sum() / 3 for k in keys])
+    w = np.exp(logw - logw.max())
+    w /= w.sum()
+    obs = np.array([counts[k] for k in keys])
+    return len(keys), float(((obs - samples * w) ** 2 / (samples * w)).sum())
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_recttri_law(golden, weighted):
+    """The side-2 hexagon: 368 tilings reached; uniform for unit weights, the face-weight
+    product law for the weights below (a wrong exponent of the vertex product, 1/2 or 1
+    instead of 1/3, gives chi-square above 1200)."""
+    from scipy.stats import chi2
+    t0 = golden["recttri_minHex2"]
+    w16 = np.ones(16, np.float32)
+    if weighted:
+        w16 = np.array([0, 1.0, 0.6, 1.4, 0.7, 1.3, 0.9, 0.7, 1.3, 0.9, 1.2, 0.8, 1.1, 1.2, 0.8, 1.1], np.float32)
+    k, c = _recttri_law_chi2(t0, w16, 12000, 200)
+    assert k == 368
+    assert c < chi2.ppf(0.999, k - 1)
