@@ -51,6 +51,9 @@ def main() -> int:
                     help="c2 (default): lozenge hexagon 1000; c1: domino Aztec diamond 128, 10^5 steps; "
                          "c4: coupling from the past on an Aztec diamond (--aztec); tridimer: triangle-dimer cone")
     ap.add_argument("--aztec", type=int, default=256, help="Aztec diamond order of the c4 workload")
+    ap.add_argument("--p12", nargs=4, default=None, help="c4: weighted heat-bath probabilities (else p = 1/2)")
+    ap.add_argument("--placement-n", type=int, default=16, help="c4: Aztec order of the placement check")
+    ap.add_argument("--placement-samples", type=int, default=200, help="c4: exact samples for the placement check")
     ap.add_argument("--cone", type=int, default=1000, help="cone side of the tridimer workload")
     ap.add_argument("--check", action="store_true", help="c1: also compare the final tiling with the oracle")
     ap.add_argument("--domains", type=int, default=64, help="c3: number of domains")
@@ -260,14 +263,20 @@ def bench_c1(args) -> int:
 
 
 def bench_c4(args) -> int:
-    """Coupling from the past on the Aztec diamond of order --aztec, uniform rule, doubling
-    schedule: wall time to coalescence of the coupled chain (one launch sequence per restart,
-    device coupling test)."""
+    """Coupling from the past on the Aztec diamond of order --aztec, doubling schedule: wall
+    time to coalescence of the coupled chain (one launch sequence per restart, device coupling
+    test), under the uniform rule (dominokernelCFTP.cl) or, with --p12, the weighted heat-bath.
+    Then --placement-samples exact samples of the order --placement-n diamond under the same
+    rule, checked against the exact placement probabilities (Kasteleyn): z-scores of every
+    domino position."""
     import randomtilings_amd as rt
+    from randomtilings_amd import placement as PL
     from randomtilings_amd import statics
     from randomtilings_amd.params import default_table
 
     n = args.aztec
+    p12 = None if args.p12 is None else [float(x) for x in args.p12]
+    rule = rt.RT_DOMINO_CFTP if p12 is None else rt.RT_DOMINO_WEIGHTED
     d = statics.call("domino.AztecDiamond", args=[n])
     tmax, tmin = statics.call("domino.MaxTiling", d), statics.call("domino.MinTiling", d)
     ctx = rt.Context(0)
@@ -275,20 +284,33 @@ def bench_c4(args) -> int:
     times, steps, restarts = [], [], []
     for k in range(args.warmup + args.steps):
         w0 = time.perf_counter()
-        out, tot, r = ctx.domino_cftp(tmax, tmin, args.seed + k, mode=rt.RT_CFTP_DOUBLING, first_steps=1024,
-                                      max_restarts=30)
+        out, tot, r = ctx.domino_cftp(tmax, tmin, args.seed + k, rule=rule, p12=p12, mode=rt.RT_CFTP_DOUBLING,
+                                      first_steps=1024, max_restarts=30)
+        print(f"# sample {k}: {time.perf_counter() - w0:.2f} s, {r} restarts", file=sys.stderr, flush=True)
         if k >= args.warmup:
             times.append(time.perf_counter() - w0)
             steps.append(tot)
             restarts.append(r)
     mean = float(np.mean(times))
     executed = [sum(1024 * 2 ** i for i in range(r + 1)) for r in restarts]  # every replay's steps
-    extra = {"aztec_order": n, "schedule": "doubling from 1024 steps", "final_replay_steps": steps,
+    extra = {"aztec_order": n, "rule": "dominokernelCFTP.cl (p = 1/2)" if p12 is None else f"weighted heat-bath p12={p12}",
+             "schedule": "doubling from 1024 steps", "final_replay_steps": steps,
              "restarts": restarts, "coupled_steps_executed_per_sample": executed,
              "chain_attempts_per_step": 2 * (tmax.size // 2), "samples": args.steps}
+    if args.placement_samples > 0:
+        m = args.placement_n
+        dp = statics.call("domino.AztecDiamond", args=[m])
+        pmax, pmin = statics.call("domino.MaxTiling", dp), statics.call("domino.MinTiling", dp)
+        w0 = time.perf_counter()
+        samples = [ctx.domino_cftp(pmax, pmin, 7919 * s + 3, rule=rule, p12=p12, mode=rt.RT_CFTP_DOUBLING)[0]
+                   for s in range(args.placement_samples)]
+        sample_s = time.perf_counter() - w0
+        z, live = PL.zscores(samples, dp, p12 if p12 is not None else (0.5, 0.5, 0.5, 0.5))
+        extra["placement_check"] = {"aztec_order": m, "samples": args.placement_samples, "positions": live,
+                                    "max_abs_z": float(np.max(np.abs(z))), "mean_z2": float(np.mean(z ** 2)),
+                                    "sampling_seconds": sample_s}
     line = _line(f"seconds to coalescence (CFTP, domino Aztec diamond n={n})", mean, "s", args, mean * 1e3,
-                 "u8x2", f"CFTP domino Aztec diamond n={n}, dominokernelCFTP.cl rule, coupled max/min chains",
-                 extra, higher=False)
+                 "u8x2", f"CFTP domino Aztec diamond n={n}, coupled max/min chains", extra, higher=False)
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0
