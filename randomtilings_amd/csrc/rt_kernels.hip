@@ -350,7 +350,10 @@ struct Cfg {
     X(64, 84, 4, 1024, 1)   \
     X(32, 44, 4, 256, 1)    \
     X(32, 84, 8, 512, 1)    \
-    X(64, 84, 8, 1024, 0)
+    X(64, 84, 8, 1024, 0)   \
+    X(32, 84, 4, 512, 1)    \
+    X(64, 44, 4, 512, 1)    \
+    X(48, 56, 4, 512, 1)
 #define RT_RECTTRI_CFGS(X)  \
     X(32, 32, 4, 256, 0)    \
     X(48, 48, 8, 512, 0)    \

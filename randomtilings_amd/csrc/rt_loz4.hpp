@@ -15,6 +15,8 @@
 #include "rt_internal.hpp"
 #include "rt_models.hpp"
 
+#include <type_traits>
+
 namespace rt {
 
 __device__ __forceinline__ void xstore(uint32_t* p, uint32_t v) {
@@ -121,12 +123,16 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
         if (rr >= 0 && rr < tiles_r && cc >= 0 && cc < a.tiles_c) nbr_tile = rr * a.tiles_c + cc;
     }
 
-    // transition intervals on the raw tempered output t0 = k * 512 + (low 9 bits):
-    // lo <= k < hi  <=>  lo*512 <= t0 <= hi*512 - 1; an empty interval disables its state
+    // Transition intervals lo <= k < hi on k = t0 >> 9 (empty: lo == hi).  The lozenge rules'
+    // two intervals are disjoint (checked on the host), so the state that toggles under a
+    // draw is a function of the draw alone: 25 when k lies in A, 38 when it lies in B, and
+    // 0xff (never a state) otherwise.  When A = [0, x) and B = [y, 2^23) are both nonempty
+    // ("split": the q-rules) each test is one comparison of the raw output t0.
     const uint32_t alo = a.rp.fr.ab_lo[0][0], ahi = a.rp.fr.ab_hi[0][0];
     const uint32_t blo = a.rp.fr.ba_lo[0][0], bhi = a.rp.fr.ba_hi[0][0];
-    const uint32_t aL = alo << 9, aH = (ahi << 9) - 1u, aS = (ahi > alo) ? 25u : 256u;
-    const uint32_t bL = blo << 9, bH = (bhi << 9) - 1u, bS = (bhi > blo) ? 38u : 256u;
+    const uint32_t asz = ahi - alo, bsz = bhi - blo;
+    const bool split = alo == 0u && ahi > 0u && ahi <= blo && blo < bhi && bhi == (1u << 23);
+    const uint32_t aN = ahi << 9, bL = blo << 9;  // split: t0 < aN <=> k in A, t0 >= bL <=> k in B
     // per quad: byte masks of the bits each update rewrites
     uint32_t um2[QPT], um3[QPT];
 #pragma unroll
@@ -144,76 +150,136 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
         return ((uint64_t)v.y << 32) | v.x;
     };
 
+    // Flip class c on this thread's quads: per byte the target state x_b of its draw, the
+    // four targets packed into one word X, and the bytes where the state equals its target
+    // (exact zero-byte test of S ^ X) toggle (25 ^ 38 = 63).  SPLIT selects the one-comparison
+    // interval tests.  Every slot draws (a slot past the class area holds zero states and its
+    // draws are discarded), so only the LDS access is predicated and no generator state
+    // flows through a divergent branch.
+    auto flip = [&](uint32_t* Ls, auto split_c) __attribute__((always_inline)) {
+        constexpr bool SPLIT = decltype(split_c)::value;
+        // the quads' states are read first, so the LDS latency hides behind the draws
+        uint32_t S[QPT];
+#pragma unroll
+        for (int g = 0; g < QPT; g++) {
+            const int w = tid + g * NT;
+            S[g] = (AREAW % NT == 0 || w < AREAW) ? Ls[w] : 0u;
+        }
+#pragma unroll
+        for (int g = 0; g < QPT; g++) {
+            const int w = tid + g * NT;
+            uint32_t x[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint32_t t0 = tm_draw_raw(s0[g][b], s1[g][b], s2[g][b], s3[g][b], UNI ? a.m1 : pm1[g][b],
+                                                UNI ? a.m2 : pm2[g][b], UNI ? a.tm : pmt[g][b]);
+                if constexpr (SPLIT) {
+                    const uint32_t xb = (t0 >= bL) ? 38u : 0xffffffffu;
+                    x[b] = (t0 < aN) ? 25u : xb;
+                } else {
+                    const uint32_t k = t0 >> 9;
+                    const uint32_t xb = (k - blo < bsz) ? 38u : 0xffffffffu;
+                    x[b] = (k - alo < asz) ? 25u : xb;
+                }
+            }
+            const uint32_t X = __builtin_amdgcn_perm(__builtin_amdgcn_perm(x[3], x[2], 0x0c0c0400u),
+                                                     __builtin_amdgcn_perm(x[1], x[0], 0x0c0c0400u), 0x05040100u);
+            const uint32_t d = S[g] ^ X;
+            const uint32_t t = (d & 0x7f7f7f7fu) + 0x7f7f7f7fu;
+            const uint32_t z = RT_BITOP3(t, d, 0x80808080u, ~(A | B) & C);  // 0x80 where S_b == x_b
+            const uint32_t tog = ((z >> 1) - (z >> 7)) & gbytes[g];       // 63 there
+            if (AREAW % NT == 0 || w < AREAW) Ls[w] = S[g] ^ tog;
+        }
+    };
+
+    // Update classes c+1, c+2 from class c (lozengekernel.cl:45-52).  Every word the quad
+    // needs is read first (one wait), the row-dependent offsets are funnel shifts chosen per
+    // quad without branching, and the rewritten bits are merged with three-input selects
+    // (a term's stray bits fall outside the bits the final select keeps).
+    auto update = [&](uint32_t* Ls, uint32_t* L2, uint32_t* L3, int c) __attribute__((always_inline)) {
+        const int t2 = (2 * c + 1) % 3, t4 = (2 * c + 2) % 3, t8 = (2 * c) % 3;
+#pragma unroll
+        for (int g = 0; g < QPT; g++) {
+            const int w = tid + g * NT;
+            if (ubytes[g]) {
+                const int r = rmod[g];
+                const uint32_t U0 = Ls[w - EW], U1 = Ls[w - EW + 1];
+                const uint32_t Mm = Ls[w - 1], M0 = Ls[w], Mp = Ls[w + 1];
+                const uint32_t Dm = Ls[w + EW - 1], D0 = Ls[w + EW];
+                const uint32_t o2 = L2[w], o3 = L3[w];
+                const uint32_t up2 = funnel(U1, U0, (r == t2) ? 1 : 0);    // s[i-1][j+p2]
+                const uint32_t mm4 = (r == t4) ? funnel(M0, Mm, 3) : M0;   // s[i][j-p4]
+                const uint32_t mp8 = (r == t8) ? funnel(Mp, M0, 1) : M0;   // s[i][j+p8]
+                const uint32_t dm16 = (r == t2) ? funnel(D0, Dm, 3) : D0;  // s[i+1][j-p16]
+                // bits 2, 4, 32 of class c+1 and 1, 8, 16 of class c+2
+                const uint32_t n2 = RT_BITOP3(0x02020202u, up2 >> 3,
+                                              RT_BITOP3(0x04040404u, mm4 >> 1, D0 << 5, (A & B) | (~A & C)),
+                                              (A & B) | (~A & C));
+                const uint32_t n3 = RT_BITOP3(0x01010101u, U0 >> 5,
+                                              RT_BITOP3(0x08080808u, mp8 << 1, dm16 << 3, (A & B) | (~A & C)),
+                                              (A & B) | (~A & C));
+                L2[w] = RT_BITOP3(um2[g], n2, o2, (A & B) | (~A & C));
+                L3[w] = RT_BITOP3(um3[g], n3, o3, (A & B) | (~A & C));
+            }
+        }
+    };
+
+    // One chain step on colour class c.  A thread flips only its own quads, whose other
+    // classes it updated itself in the previous step; only when the same class is rotated
+    // twice in a row could a flip overwrite a neighbour that another thread's update is
+    // still reading, so only then is a barrier needed before the flip.
+    auto step = [&](int c, int c_prev, auto split_c) __attribute__((always_inline)) {
+        uint32_t* Ls = L + c * AREAW;
+        uint32_t* L2 = L + (c == 2 ? 0 : c + 1) * AREAW;
+        uint32_t* L3 = L + (c == 0 ? 2 : c - 1) * AREAW;
+        if (c == c_prev) __syncthreads();
+        flip(Ls, split_c);
+        __syncthreads();
+        update(Ls, L2, L3, c);
+    };
+    // The steps of one block (blk <= H): a full block is unrolled, so the generator states
+    // rotate through registers without moves.
+    auto run_block = [&](uint64_t colours, int64_t blk, int& c_prev, auto split_c) __attribute__((always_inline)) {
+        if (blk == H) {
+#pragma unroll
+            for (int q = 0; q < H; q++) {
+                const int c = (int)((colours >> (8 * q)) & 0xffu);
+                step(c, c_prev, split_c);
+                c_prev = c;
+            }
+        } else {
+            for (int64_t q = 0; q < blk; q++) {
+                const int c = (int)((colours >> (8 * q)) & 0xffu);
+                step(c, c_prev, split_c);
+                c_prev = c;
+            }
+        }
+    };
+
     int buf = a.buf;
     uint32_t nblk = 0;
     int64_t s = 0;
     uint64_t colours = fetch_block(0);
     while (s < a.nsteps) {
         const int64_t blk = (a.nsteps - s < H) ? (a.nsteps - s) : H;
-        int c_prev = -1;
-        for (int64_t q = 0; q < blk; q++, s++) {
-            const int c = (int)((colours >> (8 * q)) & 0xffu);
-            uint32_t* Ls = L + c * AREAW;
-            uint32_t* L2 = L + (c == 2 ? 0 : c + 1) * AREAW;
-            uint32_t* L3 = L + (c == 0 ? 2 : c - 1) * AREAW;
-            // A thread flips only its own quads, whose other classes it updated itself in
-            // the previous step; only when the same class is rotated twice in a row could a
-            // flip overwrite a neighbour that another thread's update is still reading.
-            if (c == c_prev) __syncthreads();
-            c_prev = c;
-            // -------- flip class c on this thread's quads
-#pragma unroll
-            for (int g = 0; g < QPT; g++) {
-                const int w = tid + g * NT;
-                if (AREAW % NT == 0 || w < AREAW) {
-                    const uint32_t S = Ls[w];
-                    uint32_t tog = 0;
-#pragma unroll
-                    for (int b = 0; b < 4; b++) {
-                        const uint32_t t0 = tm_draw_raw(s0[g][b], s1[g][b], s2[g][b], s3[g][b], UNI ? a.m1 : pm1[g][b],
-                                                        UNI ? a.m2 : pm2[g][b], UNI ? a.tm : pmt[g][b]);
-                        const uint32_t sb = (S >> (8 * b)) & 0xffu;
-                        const bool t = ((sb == aS) & (t0 >= aL) & (t0 <= aH)) | ((sb == bS) & (t0 >= bL) & (t0 <= bH));
-                        tog |= t ? (63u << (8 * b)) : 0u;
-                    }
-                    tog &= gbytes[g];
-                    if (tog) Ls[w] = S ^ tog;
-                }
-            }
-            __syncthreads();
-            // -------- update classes c+1, c+2 from class c (lozengekernel.cl:45-52)
-            const int t2 = (2 * c + 1) % 3, t4 = (2 * c + 2) % 3, t8 = (2 * c) % 3;
-#pragma unroll
-            for (int g = 0; g < QPT; g++) {
-                const int w = tid + g * NT;
-                if (ubytes[g]) {
-                    const int r = rmod[g];
-                    const uint32_t U0 = Ls[w - EW], U1 = Ls[w - EW + 1];
-                    const uint32_t Mm = Ls[w - 1], M0 = Ls[w], Mp = Ls[w + 1];
-                    const uint32_t Dm = Ls[w + EW - 1], D0 = Ls[w + EW];
-                    const uint32_t up2 = (r == t2) ? funnel(U1, U0, 1) : U0;   // s[i-1][j+p2]
-                    const uint32_t mm4 = (r == t4) ? funnel(M0, Mm, 3) : M0;   // s[i][j-p4]
-                    const uint32_t mp8 = (r == t8) ? funnel(Mp, M0, 1) : M0;   // s[i][j+p8]
-                    const uint32_t dm16 = (r == t2) ? funnel(D0, Dm, 3) : D0;  // s[i+1][j-p16]
-                    const uint32_t n2 = RT_BITOP3((up2 >> 3) & 0x02020202u, (mm4 >> 1) & 0x04040404u,
-                                                  (D0 << 5) & 0x20202020u, A | B | C);
-                    const uint32_t n3 = RT_BITOP3((U0 >> 5) & 0x01010101u, (mp8 << 1) & 0x08080808u,
-                                                  (dm16 << 3) & 0x10101010u, A | B | C);
-                    L2[w] = (L2[w] & ~um2[g]) | (n2 & um2[g]);
-                    L3[w] = (L3[w] & ~um3[g]) | (n3 & um3[g]);
-                }
-            }
-        }
+        int c_prev = -1;  // the block starts after a barrier
+        if (split) run_block(colours, blk, c_prev, std::true_type{});
+        else run_block(colours, blk, c_prev, std::false_type{});
+        s += blk;
+        const bool last = s >= a.nsteps;
         // the next block's colours are in flight during the exchange
-        const uint64_t next_colours = (s < a.nsteps) ? fetch_block(s) : 0;
+        const uint64_t next_colours = last ? 0 : fetch_block(s);
         __syncthreads();  // every update of the block is in LDS before it is published
-        // publish owned words into the other copy
+        // Publish into the other copy: after an intermediate block only the band of width H
+        // along the tile's edges (the neighbours' halos read nothing else), after the last
+        // block every owned word (the launch's result).
         {
             uint32_t* dst = reinterpret_cast<uint32_t*>(a.cls[buf ^ 1]);
             for (int q = tid; q < TR * TW; q += NT) {
                 const int oi = q / TW, ow = q - (q / TW) * TW;
+                const bool band = oi < H || oi >= TR - H || ow < HW || ow >= TW - HW;
                 const int gi = tr * TR + oi, gw = tc * TW + ow;
-                if (gi < N && gw < PW) {
+                if ((last || band) && gi < N && gw < PW) {
                     const int lq = (oi + H) * EW + HW + ow;
 #pragma unroll
                     for (int c = 0; c < NC; c++) xstore(dst + c * CSW + (size_t)gi * PW + gw, L[c * AREAW + lq]);
@@ -221,7 +287,7 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
             }
         }
         buf ^= 1;
-        if (s < a.nsteps) {
+        if (!last) {
             nblk++;
             const uint32_t epoch = a.epoch0 + nblk;
             __builtin_amdgcn_s_waitcnt(0);

@@ -82,7 +82,21 @@ bool domino_rule(int rule, const double* p12, FlipRule& fr) {
 // Lozenge rules.  rule 0: lozengekernel.cl:21-24.  rule 1: qlozengekernel.cl:23-29 with a
 // runtime q (the float variable rd is overwritten by the double expression, as there).
 // rule 2: heat-bath `rd < q/(1+q)` adds a cube (25->38), otherwise removes (38->25).
+static bool lozenge_rule_raw(int rule, double q, FlipRule& fr);
+
+// The packed lozenge kernel relies on the add and remove intervals being disjoint (every
+// lozenge rule's are); a rule whose intervals overlapped would be refused here.
+static bool disjoint(const FlipRule& fr) {
+    const uint32_t alo = fr.ab_lo[0][0], ahi = fr.ab_hi[0][0], blo = fr.ba_lo[0][0], bhi = fr.ba_hi[0][0];
+    return alo == ahi || blo == bhi || ahi <= blo || bhi <= alo;
+}
+
 bool lozenge_rule(int rule, double q, FlipRule& fr) {
+    if (!lozenge_rule_raw(rule, q, fr)) return false;
+    return disjoint(fr);
+}
+
+static bool lozenge_rule_raw(int rule, double q, FlipRule& fr) {
     std::memset(&fr, 0, sizeof(fr));
     if (rule == 0) {
         const uint32_t t4 = first_false(0, K_END, [](uint32_t k) { return (double)rd_of(k) < .4; });

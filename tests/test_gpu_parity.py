@@ -123,6 +123,26 @@ def test_lozenge_walk_parity(gctx, golden, rule, q, path, cfg):
         assert lozenge_legal(g)
 
 
+@pytest.mark.parametrize("cfg", range(N_CFGS[rt.RT_LOZENGE]))
+@pytest.mark.parametrize("rule,q", [(rt.RT_LOZENGE_UNIFORM, 0.99), (rt.RT_LOZENGE_QREF, 0.999),
+                                    (rt.RT_LOZENGE_QHEAT, 5.0)])
+def test_lozenge_multitile_parity(gctx, rule, q, cfg):
+    """A hexagon of side 150 (N = 307: a grid of several tiles in both directions for every
+    configuration), a step count that is no multiple of the block length, persistent and
+    blocked paths: neighbours exchange only their edge bands between blocks."""
+    from randomtilings_amd import statics as St
+    t0 = St.call("lozenge.MinTiling", St.call("lozenge.AlmostHexDomain", args=[150, 0]))
+    o = O.lozenge_walk(t0, 203, 19, default_table(BIG), rule=rule, q=q)
+    for path in (rt.RT_PATH_PERSISTENT, rt.RT_PATH_BLOCKED):
+        say("lozenge multitile", rule, q, path, cfg)
+        gctx.set_path(path, cfg)
+        try:
+            g = gctx.lozenge_walk(t0, 203, 19, rule=rule, q=q)
+        finally:
+            gctx.set_path(rt.RT_PATH_AUTO, -1)
+        assert np.array_equal(g, o), path
+
+
 def test_lozenge_scrambled_params(sctx, golden):
     t0 = golden["lozenge_hex20_0_max"]
     for path, cfg in [(rt.RT_PATH_STEPWISE, -1), (rt.RT_PATH_PERSISTENT, 0), (rt.RT_PATH_PERSISTENT, 2)]:
