@@ -133,10 +133,11 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     const uint32_t asz = ahi - alo, bsz = bhi - blo;
     const bool split = alo == 0u && ahi > 0u && ahi <= blo && blo < bhi && bhi == (1u << 23);
     const uint32_t aN = ahi << 9, bL = blo << 9;  // split: t0 < aN <=> k in A, t0 >= bL <=> k in B
-    // per quad: byte masks of the bits each update rewrites
-    uint32_t um2[QPT], um3[QPT];
+    // per quad: byte masks of the bits each update rewrites (recomputed at each use when
+    // per-generator parameters take the registers)
+    uint32_t um2[UNI ? QPT : 1], um3[UNI ? QPT : 1];
 #pragma unroll
-    for (int g = 0; g < QPT; g++) {
+    for (int g = 0; g < (UNI ? QPT : 1); g++) {
         um2[g] = ubytes[g] & 0x26262626u;
         um3[g] = ubytes[g] & 0x19191919u;
     }
@@ -150,45 +151,71 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
         return ((uint64_t)v.y << 32) | v.x;
     };
 
-    // Flip class c on this thread's quads: per byte the target state x_b of its draw, the
-    // four targets packed into one word X, and the bytes where the state equals its target
-    // (exact zero-byte test of S ^ X) toggle (25 ^ 38 = 63).  SPLIT selects the one-comparison
+    // Flip decisions.  A draw depends only on its generator, and the state a draw toggles
+    // only on the draw (the rule is the same for every class), so the targets of a whole
+    // block can be drawn ahead of its flips.  targets() advances the four generators of quad
+    // g by one draw and returns the four target states packed into one word: 25 when k lies
+    // in A, 38 when in B, 0xff (never a state) otherwise.  SPLIT selects the one-comparison
     // interval tests.  Every slot draws (a slot past the class area holds zero states and its
-    // draws are discarded), so only the LDS access is predicated and no generator state
-    // flows through a divergent branch.
-    auto flip = [&](uint32_t* Ls, auto split_c) __attribute__((always_inline)) {
+    // draws are discarded), so no generator state flows through a divergent branch.
+    auto targets = [&](int g, auto split_c) __attribute__((always_inline)) -> uint32_t {
         constexpr bool SPLIT = decltype(split_c)::value;
-        // the quads' states are read first, so the LDS latency hides behind the draws
-        uint32_t S[QPT];
+        uint32_t x[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const uint32_t t0 = tm_draw_raw(s0[g][b], s1[g][b], s2[g][b], s3[g][b], UNI ? a.m1 : pm1[g][b],
+                                            UNI ? a.m2 : pm2[g][b], UNI ? a.tm : pmt[g][b]);
+            if constexpr (SPLIT) {
+                const uint32_t xb = (t0 >= bL) ? 38u : 0xffffffffu;
+                x[b] = (t0 < aN) ? 25u : xb;
+            } else {
+                const uint32_t k = t0 >> 9;
+                const uint32_t xb = (k - blo < bsz) ? 38u : 0xffffffffu;
+                x[b] = (k - alo < asz) ? 25u : xb;
+            }
+        }
+        return __builtin_amdgcn_perm(__builtin_amdgcn_perm(x[3], x[2], 0x0c0c0400u),
+                                     __builtin_amdgcn_perm(x[1], x[0], 0x0c0c0400u), 0x05040100u);
+    };
+    // Toggle the bytes of quad g whose state equals its target (exact zero-byte test of
+    // S ^ X; 25 ^ 38 = 63).
+    auto apply = [&](uint32_t* Ls, int g, uint32_t S, uint32_t X) __attribute__((always_inline)) {
+        const int w = tid + g * NT;
+        const uint32_t d = S ^ X;
+        const uint32_t t = (d & 0x7f7f7f7fu) + 0x7f7f7f7fu;
+        const uint32_t z = RT_BITOP3(t, d, 0x80808080u, ~(A | B) & C);  // 0x80 where S_b == x_b
+        const uint32_t tog = ((z >> 1) - (z >> 7)) & gbytes[g];       // 63 there
+        if (AREAW % NT == 0 || w < AREAW) Ls[w] = S ^ tog;
+    };
+    auto load_states = [&](const uint32_t* Ls, uint32_t* S) __attribute__((always_inline)) {
 #pragma unroll
         for (int g = 0; g < QPT; g++) {
             const int w = tid + g * NT;
             S[g] = (AREAW % NT == 0 || w < AREAW) ? Ls[w] : 0u;
         }
+    };
+    // flip with draws made now
+    auto flip = [&](uint32_t* Ls, auto split_c) __attribute__((always_inline)) {
+        uint32_t S[QPT];
+        load_states(Ls, S);  // read first, so the LDS latency hides behind the draws
 #pragma unroll
-        for (int g = 0; g < QPT; g++) {
-            const int w = tid + g * NT;
-            uint32_t x[4];
+        for (int g = 0; g < QPT; g++) apply(Ls, g, S[g], targets(g, split_c));
+    };
+    // the targets of the first DA steps of the next full block, drawn during the exchange
+    // (more would not fit the register budget; none with per-generator parameters, whose
+    // registers leave no room)
+    constexpr int DA = UNI ? H / 2 : 0;
+    uint32_t XP[DA > 0 ? DA : 1][QPT];
+    auto draw_ahead = [&](auto q0_c, auto q1_c, auto split_c) __attribute__((always_inline)) {
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const uint32_t t0 = tm_draw_raw(s0[g][b], s1[g][b], s2[g][b], s3[g][b], UNI ? a.m1 : pm1[g][b],
-                                                UNI ? a.m2 : pm2[g][b], UNI ? a.tm : pmt[g][b]);
-                if constexpr (SPLIT) {
-                    const uint32_t xb = (t0 >= bL) ? 38u : 0xffffffffu;
-                    x[b] = (t0 < aN) ? 25u : xb;
-                } else {
-                    const uint32_t k = t0 >> 9;
-                    const uint32_t xb = (k - blo < bsz) ? 38u : 0xffffffffu;
-                    x[b] = (k - alo < asz) ? 25u : xb;
-                }
-            }
-            const uint32_t X = __builtin_amdgcn_perm(__builtin_amdgcn_perm(x[3], x[2], 0x0c0c0400u),
-                                                     __builtin_amdgcn_perm(x[1], x[0], 0x0c0c0400u), 0x05040100u);
-            const uint32_t d = S[g] ^ X;
-            const uint32_t t = (d & 0x7f7f7f7fu) + 0x7f7f7f7fu;
-            const uint32_t z = RT_BITOP3(t, d, 0x80808080u, ~(A | B) & C);  // 0x80 where S_b == x_b
-            const uint32_t tog = ((z >> 1) - (z >> 7)) & gbytes[g];       // 63 there
-            if (AREAW % NT == 0 || w < AREAW) Ls[w] = S[g] ^ tog;
+        for (int q = decltype(q0_c)::value; q < decltype(q1_c)::value; q++)
+#pragma unroll
+            for (int g = 0; g < QPT; g++) XP[q][g] = targets(g, split_c);
+    };
+    auto draw_ahead_any = [&](auto q0_c, auto q1_c) __attribute__((always_inline)) {
+        if constexpr (DA > 0) {
+            if (split) draw_ahead(q0_c, q1_c, std::true_type{});
+            else draw_ahead(q0_c, q1_c, std::false_type{});
         }
     };
 
@@ -218,8 +245,10 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
                 const uint32_t n3 = RT_BITOP3(0x01010101u, U0 >> 5,
                                               RT_BITOP3(0x08080808u, mp8 << 1, dm16 << 3, (A & B) | (~A & C)),
                                               (A & B) | (~A & C));
-                L2[w] = RT_BITOP3(um2[g], n2, o2, (A & B) | (~A & C));
-                L3[w] = RT_BITOP3(um3[g], n3, o3, (A & B) | (~A & C));
+                const uint32_t m2 = UNI ? um2[UNI ? g : 0] : (ubytes[g] & 0x26262626u);
+                const uint32_t m3 = UNI ? um3[UNI ? g : 0] : (ubytes[g] & 0x19191919u);
+                L2[w] = RT_BITOP3(m2, n2, o2, (A & B) | (~A & C));
+                L3[w] = RT_BITOP3(m3, n3, o3, (A & B) | (~A & C));
             }
         }
     };
@@ -228,31 +257,41 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     // classes it updated itself in the previous step; only when the same class is rotated
     // twice in a row could a flip overwrite a neighbour that another thread's update is
     // still reading, so only then is a barrier needed before the flip.
-    auto step = [&](int c, int c_prev, auto split_c) __attribute__((always_inline)) {
+    auto step = [&](int c, int c_prev, auto flip_c) __attribute__((always_inline)) {
         uint32_t* Ls = L + c * AREAW;
         uint32_t* L2 = L + (c == 2 ? 0 : c + 1) * AREAW;
         uint32_t* L3 = L + (c == 0 ? 2 : c - 1) * AREAW;
         if (c == c_prev) __syncthreads();
-        flip(Ls, split_c);
+        flip_c(Ls);
         __syncthreads();
         update(Ls, L2, L3, c);
     };
-    // The steps of one block (blk <= H): a full block is unrolled, so the generator states
-    // rotate through registers without moves.
-    auto run_block = [&](uint64_t colours, int64_t blk, int& c_prev, auto split_c) __attribute__((always_inline)) {
-        if (blk == H) {
+
+    // A full block, steps unrolled (the generator states rotate through registers without
+    // moves): the first DA steps use the targets drawn ahead, the others draw now.
+    auto run_full = [&](uint64_t colours, int& c_prev, auto split_c) __attribute__((always_inline)) {
 #pragma unroll
-            for (int q = 0; q < H; q++) {
-                const int c = (int)((colours >> (8 * q)) & 0xffu);
-                step(c, c_prev, split_c);
-                c_prev = c;
+        for (int q = 0; q < H; q++) {
+            const int c = (int)((colours >> (8 * q)) & 0xffu);
+            if (q < DA) {
+                step(c, c_prev, [&](uint32_t* Ls) __attribute__((always_inline)) {
+                    uint32_t S[QPT];
+                    load_states(Ls, S);
+#pragma unroll
+                    for (int g = 0; g < QPT; g++) apply(Ls, g, S[g], XP[q < DA ? q : 0][g]);
+                });
+            } else {
+                step(c, c_prev, [&](uint32_t* Ls) __attribute__((always_inline)) { flip(Ls, split_c); });
             }
-        } else {
-            for (int64_t q = 0; q < blk; q++) {
-                const int c = (int)((colours >> (8 * q)) & 0xffu);
-                step(c, c_prev, split_c);
-                c_prev = c;
-            }
+            c_prev = c;
+        }
+    };
+    // a partial block (the last of a launch), drawing as it goes
+    auto run_part = [&](uint64_t colours, int64_t blk, int& c_prev, auto split_c) __attribute__((always_inline)) {
+        for (int64_t q = 0; q < blk; q++) {
+            const int c = (int)((colours >> (8 * q)) & 0xffu);
+            step(c, c_prev, [&](uint32_t* Ls) __attribute__((always_inline)) { flip(Ls, split_c); });
+            c_prev = c;
         }
     };
 
@@ -260,11 +299,20 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     uint32_t nblk = 0;
     int64_t s = 0;
     uint64_t colours = fetch_block(0);
+    using Z = std::integral_constant<int, 0>;
+    using DH = std::integral_constant<int, DA / 2>;
+    using DF = std::integral_constant<int, DA>;
+    if (a.nsteps >= H) draw_ahead_any(Z{}, DF{});
     while (s < a.nsteps) {
         const int64_t blk = (a.nsteps - s < H) ? (a.nsteps - s) : H;
         int c_prev = -1;  // the block starts after a barrier
-        if (split) run_block(colours, blk, c_prev, std::true_type{});
-        else run_block(colours, blk, c_prev, std::false_type{});
+        if (blk == H) {
+            if (split) run_full(colours, c_prev, std::true_type{});
+            else run_full(colours, c_prev, std::false_type{});
+        } else {
+            if (split) run_part(colours, blk, c_prev, std::true_type{});
+            else run_part(colours, blk, c_prev, std::false_type{});
+        }
         s += blk;
         const bool last = s >= a.nsteps;
         // the next block's colours are in flight during the exchange
@@ -290,9 +338,14 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
         if (!last) {
             nblk++;
             const uint32_t epoch = a.epoch0 + nblk;
+            // the next full block's targets are drawn while the published words drain and
+            // while the neighbours finish their blocks
+            const bool full_next = a.nsteps - s >= H;
+            if (full_next) draw_ahead_any(Z{}, DH{});
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (tid == 0) xstore(a.flags + (size_t)tile * kLine, epoch);
+            if (full_next) draw_ahead_any(DH{}, DF{});
             if (nbr_tile >= 0) {
                 const uint32_t* f = a.flags + (size_t)nbr_tile * kLine;
                 uint32_t polls = 0;
