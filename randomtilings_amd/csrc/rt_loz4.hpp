@@ -115,6 +115,36 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     };
     load_ext(a.buf, false);
     __syncthreads();
+    // The halo reload between blocks in two halves: issue() starts the loads of the words
+    // this thread reloads (into registers), land() writes them to LDS once they arrive, so
+    // work placed between the two overlaps the load latency.
+    constexpr int QL = (AREAW + NT - 1) / NT;
+    uint32_t hv[QL][NC];
+    auto halo_issue = [&](int bufi) __attribute__((always_inline)) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.cls[bufi]);
+#pragma unroll
+        for (int g = 0; g < QL; g++) {
+            const int q = tid + g * NT;
+            const int li = q / EW, lw = q - (q / EW) * EW;
+            const int gi = r0 + li, gw = c0w + lw;
+            const bool owned = li >= H && li < H + TR && lw >= HW && lw < HW + TW;
+            const bool in = q < AREAW && !owned && gi >= 0 && gi < N && gw >= 0 && gw < PW;
+#pragma unroll
+            for (int c = 0; c < NC; c++) hv[g][c] = in ? xload(src + c * CSW + (size_t)gi * PW + gw) : 0u;
+        }
+    };
+    auto halo_land = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int g = 0; g < QL; g++) {
+            const int q = tid + g * NT;
+            const int li = q / EW, lw = q - (q / EW) * EW;
+            const bool owned = li >= H && li < H + TR && lw >= HW && lw < HW + TW;
+            if (q < AREAW && !owned) {
+#pragma unroll
+                for (int c = 0; c < NC; c++) L[c * AREAW + q] = hv[g][c];
+            }
+        }
+    };
 
     int nbr_tile = -1;
     if (tid < 9 && tid != 4) {
@@ -300,7 +330,10 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     int64_t s = 0;
     uint64_t colours = fetch_block(0);
     using Z = std::integral_constant<int, 0>;
-    using DH = std::integral_constant<int, DA / 2>;
+    // the drawn-ahead steps are spread over the exchange: [0, D1) while the published words
+    // drain, [D1, D3) while the neighbours' flags arrive, [D3, DA) while the halo loads
+    using D1 = std::integral_constant<int, DA / 4>;
+    using D3 = std::integral_constant<int, 3 * DA / 4>;
     using DF = std::integral_constant<int, DA>;
     if (a.nsteps >= H) draw_ahead_any(Z{}, DF{});
     while (s < a.nsteps) {
@@ -341,11 +374,11 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
             // the next full block's targets are drawn while the published words drain and
             // while the neighbours finish their blocks
             const bool full_next = a.nsteps - s >= H;
-            if (full_next) draw_ahead_any(Z{}, DH{});
+            if (full_next) draw_ahead_any(Z{}, D1{});
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (tid == 0) xstore(a.flags + (size_t)tile * kLine, epoch);
-            if (full_next) draw_ahead_any(DH{}, DF{});
+            if (full_next) draw_ahead_any(D1{}, D3{});
             if (nbr_tile >= 0) {
                 const uint32_t* f = a.flags + (size_t)nbr_tile * kLine;
                 uint32_t polls = 0;
@@ -359,7 +392,13 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
                 }
             }
             __syncthreads();
-            load_ext(buf, true);
+            if constexpr (DA > 0) {
+                halo_issue(buf);
+                if (full_next) draw_ahead_any(D3{}, DF{});
+                halo_land();
+            } else {
+                load_ext(buf, true);
+            }
             __syncthreads();
         }
         colours = next_colours;
