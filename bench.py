@@ -103,7 +103,7 @@ def main() -> int:
     if args.tile_config >= 0:
         ch.set_path(rt.RT_PATH_PERSISTENT, args.tile_config)
     else:
-        ch.autotune(2048)  # on a scratch clone; the measured chain's trajectory is unchanged
+        ch.autotune(4096)  # on a scratch clone; the measured chain's trajectory is unchanged
     ch.seed(args.seed)
     S = args.chain_steps
     print(f"# bench: lozenge hexagon {args.hex} (N={N}), q={args.q}, {S} chain steps per bench step, "

@@ -808,10 +808,14 @@ int rt_chain_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
         bool res = false;
         if (choose_persist_cfg(t, &nb, &tcn, &res) != i) continue;  // does not fit
         rc = rt_chain_run(t, probe_steps / 4 + 1);  // warm-up
-        if (rc == RT_OK) rc = rt_chain_run(t, probe_steps);
-        double ms = 0;
-        if (rc == RT_OK) rc = rt_chain_elapsed_ms(t, &ms);
-        if (rc == RT_OK) rc = rt_chain_sync(t);
+        double ms = 1e30;
+        for (int rep = 0; rep < 2 && rc == RT_OK; rep++) {  // the faster of two probes
+            double m = 0;
+            rc = rt_chain_run(t, probe_steps);
+            if (rc == RT_OK) rc = rt_chain_elapsed_ms(t, &m);
+            if (rc == RT_OK) rc = rt_chain_sync(t);
+            if (rc == RT_OK && m < ms) ms = m;
+        }
         if (rc == RT_OK && ms < best_ms) {
             best_ms = ms;
             chosen = i;
