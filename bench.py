@@ -403,6 +403,7 @@ def bench_c3(args) -> int:
     gmax = max(int(t.size) for _, _, t, _ in specs)
     ctx.load_params(default_table(gmax))
     steps = args.domain_steps
+    print(f"# c3: {len(specs)} domains built (largest tiling {gmax} vertices)", file=sys.stderr, flush=True)
 
     def make():
         chs = [ctx.chain(m, r, t) for m, r, t, _ in specs]
@@ -413,10 +414,15 @@ def bench_c3(args) -> int:
     solo = make()
     ctx.sync()
     w0 = time.perf_counter()
-    for ch in solo:
+    for k, ch in enumerate(solo):
         ch.run(steps)
+        if k % 64 == 63:  # progress (the stream runs the chains one after another either way)
+            ctx.sync()
+            print(f"# c3: {k + 1} domains run one by one, {time.perf_counter() - w0:.1f} s", file=sys.stderr,
+                  flush=True)
     ctx.sync()
     solo_s = time.perf_counter() - w0
+    print(f"# c3: one run per domain {solo_s:.1f} s", file=sys.stderr, flush=True)
     batched = make()
     b = rt.Batch(ctx)
     for ch in batched:
@@ -426,8 +432,16 @@ def bench_c3(args) -> int:
     b.run(steps, True)
     b.sync()
     batch_s = time.perf_counter() - w0
+    print(f"# c3: batched {batch_s:.1f} s", file=sys.stderr, flush=True)
     att = sum(ch.attempts_per_step for ch in batched) * steps
-    same = all(np.array_equal(a.read(), c.read()) for a, c in zip(solo, batched))
+    same = True
+    for k, (a, c) in enumerate(zip(solo, batched)):
+        try:
+            same = same and bool(np.array_equal(a.read(), c.read()))
+        except rt.RTError:
+            print(f"# c3: domain {k} ({specs[k][3]}, tiling side {int(round(np.sqrt(specs[k][2].size)))}, "
+                  f"config {a.last_config}) failed", file=sys.stderr, flush=True)
+            raise
     kinds = {}
     for (_, _, t, kind), ch in zip(specs, batched):
         kinds.setdefault(kind, [0, 0])

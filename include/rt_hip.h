@@ -282,6 +282,12 @@ int rt_unet_train_step(rt_unet* net, const float* x, const float* target, const 
  * bf16; C fp32): the product on its own, for tests. */
 int rt_nn_gemm(rt_ctx* ctx, int M, int N, int K, const float* A, const float* B, float* C);
 
+/* Workgroups a persistent launch of tile configuration idx of `model` may use (at most one per
+ * CU; uniform_params: the single-parameter-set kernel), 0 when the configuration never runs
+ * persistently (its kernel needs scratch memory), -1 for a bad index.  Larger tile grids run
+ * one launch per block of steps.  Diagnostic; no counterpart in the reference. */
+int32_t rt_persist_capacity(int model, int32_t idx, int device, int uniform_params);
+
 /* Persistent tile configurations compiled for `model` (no device needed): count, and the
  * rows, columns, halo width and threads per workgroup of configuration idx. */
 int32_t rt_tile_config_count(int model);

@@ -71,6 +71,7 @@ SIGNATURES = {
     "rt_chain_attempts_per_step": (C.c_int64, [_P]),
     "rt_chain_state_bytes": (C.c_int64, [_P]),
     "rt_tile_config_count": (C.c_int32, [C.c_int]),
+    "rt_persist_capacity": (C.c_int32, [C.c_int, C.c_int32, C.c_int, C.c_int]),
     "rt_tile_config": (C.c_int, [C.c_int, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
     "rt_chain_last_launches": (C.c_int64, [_P]),
     "rt_chain_last_config": (C.c_int32, [_P]),
@@ -147,6 +148,11 @@ def rule_thresholds(model: int, rule: int, rparams=None) -> np.ndarray:
 
 def tile_config_count(model: int) -> int:
     return int(load().rt_tile_config_count(model))
+
+
+def persist_capacity(model: int, idx: int, device: int = 0, uniform: bool = True) -> int:
+    """Workgroups of tile configuration idx the device holds at once (0: never persistent)."""
+    return int(load().rt_persist_capacity(model, idx, device, int(uniform)))
 
 
 def tile_config(model: int, idx: int):

@@ -731,7 +731,7 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* re
         const int trn = (ch->N + tr - 1) / tr, tcn = (ch->W + tc - 1) / tc;
         const int cap = persist_capacity(ch->model, idx, uni, ch->ctx->device);
         if (cap <= 0) return false;
-        *res = trn * tcn <= cap && trn * tcn <= kMaxBlocks;
+        *res = trn * tcn <= cap && trn * tcn <= kMaxBlocks && !persist_uses_scratch(ch->model, idx, uni);
         *nblocks = trn * tcn;
         *tiles_c = tcn;
         return true;
@@ -976,6 +976,14 @@ int64_t rt_chain_attempts_per_step(rt_chain* ch) { return ch ? ch->G : 0; }
 int64_t rt_chain_state_bytes(rt_chain* ch) {
     if (ch && ch->model == RT_TRIDIMER) return ch->G * (int64_t)sizeof(uint4);
     return ch ? (int64_t)ch->N * ch->W * ch->NC * ch->esize : 0;
+}
+
+int32_t rt_persist_capacity(int model, int32_t idx, int device, int uniform_params) {
+    if (model == RT_TRIDIMER || idx < 0 || idx >= persist_config_count(model)) return -1;
+    if (hipSetDevice(device) != hipSuccess) return -1;
+    const bool uni = uniform_params != 0;
+    if (persist_uses_scratch(model, idx, uni)) return 0;
+    return persist_capacity(model, idx, uni, device);
 }
 
 int32_t rt_tile_config_count(int model) {

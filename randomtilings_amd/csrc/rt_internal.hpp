@@ -110,6 +110,7 @@ bool boundary_heights_host(int model, const std::vector<int>& d, std::vector<int
 int persist_config_count(int model);
 bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt);
 int persist_capacity(int model, int idx, bool uni, int device);
+bool persist_uses_scratch(int model, int idx, bool uni);
 hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t st);
 

@@ -180,7 +180,14 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const int til
     const int r0 = tr * TR - H, c0 = tc * TC - H;  // global coordinates of LDS (0, 0)
     const int c0w = c0 * ES / 4;                   // ... in dwords (c0 is a whole dword)
     const int tid = threadIdx.x;
-    const RuleParams& rp = a.rp;
+    // The rule tables are indexed per lane (rectangle-triangle states), so they are staged in
+    // LDS: indexed in place they would be copied to scratch memory, and a kernel that needs
+    // scratch cannot rely on all its workgroups being resident.
+    __shared__ RuleParams srp;
+    for (int q = tid; q < (int)(sizeof(RuleParams) / 4); q += NT)
+        reinterpret_cast<uint32_t*>(&srp)[q] = reinterpret_cast<const uint32_t*>(&a.rp)[q];
+    __syncthreads();
+    const RuleParams& rp = srp;
 
     // generator states of every tile + halo position this thread handles
     uint32_t s0[GPT], s1[GPT], s2[GPT], s3[GPT];
@@ -246,7 +253,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const int til
         for (int64_t q = 0; q < blk; q++, s++) {
             const int c = a.sched[s];
             T* lc = lds + c * AREA;
-            const auto sr = M::step_rule(rp, c);
+            const auto& sr = M::step_rule(rp, c);
 #pragma unroll
             for (int g = 0; g < GPT; g++) {
                 if (gmask & (1u << g)) {
@@ -343,7 +350,7 @@ struct Cfg {
 #define RT_DOMINO_CFGS(X)    \
     X(32, 32, 8, 256, 0)     \
     X(16, 32, 4, 256, 0)     \
-    X(64, 64, 8, 512, 0)     \
+    X(48, 64, 8, 1024, 0)    \
     X(32, 64, 8, 512, 0)
 #define RT_LOZENGE_CFGS(X)  \
     X(64, 84, 8, 1024, 1)   \
@@ -437,7 +444,12 @@ static const void* persist_kernel(int model, int idx, bool uni, bool batch = fal
     return nullptr;
 }
 
-// Largest number of co-resident workgroups of this configuration on the device.
+// Number of workgroups of this configuration a persistent launch may use: one per CU at most.
+// The occupancy query bounds what a CU could hold, not what the dispatcher places: a grid of
+// 256-thread workgroups at 92 % of the queried occupancy (7.4 per CU) was seen not to be
+// co-resident, and a persistent grid that is not co-resident waits on tiles that never start.
+// One workgroup per CU is what the persistent sweeps are built around (large tiles, 1024
+// threads) and is always placed.
 int persist_capacity(int model, int idx, bool uni, int device) {
     const void* k = persist_kernel(model, idx, uni);
     int tr, tc, h, nt;
@@ -446,7 +458,18 @@ int persist_capacity(int model, int idx, bool uni, int device) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, nt, 0) != hipSuccess) return 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
-    return per_cu * prop.multiProcessorCount;
+    return std::min(per_cu, 1) * prop.multiProcessorCount;
+}
+
+// Whether the configuration's kernel needs scratch (private) memory.  Waves that need scratch
+// are launched only as far as the runtime's scratch allocation reaches, which the occupancy
+// figure above does not bound, so such a grid is never treated as co-resident.
+bool persist_uses_scratch(int model, int idx, bool uni) {
+    const void* k = persist_kernel(model, idx, uni);
+    if (!k) return true;
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, k) != hipSuccess) return true;
+    return fa.localSizeBytes > 0;
 }
 
 hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st) {
