@@ -47,7 +47,8 @@ def main() -> int:
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (to scale HBM byte counters)")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "tridimer", "closure"],
+    ap.add_argument("--workload", default="c2",
+                    choices=["c2", "c1", "c3", "c4", "c5", "tridimer", "closure", "gemm"],
                     help="c2 (default): lozenge hexagon 1000; c1: domino Aztec diamond 128, 10^5 steps; "
                          "c4: coupling from the past on an Aztec diamond (--aztec); tridimer: triangle-dimer cone")
     ap.add_argument("--aztec", type=int, default=256, help="Aztec diamond order of the c4 workload")
@@ -89,7 +90,7 @@ def main() -> int:
 
     if args.workload != "c2":
         return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "tridimer": bench_tridimer,
-                "closure": bench_closure}[args.workload](args)
+                "closure": bench_closure, "gemm": bench_gemm}[args.workload](args)
 
     d = statics.call("lozenge.AlmostHexDomain", args=[args.hex, 0])
     t0 = statics.call("lozenge.MinTiling", d)
@@ -537,6 +538,38 @@ def bench_closure(args) -> int:
              "identical_to_host": bool(np.array_equal(t, ht))}
     line = _line("seconds per maximal tiling (device closure)", mean, "s", args, mean * 1e3, "i32",
                  f"lozenge MaxTiling of AlmostHexDomain({args.hex},0)", extra, higher=False)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+def bench_gemm(args) -> int:
+    """The U-Net's matrix-core products on device-resident bf16 operands: the convolution
+    products of the 4-level, 32-channel network on a batch of 16 64 x 64 images (forward,
+    weight gradient, input gradient at every level), and a large square product; TFLOP/s
+    against the dense bf16 peak (2.5 PFLOP/s)."""
+    import randomtilings_amd as rt
+
+    ctx = rt.Context(0)
+    shapes = []
+    for lvl, cin, cout in [(0, 32, 32), (1, 64, 64), (2, 128, 128), (3, 256, 256)]:
+        P = 16 * (64 >> lvl) ** 2
+        shapes += [(f"fwd L{lvl}", P, cout, 9 * cin, 0), (f"dW L{lvl}", 9 * cin, cout, P, 1),
+                   (f"dX L{lvl}", P, 9 * cin, cout, 2)]
+    shapes.append(("square 8192", 8192, 8192, 8192, 0))
+    rows = []
+    for name, M, N, K, lay in shapes:
+        ms = rt.nn_gemm_time(ctx, M, N, K, lay, reps=max(2, args.steps))
+        tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+        rows.append({"product": name, "M": M, "N": N, "K": K, "layout": lay, "ms": ms, "tflops": tf})
+        print(f"# gemm {name}: {M}x{N}x{K} layout {lay}: {ms:.3f} ms, {tf:.1f} TFLOP/s", file=sys.stderr, flush=True)
+    big = rows[-1]
+    peak = 2500.0
+    line = _line("bf16 matrix-core product throughput (U-Net kernel)", big["tflops"], "TFLOP/s", args, big["ms"],
+                 "bf16/fp32", "square 8192 product and the U-Net's convolution products",
+                 {"products": rows})
+    line["roofline"] = {"bound": "mfma", "achieved": big["tflops"], "peak": peak, "unit": "TFLOP/s",
+                        "frac": big["tflops"] / peak, "traffic": None}
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0

@@ -281,6 +281,13 @@ int rt_unet_train_step(rt_unet* net, const float* x, const float* target, const 
 /* C = A B through the bf16 matrix-core kernel (A: M x K, B: K x N row-major fp32, rounded to
  * bf16; C fp32): the product on its own, for tests. */
 int rt_nn_gemm(rt_ctx* ctx, int M, int N, int K, const float* A, const float* B, float* C);
+/* The same with the operands stored as `layout` says (0: A [M x K] and B [K x N] row-major;
+ * 1: A stored as [K x M]; 2: B stored as [N x K]). */
+int rt_nn_gemm_ex(rt_ctx* ctx, int M, int N, int K, int layout, const float* A, const float* B, float* C);
+/* Device-resident timing of that product for one operand layout (0: A [M x K] and B [K x N]
+ * row-major, a convolution's forward product; 1: A stored as [K x M], the weight gradient;
+ * 2: B stored as [N x K], the input gradient): *ms = milliseconds per product over reps. */
+int rt_nn_gemm_time(rt_ctx* ctx, int M, int N, int K, int layout, int reps, double* ms);
 
 /* Workgroups a persistent launch of tile configuration idx of `model` may use (at most one per
  * CU; uniform_params: the single-parameter-set kernel), 0 when the configuration never runs

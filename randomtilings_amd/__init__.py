@@ -86,6 +86,8 @@ SIGNATURES = {
     "rt_unet_loss_grad": (C.c_int, [_P, _FP, _FP, _FP, C.c_int, C.c_int, C.c_int, _FP, _FP]),
     "rt_unet_train_step": (C.c_int, [_P, _FP, _FP, _FP, C.c_int, C.c_int, C.c_int, C.c_float, _FP]),
     "rt_nn_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _FP, _FP, _FP]),
+    "rt_nn_gemm_ex": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _FP, _FP, _FP]),
+    "rt_nn_gemm_time": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "rt_batch_create": (C.c_int, [_P, C.POINTER(_P)]),
     "rt_batch_destroy": (C.c_int, [_P]),
     "rt_batch_add": (C.c_int, [_P, _P]),
@@ -418,15 +420,24 @@ def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
-def nn_gemm(ctx: Context, A: np.ndarray, B: np.ndarray) -> np.ndarray:
-    """A @ B through the bf16 matrix-core kernel (inputs rounded to bf16, fp32 result)."""
+def nn_gemm(ctx: Context, A: np.ndarray, B: np.ndarray, layout: int = 0) -> np.ndarray:
+    """A @ B through the bf16 matrix-core kernel (inputs rounded to bf16, fp32 result).  With
+    layout 1, A is passed as its transpose ([K x M]); with layout 2, B as its transpose."""
     A, B = _f32(A), _f32(B)
-    M, K = A.shape
-    K2, N = B.shape
+    M, K = A.shape if layout != 1 else A.shape[::-1]
+    K2, N = B.shape if layout != 2 else B.shape[::-1]
     assert K == K2
     Cm = np.zeros((M, N), np.float32)
-    _check(ctx._lib.rt_nn_gemm(ctx.h, M, N, K, A.ctypes.data_as(_FP), B.ctypes.data_as(_FP), Cm.ctypes.data_as(_FP)))
+    _check(ctx._lib.rt_nn_gemm_ex(ctx.h, M, N, K, layout, A.ctypes.data_as(_FP), B.ctypes.data_as(_FP),
+                                  Cm.ctypes.data_as(_FP)))
     return Cm
+
+
+def nn_gemm_time(ctx: Context, M: int, N: int, K: int, layout: int = 0, reps: int = 10) -> float:
+    """Milliseconds per device-resident bf16 product (rt_nn_gemm_time)."""
+    ms = C.c_double()
+    _check(ctx._lib.rt_nn_gemm_time(ctx.h, M, N, K, layout, reps, C.byref(ms)))
+    return ms.value
 
 
 class UNet:

@@ -8,6 +8,9 @@ namespace rt {
 
 // C[m][n] (row stride ldc) = alpha * sum_k A(m, k) B(k, n) + beta * C[m][n], with
 // A(m, k) = A[m * sam + k * sak] and B(k, n) = B[k * sbk + n * sbn] in bf16.
+// ws (ws_elems bf16, 16-byte aligned; may be null): room to transpose an operand that is
+// stored along M or N, which lets the product use the large-tile kernel (nn_gemm_workspace
+// elements suffice for any layout).
 struct GemmArgs {
     const uint16_t* A;
     const uint16_t* B;
@@ -15,8 +18,11 @@ struct GemmArgs {
     int M, N, K;
     int64_t sam, sak, sbk, sbn, ldc;
     float alpha, beta;
+    uint16_t* ws = nullptr;
+    int64_t ws_elems = 0;
 };
 
+int64_t nn_gemm_workspace(int M, int N, int K);
 hipError_t nn_gemm(const GemmArgs& g, hipStream_t st);
 hipError_t nn_to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t st);
 

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -66,6 +67,8 @@ struct rt_unet {
     uint16_t* col = nullptr;
     float* dcol = nullptr;
     uint16_t* dybf = nullptr;
+    uint16_t* gws = nullptr;  // transposed-operand workspace of the matrix products
+    int64_t gws_elems = 0;
 };
 
 namespace {
@@ -111,9 +114,13 @@ hipError_t ensure_ws(rt_unet* n, int B, int H, int W) {
     n->ds.assign(L, nullptr);
     n->dd.assign(L, nullptr);
     size_t colmax = 0, dymax = 0;
+    int64_t gmax = 0;
     auto conv_ws = [&](const Conv& c, int l) {
         colmax = std::max(colmax, px(l) * 9 * c.cin);
         dymax = std::max(dymax, px(l) * c.cout);
+        const int P = (int)px(l);
+        gmax = std::max({gmax, rt::nn_gemm_workspace(P, c.cout, 9 * c.cin), rt::nn_gemm_workspace(9 * c.cin, c.cout, P),
+                         rt::nn_gemm_workspace(P, 9 * c.cin, c.cout)});
     };
     for (int l = 0; l < L && e == hipSuccess; l++) {
         const int cl = width(n, l), cprev = l ? width(n, l - 1) : n->cin;
@@ -149,6 +156,8 @@ hipError_t ensure_ws(rt_unet* n, int B, int H, int W) {
     if (e == hipSuccess) e = alloc(n, colmax * 2, (void**)&n->col);
     if (e == hipSuccess) e = F(colmax, &n->dcol);
     if (e == hipSuccess) e = alloc(n, dymax * 2, (void**)&n->dybf);
+    if (e == hipSuccess) e = alloc(n, (size_t)gmax * 2, (void**)&n->gws);
+    n->gws_elems = e == hipSuccess ? gmax : 0;
     if (e == hipSuccess) {
         n->B = B;
         n->H = H;
@@ -183,6 +192,8 @@ hipError_t conv_fwd(rt_unet* n, const Conv& c, const float* x, float* y, int h, 
     g.ldc = c.cout;
     g.alpha = 1.f;
     g.beta = 0.f;
+    g.ws = n->gws;
+    g.ws_elems = n->gws_elems;
     NNCHK(rt::nn_gemm(g, n->st));
     return rt::nn_bias_relu(y, n->w + c.b, P, c.cout, relu, n->st);
 }
@@ -208,6 +219,8 @@ hipError_t conv_bwd(rt_unet* n, const Conv& c, const float* x, const float* dy, 
     g.ldc = c.cout;
     g.alpha = 1.f;
     g.beta = 0.f;
+    g.ws = n->gws;
+    g.ws_elems = n->gws_elems;
     NNCHK(rt::nn_gemm(g, n->st));
     NNCHK(rt::nn_bias_grad(dy, n->g + c.b, P, c.cout, n->st));
     if (!dx) return hipSuccess;
@@ -466,8 +479,9 @@ int rt_unet_train_step(rt_unet* n, const float* x, const float* target, const fl
 
 // C = A B (A: M x K, B: K x N, both converted to bf16), C: M x N fp32 — the matrix-core
 // product on its own, for tests of the kernel.
-int rt_nn_gemm(rt_ctx* ctx, int M, int N, int K, const float* A, const float* B, float* C) {
-    if (!ctx || !A || !B || !C || M <= 0 || N <= 0 || K <= 0) return ufail(RT_ERR_ARG, "rt_nn_gemm: bad arguments");
+int rt_nn_gemm_ex(rt_ctx* ctx, int M, int N, int K, int layout, const float* A, const float* B, float* C) {
+    if (!ctx || !A || !B || !C || M <= 0 || N <= 0 || K <= 0 || layout < 0 || layout > 2)
+        return ufail(RT_ERR_ARG, "rt_nn_gemm: bad arguments");
     hipStream_t st = rt::ctx_stream(ctx);
     (void)hipSetDevice(rt::ctx_device(ctx));
     float *a = nullptr, *b = nullptr, *c = nullptr;
@@ -477,17 +491,86 @@ int rt_nn_gemm(rt_ctx* ctx, int M, int N, int K, const float* A, const float* B,
     if (e == hipSuccess) e = hipMalloc(&c, sizeof(float) * (size_t)M * N);
     if (e == hipSuccess) e = hipMalloc(&ah, 2 * (size_t)M * K);
     if (e == hipSuccess) e = hipMalloc(&bh, 2 * (size_t)K * N);
+    uint16_t* ws = nullptr;
+    const int64_t wse = rt::nn_gemm_workspace(M, N, K);
+    if (e == hipSuccess) e = hipMalloc(&ws, 2 * (size_t)wse);
     if (e == hipSuccess) e = hipMemcpy(a, A, sizeof(float) * (size_t)M * K, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(b, B, sizeof(float) * (size_t)K * N, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = rt::nn_to_bf16(a, ah, (int64_t)M * K, st);
     if (e == hipSuccess) e = rt::nn_to_bf16(b, bh, (int64_t)K * N, st);
     rt::GemmArgs g{ah, bh, c, M, N, K, K, 1, N, 1, N, 1.f, 0.f};
+    if (layout == 1) {  // A stored [K x M]
+        g.sam = 1;
+        g.sak = M;
+    } else if (layout == 2) {  // B stored [N x K]
+        g.sbk = 1;
+        g.sbn = K;
+    }
+    g.ws = ws;
+    g.ws_elems = wse;
     if (e == hipSuccess) e = rt::nn_gemm(g, st);
     if (e == hipSuccess) e = hipMemcpyAsync(C, c, sizeof(float) * (size_t)M * N, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    for (void* p : {(void*)a, (void*)b, (void*)c, (void*)ah, (void*)bh})
+    for (void* p : {(void*)a, (void*)b, (void*)c, (void*)ah, (void*)bh, (void*)ws})
         if (p) (void)hipFree(p);
     if (e != hipSuccess) return ufail(RT_ERR_HIP, std::string("rt_nn_gemm: ") + hipGetErrorString(e));
+    return RT_OK;
+}
+
+int rt_nn_gemm(rt_ctx* ctx, int M, int N, int K, const float* A, const float* B, float* C) {
+    return rt_nn_gemm_ex(ctx, M, N, K, 0, A, B, C);
+}
+
+// Device-resident timing of the matrix-core product for one operand layout (0: A and B
+// row-major as in a convolution's forward product; 1: A stored transposed, the weight
+// gradient; 2: B stored transposed, the input gradient), on bf16 operands filled on the
+// device: milliseconds per product averaged over `reps` launches (HIP events on the context
+// stream).
+int rt_nn_gemm_time(rt_ctx* ctx, int M, int N, int K, int layout, int reps, double* ms) {
+    if (!ctx || !ms || M <= 0 || N <= 0 || K <= 0 || reps <= 0 || layout < 0 || layout > 2)
+        return ufail(RT_ERR_ARG, "rt_nn_gemm_time: bad arguments");
+    hipStream_t st = rt::ctx_stream(ctx);
+    (void)hipSetDevice(rt::ctx_device(ctx));
+    float* f = nullptr;
+    float* c = nullptr;
+    uint16_t *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int64_t na = (int64_t)M * K, nb = (int64_t)K * N;
+    hipError_t e = hipMalloc(&f, sizeof(float) * (size_t)std::max(na, nb));
+    if (e == hipSuccess) e = hipMalloc(&a, 2 * (size_t)na);
+    if (e == hipSuccess) e = hipMalloc(&b, 2 * (size_t)nb);
+    if (e == hipSuccess) e = hipMalloc(&c, sizeof(float) * (size_t)M * N);
+    uint16_t* ws = nullptr;
+    const int64_t wse = rt::nn_gemm_workspace(M, N, K);
+    if (e == hipSuccess) e = hipMalloc(&ws, 2 * (size_t)wse);
+    if (e == hipSuccess) e = hipMemsetAsync(f, 0, sizeof(float) * (size_t)std::max(na, nb), st);
+    if (e == hipSuccess) e = rt::nn_to_bf16(f, a, na, st);
+    if (e == hipSuccess) e = rt::nn_to_bf16(f, b, nb, st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    rt::GemmArgs g{a, b, c, M, N, K, K, 1, N, 1, N, 1.f, 0.f};
+    if (layout == 1) {
+        g.sam = 1;
+        g.sak = M;
+    } else if (layout == 2) {
+        g.sbk = 1;
+        g.sbn = K;
+    }
+    g.ws = ws;
+    g.ws_elems = wse;
+    if (e == hipSuccess) e = rt::nn_gemm(g, st);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = rt::nn_gemm(g, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    *ms = (double)t / reps;
+    for (void* p : {(void*)f, (void*)c, (void*)a, (void*)b, (void*)ws})
+        if (p) (void)hipFree(p);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return ufail(RT_ERR_HIP, std::string("rt_nn_gemm_time: ") + hipGetErrorString(e));
     return RT_OK;
 }
 

@@ -26,13 +26,18 @@ def nctx():
     c.close()
 
 
-@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (100, 70, 45), (32, 288, 9000), (5000, 32, 288), (7, 3, 1)])
-def test_gemm_against_numpy(nctx, M, N, K):
-    rng = np.random.default_rng(M * 7 + N)
+@pytest.mark.parametrize("layout", [0, 1, 2])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (100, 70, 45), (32, 288, 9000), (5000, 32, 288), (7, 3, 1),
+                                   (264, 136, 104), (1024, 512, 2056), (8, 8, 8)])
+def test_gemm_against_numpy(nctx, M, N, K, layout):
+    """Products through both kernels (128 x 128 tiles when every operand vector is aligned and
+    whole, the general kernel otherwise), every operand layout, split K, partial tiles.  The
+    tolerance is that of fp32 accumulation of bf16 products."""
+    rng = np.random.default_rng(M * 7 + N + layout)
     A = rng.normal(size=(M, K)).astype(np.float32)
     B = rng.normal(size=(K, N)).astype(np.float32)
-    print("gemm", M, N, K, flush=True)
-    C = rt.nn_gemm(nctx, A, B)
+    print("gemm", M, N, K, layout, flush=True)
+    C = rt.nn_gemm(nctx, A.T.copy() if layout == 1 else A, B.T.copy() if layout == 2 else B, layout)
     ref = bf16(A).astype(np.float64) @ bf16(B).astype(np.float64)
     assert np.abs(C - ref).max() <= 1e-4 * np.abs(ref).max() + 1e-5
 
