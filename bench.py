@@ -257,6 +257,15 @@ def bench_c1(args) -> int:
     line = _line("chain steps/s (domino Aztec diamond n=128, CFTP rule)", S * args.steps / wall, "steps/s", args,
                  wall * 1e3 / args.steps, "u8", "domino Aztec diamond n=128, dominokernelCFTP.cl rule, minimal start, "
                  "10^5 chain steps per bench step", extra)
+    if not args.no_cpu:
+        import oracle
+        cs = 20000
+        w0 = time.perf_counter()
+        oracle.domino_walk(t0, cs, args.seed, default_table(t0.size // 2), rule=1)
+        cpu_s = time.perf_counter() - w0
+        line["cpu_baseline"] = {"value": cs / cpu_s, "unit": "steps/s", "cores": 1, "kind": "port",
+                                "sample": f"oracle (sequential C++ restatement) DominoTiler Walk of {cs} steps on the "
+                                          "same Aztec diamond, host split/merge included"}
     print(json.dumps(line), flush=True)
     ch.close()
     ctx.close()
@@ -312,19 +321,36 @@ def bench_c4(args) -> int:
                                     "sampling_seconds": sample_s}
     line = _line(f"seconds to coalescence (CFTP, domino Aztec diamond n={n})", mean, "s", args, mean * 1e3,
                  "u8x2", f"CFTP domino Aztec diamond n={n}, coupled max/min chains", extra, higher=False)
+    if not args.no_cpu:
+        # the oracle's walk on one chain for a bounded sample; the reference's driver replays
+        # both chains separately, so its time to coalescence projects as 2 x steps / rate
+        import oracle
+        cs = 500
+        w0 = time.perf_counter()
+        oracle.domino_walk(tmax, cs, args.seed, default_table(tmax.size // 2), rule=1 if p12 is None else 2, p12=p12)
+        rate = cs / (time.perf_counter() - w0)
+        line["cpu_baseline"] = {"value": 2 * float(np.mean(executed)) / rate, "unit": "s", "cores": 1, "kind": "port",
+                                "sample": f"oracle (sequential C++ restatement) DominoTiler Walk of {cs} steps on the "
+                                          f"maximal tiling ({rate:.0f} steps/s), projected over the "
+                                          f"{float(np.mean(executed)):.0f} steps of both chains' replays"}
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0
 
 
 def bench_tridimer(args) -> int:
-    """Triangle-dimer chain on IceCreamCone(--cone, 0): flip attempts per second of the
-    one-launch-per-step tiled kernel (all three move types, reference schedule)."""
+    """Triangle-dimer chain on IceCreamCone(--cone, 0) (the next tileable side: the cone's
+    triangle count must be even): flip attempts per second of the one-launch-per-step tiled
+    kernel (all three move types, reference schedule)."""
     import randomtilings_amd as rt
     from randomtilings_amd import statics
     from randomtilings_amd.params import default_table
 
-    t0 = statics.call("tridimer.IceCreamCone", args=[args.cone, 0])
+    cone = args.cone
+    while ((cone + 1) * (cone + 2) // 2) % 2:  # an odd number of triangles has no dimer cover
+        cone += 1
+    args.cone = cone
+    t0 = statics.call("tridimer.IceCreamCone", args=[cone, 0])
     N = int(round(np.sqrt(t0.size)))
     ctx = rt.Context(0)
     ctx.load_params(default_table((N // 2) ** 2))
@@ -351,6 +377,15 @@ def bench_tridimer(args) -> int:
     line = _line("flip attempts/sec (triangle dimers)", att * S * args.steps / wall, "flip attempts/s", args,
                  wall * 1e3 / args.steps, "u16", f"triangle-dimer IceCreamCone({args.cone},0), reference move schedule",
                  extra)
+    if not args.no_cpu:
+        import oracle
+        cs = 20
+        w0 = time.perf_counter()
+        oracle.tridimer_walk(t0, cs, args.seed, default_table((N // 2) ** 2))
+        cpu_s = time.perf_counter() - w0
+        line["cpu_baseline"] = {"value": att * cs / cpu_s, "unit": "flip attempts/s", "cores": 1, "kind": "port",
+                                "sample": f"oracle (sequential C++ restatement) TriangleDimerTiler Walk of {cs} steps "
+                                          "on the same cone, host state derivation included"}
     print(json.dumps(line), flush=True)
     ch.close()
     ctx.close()
@@ -457,6 +492,25 @@ def bench_c3(args) -> int:
                  batch_s * 1e3, "u8/i32/u16",
                  f"{args.domains} domains, sides U[{lo},{hi}], Aztec/notched-rectangle dominoes, triangle-dimer cones, "
                  f"rect-triangle hexagons, {steps} steps each", extra)
+    if not args.no_cpu:
+        # the oracle on one domain of each kind (the largest), a bounded number of steps each
+        import oracle
+        walk = {rt.RT_DOMINO: lambda t, n, sd, p: oracle.domino_walk(t, n, sd, p, rule=1),
+                rt.RT_TRIDIMER: lambda t, n, sd, p: oracle.tridimer_walk(t, n, sd, p),
+                rt.RT_RECTTRI: lambda t, n, sd, p: oracle.recttri_walk(t, n, sd, p)}
+        cpu_att, cpu_s, picked = 0, 0.0, []
+        for kind in ("aztec", "notched", "tridimer", "recttri"):
+            k = max((i for i in range(len(specs)) if specs[i][3] == kind), key=lambda i: specs[i][2].size)
+            m, _, t, _ = specs[k]
+            cs = 20
+            w0 = time.perf_counter()
+            walk[m](t, cs, args.seed + k, default_table(gmax))
+            cpu_s += time.perf_counter() - w0
+            cpu_att += batched[k].attempts_per_step * cs
+            picked.append(f"{kind} side {int(round(np.sqrt(t.size)))}")
+        line["cpu_baseline"] = {"value": cpu_att / cpu_s, "unit": "flip attempts/s", "cores": 1, "kind": "port",
+                                "sample": "oracle (sequential C++ restatement) Walk of 20 steps on the largest domain "
+                                          "of each kind: " + ", ".join(picked)}
     print(json.dumps(line), flush=True)
     b.close()
     for ch in solo + batched:

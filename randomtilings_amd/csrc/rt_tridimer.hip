@@ -416,21 +416,38 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     const int rp = rotate_plane(mv);
     const int n = R.rows * R.cols;
     const int oi1 = min(M, oi + TR), oj1 = min(M, oj + TC);
-    // load the region, draw every site's number, rotate
-    for (int idx = threadIdx.x; idx < n; idx += NT) {
-        const int i = idx / R.cols, j = idx % R.cols;
-        const int gi = R.gi0 + i, gj = R.gj0 + j;
-        const size_t id = (size_t)gi * M + gj;
-        int v[8];
-        unpack_cell(a.cells_in[id], v);
-        uint4 g = a.gen_in[id];
-        uint32_t m1, m2, tm;
-        load_params<UNI>(a, id, m1, m2, tm);
-        const uint32_t u = tm_draw_raw(g.x, g.y, g.z, g.w, m1, m2, tm);
-        if (gi >= oi && gi < oi1 && gj >= oj && gj < oj1) a.gen_out[id] = g;
-        v[rp] = rotate_value(mv, v[rp], gi, gj, u < 0x80000000u);
+    // load the region, draw every site's number, rotate.  Every load of the thread's sites is
+    // issued before the first is used, so the loads overlap instead of waiting one by one.
+    constexpr int SPT = (RR * RC + NT - 1) / NT;  // sites per thread
+    uint4 cin[SPT], gin[SPT];
 #pragma unroll
-        for (int p = 0; p < 8; p++) V.s(p, i, j, v[p]);
+    for (int q = 0; q < SPT; q++) {
+        const int idx = threadIdx.x + q * NT;
+        if (idx < n) {
+            const int gi = R.gi0 + idx / R.cols, gj = R.gj0 + idx % R.cols;
+            const size_t id = (size_t)gi * M + gj;
+            cin[q] = a.cells_in[id];
+            gin[q] = a.gen_in[id];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < SPT; q++) {
+        const int idx = threadIdx.x + q * NT;
+        if (idx < n) {
+            const int i = idx / R.cols, j = idx % R.cols;
+            const int gi = R.gi0 + i, gj = R.gj0 + j;
+            const size_t id = (size_t)gi * M + gj;
+            int v[8];
+            unpack_cell(cin[q], v);
+            uint4 g = gin[q];
+            uint32_t m1, m2, tm;
+            load_params<UNI>(a, id, m1, m2, tm);
+            const uint32_t u = tm_draw_raw(g.x, g.y, g.z, g.w, m1, m2, tm);
+            if (gi >= oi && gi < oi1 && gj >= oj && gj < oj1) a.gen_out[id] = g;
+            v[rp] = rotate_value(mv, v[rp], gi, gj, u < 0x80000000u);
+#pragma unroll
+            for (int p = 0; p < 8; p++) V.s(p, i, j, v[p]);
+        }
     }
     __syncthreads();
     replay_updates<NT>(V, R, mv);
