@@ -149,9 +149,10 @@ __device__ __forceinline__ void tri_flipped(const View& V, int i, int j, int gi,
     const int cl = mod3(gj - gi);
     if (cl == c) return;
     const int p = (cl == (c + 1) % 3) ? 1 : 0;
-    const int t1 = (4 - 3 * p) * 12, m1 = (4 - 3 * p) * 4;
-    const int t4 = (15 * p + 1) * 3, m4 = 15 * p + 1;
-    const int t16 = (4 - 3 * p) * 3, m16 = 4 - 3 * p;
+    // masks and the power-of-two divisors (4 - 3p) * 4, 15p + 1, 4 - 3p as shifts
+    const int t1 = (4 - 3 * p) * 12, s1 = p ? 2 : 4;
+    const int t4 = (15 * p + 1) * 3, s4 = p ? 4 : 0;
+    const int t16 = (4 - 3 * p) * 3, s16 = p ? 0 : 2;
     const int a = down ? TD : TU;
     int x1, x4, x16;
     if (!down) {
@@ -163,7 +164,7 @@ __device__ __forceinline__ void tri_flipped(const View& V, int i, int j, int gi,
         x4 = V.g(a, i + p, j + 1 - p);
         x16 = V.g(a, i - 1, j + p);
     }
-    V.s(a, i, j, (x1 & t1) / m1 + ((x4 & t4) / m4) * 4 + ((x16 & t16) / m16) * 16);
+    V.s(a, i, j, ((x1 & t1) >> s1) + (((x4 & t4) >> s4) << 2) + (((x16 & t16) >> s16) << 4));
 }
 
 // UpdateTriangles, :258-281: t = 0 down from up, t = 1 up from down
@@ -301,6 +302,16 @@ __device__ __forceinline__ void loz_from_bfly(const View& V, int i, int j, int t
 
 // ---- phase driver ----------------------------------------------------------------------
 
+// x / d by a multiply-high with m = floor(2^32 / d) + 1: exact for x < 2^17 and d <= 1024
+// (checked exhaustively), which covers every site index of a region (at most 76 x 76 sites).
+// A hardware division would cost dozens of instructions per site and phase.
+struct FastDiv {
+    uint32_t d, m;
+    __device__ __forceinline__ explicit FastDiv(uint32_t d_)
+        : d(d_), m(d_ > 1 ? (uint32_t)(0x100000000ull / d_ + 1) : 0u) {}
+    __device__ __forceinline__ uint32_t div(uint32_t x) const { return d > 1 ? __umulhi(x, m) : x; }
+};
+
 // Sites of a region a phase may compute: at least `m` away from every region edge that cuts
 // the domain, and inside the reference's update range 0 < gi, gj < M - 1.
 struct Region {
@@ -313,8 +324,10 @@ __device__ __forceinline__ void for_sites(const Region& R, int m, F f) {
     const int i0 = R.cutT ? m : 0, i1 = R.rows - (R.cutB ? m : 0);
     const int j0 = R.cutL ? m : 0, j1 = R.cols - (R.cutR ? m : 0);
     const int w = j1 - j0, n = (i1 - i0) * w;
+    const FastDiv fw((uint32_t)w);
     for (int idx = threadIdx.x; idx < n; idx += NT) {
-        const int i = i0 + idx / w, j = j0 + idx % w;
+        const int qi = (int)fw.div((uint32_t)idx);
+        const int i = i0 + qi, j = j0 + idx - qi * w;
         const int gi = R.gi0 + i, gj = R.gj0 + j;
         if (gi > 0 && gj > 0 && gi < R.M - 1 && gj < R.M - 1) f(i, j, gi, gj);
     }
@@ -419,12 +432,14 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     // load the region, draw every site's number, rotate.  Every load of the thread's sites is
     // issued before the first is used, so the loads overlap instead of waiting one by one.
     constexpr int SPT = (RR * RC + NT - 1) / NT;  // sites per thread
+    const FastDiv fc((uint32_t)R.cols);
     uint4 cin[SPT], gin[SPT];
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
         const int idx = threadIdx.x + q * NT;
         if (idx < n) {
-            const int gi = R.gi0 + idx / R.cols, gj = R.gj0 + idx % R.cols;
+            const int qi = (int)fc.div((uint32_t)idx);
+            const int gi = R.gi0 + qi, gj = R.gj0 + idx - qi * R.cols;
             const size_t id = (size_t)gi * M + gj;
             cin[q] = a.cells_in[id];
             gin[q] = a.gen_in[id];
@@ -434,7 +449,7 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     for (int q = 0; q < SPT; q++) {
         const int idx = threadIdx.x + q * NT;
         if (idx < n) {
-            const int i = idx / R.cols, j = idx % R.cols;
+            const int i = (int)fc.div((uint32_t)idx), j = idx - i * R.cols;
             const int gi = R.gi0 + i, gj = R.gj0 + j;
             const size_t id = (size_t)gi * M + gj;
             int v[8];
@@ -454,8 +469,10 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     __syncthreads();
     // write the owned tile
     const int ow = oj1 - oj, on = (oi1 - oi) * ow;
+    const FastDiv fo((uint32_t)ow);
     for (int idx = threadIdx.x; idx < on; idx += NT) {
-        const int gi = oi + idx / ow, gj = oj + idx % ow;
+        const int qi = (int)fo.div((uint32_t)idx);
+        const int gi = oi + qi, gj = oj + idx - qi * ow;
         const int i = gi - R.gi0, j = gj - R.gj0;
         uint4 c;
         c.x = V.g(0, i, j) | (V.g(1, i, j) << 16);
@@ -519,6 +536,14 @@ __global__ __launch_bounds__(kSmallNT) void k_td_small(TdArgs a) {
             for (int p = 0; p < 8; p++) V.s(p, id / M, id % M, v[p]);
         }
     }
+    // the sites' coordinates, once (a division per site and step would dominate the step)
+    int si[kSmallPer], sj[kSmallPer];
+#pragma unroll
+    for (int k = 0; k < kSmallPer; k++) {
+        const int id = threadIdx.x + k * kSmallNT;
+        si[k] = id / M;
+        sj[k] = id - si[k] * M;
+    }
     __syncthreads();
     for (int64_t s = 0; s < a.nsteps; s++) {
         const Move mv = decode(a.sched[s]);
@@ -528,7 +553,7 @@ __global__ __launch_bounds__(kSmallNT) void k_td_small(TdArgs a) {
             const int id = threadIdx.x + k * kSmallNT;
             if (id < n) {
                 const uint32_t u = tm_draw_raw(g[k].x, g[k].y, g[k].z, g[k].w, pm1[k], pm2[k], ptm[k]);
-                const int gi = id / M, gj = id % M;
+                const int gi = si[k], gj = sj[k];
                 const int v = V.g(rp, gi, gj);
                 const int w = rotate_value(mv, v, gi, gj, u < 0x80000000u);
                 if (w != v) V.s(rp, gi, gj, w);
