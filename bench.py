@@ -61,9 +61,10 @@ def main() -> int:
     ap.add_argument("--sides", type=int, nargs=2, default=[64, 512], help="c3: domain side range")
     ap.add_argument("--domain-steps", type=int, default=1000, help="c3: chain steps per domain")
     ap.add_argument("--canvas", type=int, default=64, help="c5: largest vertex grid side (training canvas)")
-    ap.add_argument("--train-domains", type=int, default=48, help="c5: training domains")
+    ap.add_argument("--train-domains", type=int, default=96, help="c5: training domains")
+    ap.add_argument("--samples-per-domain", type=int, default=16, help="c5: chain samples averaged per domain")
     ap.add_argument("--eq-steps", type=int, default=30000, help="c5: chain steps per training sample")
-    ap.add_argument("--epochs", type=int, default=30, help="c5: training epochs")
+    ap.add_argument("--epochs", type=int, default=200, help="c5: training epochs")
     ap.add_argument("--horizon", type=int, default=40000, help="c5: chain steps of each evaluation trajectory")
     args = ap.parse_args()
 
@@ -521,7 +522,7 @@ def bench_c3(args) -> int:
 
 def bench_c5(args) -> int:
     """Learned warm start: train the U-Net (4 levels, 32 base channels, bf16 matrix cores) on
-    chain-equilibrated height functions of random domino domains (Aztec diamonds and notched
+    the mean of --samples-per-domain chain-equilibrated height functions of random domino domains (Aztec diamonds and notched
     squares fitting a --canvas vertex grid), then compare chain steps to equilibrium (mean
     interior height within 1% of the max-min range of its long-run value) from the projected
     prediction and from the minimal tiling on held-out domains."""
@@ -534,8 +535,8 @@ def bench_c5(args) -> int:
     ctx.load_params(default_table(args.canvas * args.canvas))
     t0 = time.perf_counter()
     doms = [ws.Domain(ctx, d) for _, d in ws.domain_bank(rng, args.train_domains, args.canvas)]
-    samples = ws.equilibrium_samples(ctx, doms, args.eq_steps, 8, args.seed)
-    X, T, M = ws.build_dataset(ctx, doms, samples, args.canvas)
+    samples = ws.equilibrium_samples(ctx, doms, args.eq_steps, args.samples_per_domain, args.seed)
+    X, T, M = ws.build_dataset(ctx, doms, samples, args.canvas, average=True)
     data_s = time.perf_counter() - t0
     net = rt.UNet(ctx, 4, 32, ws.IN_CH, seed=args.seed)
     t0 = time.perf_counter()
@@ -545,20 +546,42 @@ def bench_c5(args) -> int:
     results = []
     held = [ws.Domain(ctx, d) for _, d in ws.domain_bank(np.random.default_rng(args.seed + 99), 4, args.canvas)]
     for k, dom in enumerate(held):
-        tw = ws.warm_tiling(ctx, net, dom, args.canvas)
+        ws.warm_tiling(ctx, net, dom, args.canvas)  # warm-up of the inference path
+        w0 = time.perf_counter()
+        tw = ws.warm_tiling(ctx, net, dom, args.canvas)  # U-Net forward, projection, tiling read-off
+        prep_s = time.perf_counter() - w0
         plain = ws.trajectory(ctx, dom, dom.tmin, args.horizon, every, 77 + k)
         warm = ws.trajectory(ctx, dom, tw, args.horizon, every, 77 + k)
         eq = 0.5 * (plain[len(plain) // 2:].mean() + warm[len(warm) // 2:].mean())
         sp = ws.steps_to_equilibrium(plain, eq, every)
         sw = ws.steps_to_equilibrium(warm, eq, every)
+        # wall clock of one chain on this domain: seconds per chain step
+        ch = ctx.chain(rt.RT_DOMINO, rt.RT_DOMINO_CFTP, dom.tmin)
+        ch.seed(5)
+        ch.run(2000)
+        ch.sync()
+        w0 = time.perf_counter()
+        ch.run(20000)
+        ch.sync()
+        step_s = (time.perf_counter() - w0) / 20000
+        ch.close()
+        r = ws.predicted_fraction(net, dom, args.canvas)
+        inn = dom.inner
+        pred_obs = float(((dom.hmin[inn] + r[inn] * (dom.hmax[inn] - dom.hmin[inn])).mean() - dom.hmin[inn].mean()) /
+                         (dom.hmax[inn].mean() - dom.hmin[inn].mean()))
         results.append({"N": dom.N, "plain_steps": sp, "warm_steps": sw, "eq_fraction": eq,
-                        "warm_start_fraction": float(warm[0]), "plain_start_fraction": float(plain[0])})
+                        "predicted_fraction": pred_obs,
+                        "warm_start_fraction": float(warm[0]), "plain_start_fraction": float(plain[0]),
+                        "seconds_per_chain_step": step_s, "warm_start_prep_seconds": prep_s,
+                        "plain_seconds_to_eq": sp * step_s, "warm_seconds_to_eq": prep_s + sw * step_s})
     ratio = float(np.mean([r["plain_steps"] for r in results]) / max(1.0, np.mean([r["warm_steps"] for r in results])))
-    extra = {"canvas": args.canvas, "train_domains": args.train_domains, "train_samples": int(len(X)),
-             "eq_steps_per_sample": args.eq_steps, "unet": "4 levels x 32 base channels, bf16 MFMA, fp32 accumulate",
+    extra = {"canvas": args.canvas, "train_domains": args.train_domains, "train_examples": int(len(X)),
+             "chain_samples_averaged_per_domain": args.samples_per_domain, "eq_steps_per_sample": args.eq_steps, "unet": "4 levels x 32 base channels, bf16 MFMA, fp32 accumulate",
              "params": net.param_count, "epochs": args.epochs, "final_loss": float(np.mean(losses[-10:])),
              "first_loss": float(losses[0]), "data_seconds": data_s, "train_seconds": train_s,
-             "held_out": results, "steps_plain_over_warm": ratio}
+             "held_out": results, "steps_plain_over_warm": ratio,
+             "wall_clock_plain_over_warm": float(np.mean([r["plain_seconds_to_eq"] for r in results]) /
+                                                 np.mean([r["warm_seconds_to_eq"] for r in results]))}
     line = _line("sweeps-to-equilibrium, plain / warm (domino, learned warm start)", ratio, "x", args,
                  train_s * 1e3, "bf16/fp32", "warm start from the projected U-Net prediction vs the minimal tiling",
                  extra)

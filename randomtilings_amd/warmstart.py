@@ -131,13 +131,25 @@ def equilibrium_samples(ctx: rt.Context, doms, steps: int, per_domain: int, seed
     return out
 
 
-def build_dataset(ctx, doms, samples, canvas):
+def build_dataset(ctx, doms, samples, canvas, average: bool = False):
+    """Inputs, targets and masks.  With average, one example per domain whose target is the
+    mean fraction over that domain's samples: an estimate of the expected height with far less
+    noise than any single sample, which is what the network is asked to predict."""
     X, T, M = [], [], []
+    acc = {}
     for i, t in samples:
         dom = doms[i]
-        X.append(features(dom, canvas))
         tt, mm = target_fraction(dom, dom.heights(t), canvas)
-        T.append(tt)
+        if average:
+            s, n, _ = acc.get(i, (0.0, 0, mm))
+            acc[i] = (s + tt, n + 1, mm)
+        else:
+            X.append(features(dom, canvas))
+            T.append(tt)
+            M.append(mm)
+    for i, (s, n, mm) in sorted(acc.items()):
+        X.append(features(doms[i], canvas))
+        T.append((s / n).astype(np.float32))
         M.append(mm)
     return np.stack(X), np.stack(T), np.stack(M)
 
@@ -155,19 +167,44 @@ def train(net: rt.UNet, X, T, M, epochs: int, batch: int, lr: float, rng) -> lis
 
 # ---- warm start ------------------------------------------------------------------------------
 
-def warm_heights(ctx: rt.Context, net: rt.UNet, dom: Domain, canvas: int) -> np.ndarray:
-    """The network's prediction made an admissible height function."""
+def predicted_fraction(net: rt.UNet, dom: Domain, canvas: int) -> np.ndarray:
+    """The network's fraction field on the domain's vertices (clipped to [0, 1])."""
     N = dom.N
-    r = net.forward(features(dom, canvas)[None])[0, :N, :N].ravel()
-    r = np.clip(r, 0.0, 1.0)
+    return np.clip(net.forward(features(dom, canvas)[None])[0, :N, :N].ravel(), 0.0, 1.0)
+
+
+def project(ctx: rt.Context, dom: Domain, hp: np.ndarray) -> np.ndarray:
+    """An admissible height function near the real-valued field hp: rounded to the residue
+    class of domino heights (h = h_min mod 4), clamped between h_min and h_max, and closed on
+    the device (the largest admissible function below it, which keeps the boundary)."""
     h = dom.hmin.astype(np.float64).copy()
     inn = dom.inner
-    pred = dom.hmin[inn] + r[inn] * (dom.hmax[inn] - dom.hmin[inn])
-    h[inn] = dom.hmin[inn] + 4 * np.round((pred - dom.hmin[inn]) / 4.0)
+    h[inn] = dom.hmin[inn] + 4 * np.round((hp[inn] - dom.hmin[inn]) / 4.0)
     h = np.clip(h, dom.hmin, dom.hmax).astype(np.int64)
     h = np.where(dom.hmax == INF, INF, h).astype(np.int32)
     closed, _ = ctx.height_closure(rt.RT_DOMINO, dom.d, h, 1, dom.fixed)
     return closed
+
+
+def warm_heights(ctx: rt.Context, net: rt.UNet, dom: Domain, canvas: int, rounds: int = 3) -> np.ndarray:
+    """The network's prediction made an admissible height function whose observable matches
+    the prediction's: rounding and closure move the mean height, so the real-valued field is
+    shifted by the difference and projected again (a few rounds)."""
+    r = predicted_fraction(net, dom, canvas)
+    inn = dom.inner
+    hp = dom.hmin.astype(np.float64).copy()
+    hp[inn] = dom.hmin[inn] + r[inn] * (dom.hmax[inn] - dom.hmin[inn])
+    lo, hi = dom.hmin[inn].mean(), dom.hmax[inn].mean()
+    want = (hp[inn].mean() - lo) / (hi - lo)
+    shift = 0.0
+    best, best_err = None, np.inf
+    for _ in range(max(1, rounds)):
+        h = project(ctx, dom, hp + shift)
+        err = dom.observable(h) - want
+        if abs(err) < best_err:
+            best, best_err = h, abs(err)
+        shift -= err * (hi - lo)
+    return best
 
 
 def warm_tiling(ctx, net, dom: Domain, canvas: int) -> np.ndarray:

@@ -19,11 +19,12 @@ def test_warm_start_beats_the_minimal_start():
     ctx = rt.Context(0)
     ctx.load_params(default_table(32 * 32))
     canvas = 32
-    doms = [ws.Domain(ctx, d) for _, d in ws.domain_bank(rng, 16, canvas)]
-    samples = ws.equilibrium_samples(ctx, doms, 6000, 4, 11)
-    X, T, M = ws.build_dataset(ctx, doms, samples, canvas)
+    doms = [ws.Domain(ctx, d) for _, d in ws.domain_bank(rng, 32, canvas)]
+    samples = ws.equilibrium_samples(ctx, doms, 6000, 8, 11)
+    X, T, M = ws.build_dataset(ctx, doms, samples, canvas, average=True)
+    assert len(X) == 32
     net = rt.UNet(ctx, 3, 16, ws.IN_CH, seed=2)
-    losses = ws.train(net, X, T, M, 25, 8, 3e-3, rng)
+    losses = ws.train(net, X, T, M, 150, 8, 3e-3, rng)
     print("loss", losses[0], losses[-1], flush=True)
     assert np.mean(losses[-5:]) < 0.5 * losses[0]
     gains = []
