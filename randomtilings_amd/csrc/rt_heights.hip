@@ -135,7 +135,7 @@ void masks(const Geom& geo, const std::vector<int>& vertices, int dir, bool loze
     const int N = geo.N;
     pull.assign((size_t)N * N, 0);
     rmask.assign((size_t)N * N, 0);
-    for (int i = 0; i < N; i++)
+    rtlat::for_rows(N, [&](int i) {
         for (int j = 0; j < N; j++) {
             const size_t v = (size_t)i * N + j;
             for (int nb = 0; nb < Geom::DEG; nb++) {
@@ -156,6 +156,7 @@ void masks(const Geom& geo, const std::vector<int>& vertices, int dir, bool loze
                 if (a || b) rmask[v] |= (uint8_t)(1u << nb);
             }
         }
+    });
 }
 
 template <class Geom>
@@ -237,10 +238,12 @@ int height_closure_device(int device, hipStream_t s, int model, const std::vecto
     }
     const size_t n = (size_t)N * N;
     std::vector<int32_t> g(n);
-    for (size_t k = 0; k < n; k++) {
-        g[k] = heights[k] == kInf ? kInf : dir * heights[k];
-        if (fixed[k]) pull[k] |= 0x80;
-    }
+    rtlat::for_rows(N, [&](int i) {
+        for (size_t k = (size_t)i * N; k < (size_t)(i + 1) * N; k++) {
+            g[k] = heights[k] == kInf ? kInf : dir * heights[k];
+            if (fixed[k]) pull[k] |= 0x80;
+        }
+    });
     DevBufs b;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipMalloc(&b.g, n * 4);

@@ -14,9 +14,27 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <thread>
 #include <vector>
 
 namespace rtlat {
+
+// f(i) for every row i in [0, n), the rows split over up to 16 host threads (row loops over
+// vertex arrays of large domains; every row writes only its own entries).
+template <class F>
+void for_rows(int n, F f) {
+    const int t = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(), n / 64}));
+    if (t <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int k = 0; k < t; ++k)
+        pool.emplace_back([&, k] {
+            for (int i = k * n / t; i < (k + 1) * n / t; ++i) f(i);
+        });
+    for (auto& th : pool) th.join();
+}
 
 inline int side_of(size_t n) { return (int)std::lround(std::sqrt((double)n)); }
 
@@ -78,13 +96,14 @@ inline std::vector<int> domino_vertices(const std::vector<int>& d) {
         return d[(size_t)fi * M + fj] != 0;
     };
     std::vector<int> v((size_t)N * N, 0);
-    for (int i = 0; i < N; ++i)
+    for_rows(N, [&](int i) {
         for (int j = 0; j < N; ++j) {
             const bool a = at(i - 1, j - 1), b = at(i - 1, j), c = at(i, j - 1), e = at(i, j);
             const bool border = (i == 0 || j == 0 || i == M || j == M);
             if (!border && a && b && c && e) v[(size_t)i * N + j] = 1;
             else if (a || b || c || e) v[(size_t)i * N + j] = 2;
         }
+    });
     return v;
 }
 
@@ -93,7 +112,8 @@ inline std::vector<int> lozenge_vertices(const std::vector<int>& d) {
     const int N = M + 1;
     std::vector<int> v((size_t)N * N, 0);
     auto f = [&](int r, int c) { return d[(size_t)r * 2 * M + c] != 0; };
-    for (int i = 1; i < M; ++i)
+    for_rows(M, [&](int i) {
+        if (i < 1) return;
         for (int j = 1; j < M; ++j) {
             // the six triangles around vertex (i,j)
             const bool a = f(i, 2 * j), b = f(i, 2 * j - 1), c = f(i, 2 * j - 2);
@@ -101,6 +121,7 @@ inline std::vector<int> lozenge_vertices(const std::vector<int>& d) {
             if (a && b && c && e && g && h) v[(size_t)i * N + j] = 1;
             else if (a || b || c || e || g || h) v[(size_t)i * N + j] = 2;
         }
+    });
     return v;
 }
 
