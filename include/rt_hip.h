@@ -236,6 +236,25 @@ int64_t rt_chain_state_bytes(rt_chain* ch);
 int64_t rt_chain_last_launches(rt_chain* ch);
 int32_t rt_chain_last_config(rt_chain* ch);
 
+/* Skipped tiles (lozenge chains, packed sweep).  The reference rotates every position of the
+ * N x N/3 class arrays (lozengekernel.cl:11-12); positions outside the domain hold code 0,
+ * which never flips, and a cell whose neighbourhood is all zero is never rewritten by the
+ * update.  When the uploaded tiling is a fixed point of the update (every valid tiling is),
+ * tiles whose cells, halo and a two-cell margin are all zero never change, and the packed
+ * persistent sweep does not run them: the tiling after any number of steps is bit-identical.
+ * The sweep checks while it runs that no change reaches a skipped tile (rt_chain_sync then
+ * returns RT_ERR_DEVICE).  No counterpart in the reference.
+ * rt_chain_set_skip: enable (default) or disable skipping for this chain.
+ * rt_chain_last_grid: tiles of the most recent persistent run's grid and how many ran.
+ * rt_lozenge_skip_map: host only (no device): for an N x N lozenge tiling and a tile of
+ * tr x tc positions (rows x class columns) with halo h, one byte per tile, row-major, 1 =
+ * runs (map may be NULL; every tile runs when the tiling is not a fixed point of the update),
+ * the number that run, and whether the tiling is a fixed point. */
+int rt_chain_set_skip(rt_chain* ch, int enable);
+int rt_chain_last_grid(rt_chain* ch, int32_t* tiles, int32_t* running);
+int rt_lozenge_skip_map(const int32_t* tiling, int32_t N, int32_t tr, int32_t tc, int32_t h, uint8_t* map,
+                        int32_t* running, int32_t* consistent);
+
 /* ---- batches of independent chains (no counterpart in the reference: its callers loop
  * over tilers) ---------------------------------------------------------------------------
  * A batch runs many seeded chains of one context together: the domino, lozenge and

@@ -75,6 +75,10 @@ SIGNATURES = {
     "rt_tile_config": (C.c_int, [C.c_int, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
     "rt_chain_last_launches": (C.c_int64, [_P]),
     "rt_chain_last_config": (C.c_int32, [_P]),
+    "rt_chain_set_skip": (C.c_int, [_P, C.c_int]),
+    "rt_chain_last_grid": (C.c_int, [_P, _I32P, _I32P]),
+    "rt_lozenge_skip_map": (C.c_int, [_I32P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_uint8), _I32P,
+                                      _I32P]),
     "rt_chain_tile_configs": (C.c_int32, [_P]),
     "rt_chain_tile_config_info": (C.c_int, [_P, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
     "rt_unet_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int64, C.POINTER(_P)]),
@@ -135,6 +139,21 @@ def _check(rc: int) -> None:
 
 def _i32(a: np.ndarray):
     return a.ctypes.data_as(_I32P)
+
+
+def lozenge_skip_map(tiling: np.ndarray, tr: int, tc: int, h: int):
+    """Host-only (no device): which tiles of tr x tc positions (rows x class columns, halo h)
+    the packed lozenge sweep runs for this N x N tiling (rt_lozenge_skip_map).  Returns
+    (map of shape (ceil(N/tr), ceil(N/3/tc)), 1 = runs; whether the tiling is a fixed point of
+    the update)."""
+    t = np.ascontiguousarray(tiling, dtype=np.int32).ravel()
+    N = int(round(np.sqrt(t.size)))
+    W = N // 3
+    m = np.zeros(((N + tr - 1) // tr) * ((W + tc - 1) // tc), dtype=np.uint8)
+    run, ok = C.c_int32(), C.c_int32()
+    _check(load().rt_lozenge_skip_map(_i32(t), N, tr, tc, h, m.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(run),
+                                      C.byref(ok)))
+    return m.reshape((N + tr - 1) // tr, (W + tc - 1) // tc), bool(ok.value)
 
 
 def rule_thresholds(model: int, rule: int, rparams=None) -> np.ndarray:
@@ -360,6 +379,17 @@ class Chain:
     @property
     def last_config(self) -> int:
         return int(self._lib.rt_chain_last_config(self.h))
+
+    def set_skip(self, enable: bool) -> None:
+        """Skip (default) or run the tiles whose cells never change (lozenge, packed sweep)."""
+        _check(self._lib.rt_chain_set_skip(self.h, int(bool(enable))))
+
+    @property
+    def last_grid(self):
+        """(tiles of the most recent persistent run's grid, tiles that ran)."""
+        t, r = C.c_int32(), C.c_int32()
+        _check(self._lib.rt_chain_last_grid(self.h, C.byref(t), C.byref(r)))
+        return t.value, r.value
 
     def tile_configs(self):
         out = []

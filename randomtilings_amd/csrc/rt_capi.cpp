@@ -23,6 +23,10 @@ namespace rt {
 bool domino_rule(int rule, const double* p12, FlipRule& fr);
 bool lozenge_rule(int rule, double q, FlipRule& fr);
 bool recttri_rule(const float* weights16, RectTriRule& r);
+bool lozenge_consistent(const uint8_t* cls, int N, int W, int P);
+void lozenge_nonzero_counts(const uint8_t* cls, int N, int P, std::vector<int32_t>& ps);
+int lozenge_live_tiles(const std::vector<int32_t>& ps, int N, int W, int P, int TR, int TC, int H,
+                       std::vector<uint8_t>& map);
 }  // namespace rt
 
 using namespace rt;
@@ -95,7 +99,67 @@ struct rt_chain {
     int64_t last_launches = 0;
     int32_t last_cfg = -1;
     bool counted = false;  // counted in ctx->live_chains (chains handed out by rt_chain_create)
+    // Lozenge chains: tiles whose cells never change are not run by the packed persistent
+    // sweep (rt_skip.cpp).  consistent: the uploaded tiling is a fixed point of the update;
+    // runs[cfg]: which tiles of that configuration run (built at creation, uploaded on use).
+    struct Runs {
+        std::vector<uint8_t> map;  // one byte per tile, row-major; empty: every tile runs
+        std::vector<int32_t> list;
+        int32_t* d_list = nullptr;
+        uint8_t* d_map = nullptr;
+    };
+    bool skip = true;
+    bool consistent = false;
+    std::vector<Runs> runs;
+    int32_t last_tiles = 0, last_running = 0;
 };
+
+// The tiles a persistent run of configuration cfg skips, or nullptr when every tile runs.
+static rt_chain::Runs* skip_runs(rt_chain* ch, int cfg) {
+    if (!ch->skip || !ch->consistent || cfg < 0 || cfg >= (int)ch->runs.size()) return nullptr;
+    rt_chain::Runs& r = ch->runs[cfg];
+    return r.map.empty() ? nullptr : &r;
+}
+
+static void build_runs(rt_chain* ch, const std::vector<uint8_t>& h) {
+    ch->runs.clear();
+    if (ch->model != RT_LOZENGE) return;
+    ch->consistent = lozenge_consistent(h.data(), ch->N, ch->W, ch->P);
+    if (!ch->consistent) return;
+    std::vector<int32_t> ps;
+    lozenge_nonzero_counts(h.data(), ch->N, ch->P, ps);
+    const int n = persist_config_count(ch->model);
+    ch->runs.resize(n);
+    for (int i = 0; i < n; i++) {
+        int tr, tc, hh, nt;
+        if (!persist_config_packed(ch->model, i) || !persist_config(ch->model, i, &tr, &tc, &hh, &nt)) continue;
+        std::vector<uint8_t> map;
+        const int running = lozenge_live_tiles(ps, ch->N, ch->W, ch->P, tr, tc, hh, map);
+        if (running == (int)map.size() || running == 0) continue;
+        rt_chain::Runs& r = ch->runs[i];
+        r.map = map;
+        for (int t = 0; t < (int)map.size(); t++)
+            if (map[t]) r.list.push_back(t);
+    }
+}
+
+static hipError_t upload_runs(rt_chain::Runs& r) {
+    if (r.d_list) return hipSuccess;
+    hipError_t e = hipMalloc(&r.d_list, r.list.size() * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&r.d_map, r.map.size());
+    if (e == hipSuccess) e = hipMemcpy(r.d_list, r.list.data(), r.list.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(r.d_map, r.map.data(), r.map.size(), hipMemcpyHostToDevice);
+    return e;
+}
+
+static void free_runs(rt_chain* ch) {
+    for (auto& r : ch->runs) {
+        if (r.d_list) hipFree(r.d_list);
+        if (r.d_map) hipFree(r.d_map);
+        r.d_list = nullptr;
+        r.d_map = nullptr;
+    }
+}
 
 // Tile configurations chosen by rt_chain_autotune, keyed by (model, side, uniform params).
 static std::mutex g_tune_mu;
@@ -548,6 +612,7 @@ int rt_chain_destroy(rt_chain* ch) {
         if (ch->cls[b]) hipFree(ch->cls[b]);
         if (ch->gens[b]) hipFree(ch->gens[b]);
     }
+    free_runs(ch);
     if (ch->dsched) hipFree(ch->dsched);
     if (ch->tsave) hipFree(ch->tsave);
     if (ch->misc) hipFree(ch->misc);
@@ -666,6 +731,7 @@ int rt_chain_create(rt_ctx* ctx, int model, int rule, const double* rparams, int
     }
     std::vector<uint8_t> h;
     split(ch, tiling, h);
+    build_runs(ch, h);
     e = hipMemcpy(ch->cls[0], h.data(), bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ch->cls[1], h.data(), bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(ch->misc, 0, 64);
@@ -731,8 +797,10 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* re
         const int trn = (ch->N + tr - 1) / tr, tcn = (ch->W + tc - 1) / tc;
         const int cap = persist_capacity(ch->model, idx, uni, ch->ctx->device);
         if (cap <= 0) return false;
-        *res = trn * tcn <= cap && trn * tcn <= kMaxBlocks && !persist_uses_scratch(ch->model, idx, uni);
-        *nblocks = trn * tcn;
+        const rt_chain::Runs* r = skip_runs(ch, idx);
+        const int running = r ? (int)r->list.size() : trn * tcn;  // workgroups: skipped tiles do not run
+        *res = running <= cap && trn * tcn <= kMaxBlocks && !persist_uses_scratch(ch->model, idx, uni);
+        *nblocks = running;
         *tiles_c = tcn;
         return true;
     };
@@ -780,6 +848,10 @@ int rt_chain_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
     t->esize = ch->esize;
     t->G = ch->G;
     t->rp = ch->rp;
+    t->skip = ch->skip;
+    t->consistent = ch->consistent;
+    t->runs = ch->runs;
+    for (auto& r : t->runs) r.d_list = nullptr, r.d_map = nullptr;  // the clone uploads its own
     const size_t bytes = (size_t)ch->N * ch->P * ch->NC * ch->esize;
     hipError_t e = hipSuccess;
     for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&t->cls[b], bytes);
@@ -896,6 +968,10 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         ch->sched_pending = true;
         int tr, tc, h, nt;
         persist_config(ch->model, cfg, &tr, &tc, &h, &nt);
+        rt_chain::Runs* runs = skip_runs(ch, cfg);
+        if (runs) HIPCHK(upload_runs(*runs));
+        ch->last_tiles = ((ch->N + tr - 1) / tr) * tiles_c;
+        ch->last_running = nblocks;
         HIPCHK(hipEventRecord(ch->ev0, c->stream));
         // one launch per chunk keeps single kernels short; chunk is a multiple of h.  A grid
         // that is not resident runs one block of h steps per launch.
@@ -923,6 +999,9 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
             a.epoch0 = ch->epoch;
             a.err = (int*)(ch->misc + 1);
             a.spin_limit = 1u << 22;
+            a.live = runs ? runs->d_list : nullptr;
+            a.live_map = runs ? runs->d_map : nullptr;
+            a.ntiles = ch->last_tiles;
             HIPCHK(launch_persist(ch->model, cfg, c->uniform, a, nblocks, c->stream));
             ch->gcur ^= 1;
             const int64_t nblk = (ns + h - 1) / h;  // each block of h steps publishes once
@@ -943,7 +1022,52 @@ int rt_chain_sync(rt_chain* ch) {
     HIPCHK(hipStreamSynchronize(ch->ctx->stream));
     uint32_t m[2];
     HIPCHK(hipMemcpy(m, ch->misc, sizeof(m), hipMemcpyDeviceToHost));
+    if (m[1] == 2)
+        return fail(RT_ERR_DEVICE, "a skipped tile's cells would have changed: the tiling is not closed under the "
+                                   "chain (rt_chain_set_skip(ch, 0) runs every tile)");
     if (m[1] != 0) return fail(RT_ERR_DEVICE, "device error flag raised (grid barrier wait expired)");
+    return RT_OK;
+}
+
+int rt_chain_set_skip(rt_chain* ch, int enable) {
+    if (!ch) return fail(RT_ERR_ARG, "rt_chain_set_skip: null chain");
+    ch->skip = enable != 0;
+    return RT_OK;
+}
+
+int rt_chain_last_grid(rt_chain* ch, int32_t* tiles, int32_t* running) {
+    if (!ch) return fail(RT_ERR_ARG, "rt_chain_last_grid: null chain");
+    if (tiles) *tiles = ch->last_cfg >= 0 ? ch->last_tiles : 0;
+    if (running) *running = ch->last_cfg >= 0 ? ch->last_running : 0;
+    return RT_OK;
+}
+
+int rt_lozenge_skip_map(const int32_t* tiling, int32_t N, int32_t tr, int32_t tc, int32_t h, uint8_t* map,
+                        int32_t* running, int32_t* consistent) {
+    if (!tiling || N < 3 || tr <= 0 || tc <= 0 || h < 0) return fail(RT_ERR_ARG, "rt_lozenge_skip_map: bad arguments");
+    for (int64_t i = 0; i < (int64_t)N * N; i++)
+        if (tiling[i] < 0 || tiling[i] > 63) return fail(RT_ERR_ARG, "lozenge vertex code %d out of range", tiling[i]);
+    rt_chain ch;
+    ch.model = RT_LOZENGE;
+    ch.N = N;
+    ch.W = N / 3;
+    ch.NC = 3;
+    ch.esize = 1;
+    ch.P = (int)(((size_t)ch.W + 15) / 16 * 16);
+    std::vector<uint8_t> cls;
+    split(&ch, tiling, cls);
+    const bool ok = lozenge_consistent(cls.data(), N, ch.W, ch.P);
+    std::vector<int32_t> ps;
+    lozenge_nonzero_counts(cls.data(), N, ch.P, ps);
+    std::vector<uint8_t> m;
+    int run = lozenge_live_tiles(ps, N, ch.W, ch.P, tr, tc, h, m);
+    if (!ok) {
+        std::fill(m.begin(), m.end(), 1);
+        run = (int)m.size();
+    }
+    if (map) std::memcpy(map, m.data(), m.size());
+    if (running) *running = run;
+    if (consistent) *consistent = ok;
     return RT_OK;
 }
 

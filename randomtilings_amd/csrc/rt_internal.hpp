@@ -58,6 +58,12 @@ struct PersistArgs {
     uint32_t epoch0;  // block epochs completed before this launch
     int* err;
     uint32_t spin_limit;
+    // Skipped tiles (packed lozenge kernel): tiles whose cells can never change are not run.
+    // live lists the tiles that run, one workgroup each, and live_map[t] is 1 for those;
+    // ntiles counts every tile of the grid.  live == nullptr: every tile runs.
+    const int32_t* live;
+    const uint8_t* live_map;
+    int ntiles;
     // batched runs (blocked path): every block looks up its chain; the steps done so far in
     // the run are read from step_ctr (a device counter, so one captured round replays as is)
     const BatchEntry* batch;
@@ -109,6 +115,7 @@ bool boundary_heights_host(int model, const std::vector<int>& d, std::vector<int
 
 int persist_config_count(int model);
 bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt);
+bool persist_config_packed(int model, int idx);
 int persist_capacity(int model, int idx, bool uni, int device);
 bool persist_uses_scratch(int model, int idx, bool uni);
 hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st);

@@ -360,7 +360,11 @@ struct Cfg {
     X(64, 84, 8, 1024, 0)   \
     X(32, 84, 4, 512, 1)    \
     X(64, 44, 4, 512, 1)    \
-    X(48, 56, 4, 512, 1)
+    X(48, 56, 4, 512, 1)    \
+    X(68, 68, 4, 768, 1)    \
+    X(56, 84, 4, 768, 1)    \
+    X(80, 56, 4, 768, 1)    \
+    X(68, 68, 4, 512, 1)
 #define RT_RECTTRI_CFGS(X)  \
     X(32, 32, 4, 256, 0)    \
     X(48, 48, 8, 512, 0)    \
@@ -397,6 +401,12 @@ bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt) {
     *h = t[idx].h;
     *nt = t[idx].nt;
     return true;
+}
+
+bool persist_config_packed(int model, int idx) {
+    int n = 0;
+    const Cfg* t = cfg_table(model, &n);
+    return t && idx >= 0 && idx < n && t[idx].packed != 0;
 }
 
 template <class M, int TR, int TC, int H, int NT, int PACKED>

@@ -146,11 +146,35 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
         }
     };
 
+    // a neighbour that is skipped (its cells never change) publishes nothing and is not
+    // waited for; its cells keep their initial values in both copies
+    auto skipped = [&](int rr, int cc) {
+        return rr >= 0 && rr < tiles_r && cc >= 0 && cc < a.tiles_c && a.live_map != nullptr &&
+               a.live_map[rr * a.tiles_c + cc] == 0;
+    };
     int nbr_tile = -1;
     if (tid < 9 && tid != 4) {
         const int dr = tid / 3 - 1, dc = tid % 3 - 1;
         const int rr = tr + dr, cc = tc + dc;
-        if (rr >= 0 && rr < tiles_r && cc >= 0 && cc < a.tiles_c) nbr_tile = rr * a.tiles_c + cc;
+        if (rr >= 0 && rr < tiles_r && cc >= 0 && cc < a.tiles_c && !skipped(rr, cc)) nbr_tile = rr * a.tiles_c + cc;
+    }
+    // Halo words inside a skipped neighbour start at zero (the host skips a tile only when
+    // every cell within its halo and a margin is zero).  This tile's own computation of them
+    // is exact one cell deep for a whole block, so the first change that would reach a
+    // skipped tile shows here: they are checked after every block and a change raises the
+    // error flag (value 2) instead of going unnoticed.
+    uint32_t wmask = 0;
+    if (a.live_map != nullptr) {
+#pragma unroll
+        for (int g = 0; g < QPT; g++) {
+            const int w = tid + g * NT;
+            if (w < AREAW) {
+                const int li = w / EW, lw = w - (w / EW) * EW;
+                const int dr = li < H ? -1 : (li >= H + TR ? 1 : 0);
+                const int dc = lw < HW ? -1 : (lw >= HW + TW ? 1 : 0);
+                if ((dr != 0 || dc != 0) && skipped(tr + dr, tc + dc)) wmask |= 1u << g;
+            }
+        }
     }
 
     // Transition intervals lo <= k < hi on k = t0 >> 9 (empty: lo == hi).  The lozenge rules'
@@ -351,6 +375,15 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
         // the next block's colours are in flight during the exchange
         const uint64_t next_colours = last ? 0 : fetch_block(s);
         __syncthreads();  // every update of the block is in LDS before it is published
+        if (wmask != 0) {
+            uint32_t any = 0;
+#pragma unroll
+            for (int g = 0; g < QPT; g++) {
+                const int w = tid + g * NT;
+                if (wmask & (1u << g)) any |= L[w] | L[AREAW + w] | L[2 * AREAW + w];
+            }
+            if (any != 0) __hip_atomic_store(a.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         // Publish into the other copy: after an intermediate block only the band of width H
         // along the tile's edges (the neighbours' halos read nothing else), after the last
         // block every owned word (the launch's result).
@@ -420,7 +453,9 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
 
 template <int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
-    loz4_body<TR, TC, H, NT, UNI>(a, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const bool skip = a.live != nullptr;
+    loz4_body<TR, TC, H, NT, UNI>(a, skip ? a.live[b] : b, skip ? a.ntiles : (int)gridDim.x);
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>

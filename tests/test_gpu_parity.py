@@ -143,6 +143,42 @@ def test_lozenge_multitile_parity(gctx, rule, q, cfg):
         assert np.array_equal(g, o), path
 
 
+@pytest.mark.parametrize("rule,q", [(rt.RT_LOZENGE_QREF, 0.999), (rt.RT_LOZENGE_QHEAT, 5.0)])
+def test_lozenge_skipped_tiles_parity(gctx, rule, q):
+    """A hexagon of side 300 (N = 609): the packed sweep leaves out the corner tiles that lie
+    outside the hexagon (fewer workgroups ran than the grid has tiles, exactly the tiles the
+    host skip map keeps), with skipping on and off, over every tile configuration: 203 steps
+    equal to the oracle's."""
+    from randomtilings_amd import statics as St
+    t0 = St.call("lozenge.MaxTiling", St.call("lozenge.AlmostHexDomain", args=[300, 0]))
+    o = O.lozenge_walk(t0, 203, 23, default_table(BIG), rule=rule, q=q)
+    skipped_cfgs = 0
+    for cfg in range(N_CFGS[rt.RT_LOZENGE]):
+        tr, tc, h, _ = rt.tile_config(rt.RT_LOZENGE, cfg)
+        m, ok = rt.lozenge_skip_map(t0, tr, tc, h)
+        assert ok
+        for skip in (True, False):
+            say("lozenge skipped tiles", rule, q, cfg, skip)
+            ch = gctx.chain(rt.RT_LOZENGE, rule, t0, [q])
+            try:
+                ch.set_skip(skip)
+                ch.set_path(rt.RT_PATH_PERSISTENT, cfg)
+                ch.seed(23)
+                ch.run(203)
+                g = ch.read()
+                tiles, ran = ch.last_grid
+            finally:
+                ch.close()
+            assert tiles == m.size
+            if skip and ran < tiles:
+                assert ran == int(m.sum())
+                skipped_cfgs += 1
+            else:
+                assert ran == tiles
+            assert np.array_equal(g, o), (cfg, skip)
+    assert skipped_cfgs >= 8
+
+
 def test_lozenge_scrambled_params(sctx, golden):
     t0 = golden["lozenge_hex20_0_max"]
     for path, cfg in [(rt.RT_PATH_STEPWISE, -1), (rt.RT_PATH_PERSISTENT, 0), (rt.RT_PATH_PERSISTENT, 2)]:
