@@ -105,7 +105,7 @@ def main() -> int:
     if args.tile_config >= 0:
         ch.set_path(rt.RT_PATH_PERSISTENT, args.tile_config)
     else:
-        ch.autotune(4096)  # on a scratch clone; the measured chain's trajectory is unchanged
+        ch.autotune(16384)  # on a scratch clone; the measured chain's trajectory is unchanged
     ch.seed(args.seed)
     S = args.chain_steps
     print(f"# bench: lozenge hexagon {args.hex} (N={N}), q={args.q}, {S} chain steps per bench step, "
@@ -125,6 +125,8 @@ def main() -> int:
     gpu_ms = ctx.elapsed_ms(0, 1)
     wall = w1 - w0
     attempts = ch.attempts_per_step
+    # vertices of the tiled region (nonzero codes) in one colour class, what a step visits
+    in_domain = int(np.count_nonzero(t0)) // 3
     total_attempts = args.steps * S * attempts
     value = total_attempts / wall
 
@@ -187,6 +189,13 @@ def main() -> int:
             "sweeps_per_step": S / 3.0,
             "parallelism": "single GPU",
             "tile_config": ch.tile_configs()[ch.last_config] if ch.last_config >= 0 else "stepwise",
+            # tiles of the sweep's grid and how many ran: tiles whose cells are all zero (outside
+            # the hexagon) never change and are left out; the attempts above still count every
+            # Rotate work-item, as the reference does
+            "tiles": ch.last_grid[0],
+            "tiles_run": ch.last_grid[1],
+            "in_domain_sites_per_chain_step": in_domain,
+            "in_domain_attempts_per_s": in_domain * args.steps * S / wall,
         },
         "roofline": {
             "bound": "hbm",
