@@ -128,7 +128,16 @@ def test_svg_documents_match_reference(golden):
     with tempfile.TemporaryDirectory() as td:
         for name, a, b in cases:
             p1, p2 = os.path.join(td, "ref.svg"), os.path.join(td, "mine.svg")
-            O.ref_write(name, a, p1, b)
+            if ".DimerToSVG" in name:
+                # The reference's dimer drawings index the domain past its end (LozengeTiler.cpp:
+                # 540-545 reads row i+1 of the last row), so what it draws there depends on
+                # whatever memory follows the vector.  Ours counts those cells as empty; the
+                # reference gets the same domain followed by zeros, so both read defined cells.
+                d = O.ref_call(name.split(".")[0] + ".TilingToDomain", a) if b is None else b
+                dpad = np.concatenate([np.asarray(d, dtype=np.int32).ravel(), np.zeros(np.size(d) + 64, np.int32)])
+                O.ref_write(name.split(".")[0] + ".DimerToSVG2", a, p1, dpad)
+            else:
+                O.ref_write(name, a, p1, b)
             S.write(name, a, p2, b)
             assert open(p1, "rb").read() == open(p2, "rb").read(), name
 
