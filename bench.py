@@ -551,10 +551,11 @@ def bench_c5(args) -> int:
     t0 = time.perf_counter()
     losses = ws.train(net, X, T, M, args.epochs, 16, 2e-3, rng)
     train_s = time.perf_counter() - t0
-    every = 100
+    every = max(100, args.horizon // 800)  # at most ~800 readings of the ensemble per trajectory
     results = []
     held = [ws.Domain(ctx, d) for _, d in ws.domain_bank(np.random.default_rng(args.seed + 99), 4, args.canvas)]
     for k, dom in enumerate(held):
+        print(f"# held-out domain {k + 1}/{len(held)}: N = {dom.N}", file=sys.stderr, flush=True)
         ws.warm_tiling(ctx, net, dom, args.canvas)  # warm-up of the inference path
         w0 = time.perf_counter()
         tw = ws.warm_tiling(ctx, net, dom, args.canvas)  # U-Net forward, projection, tiling read-off

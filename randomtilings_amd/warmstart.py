@@ -14,6 +14,8 @@ statics of librandomtilings.so.
 """
 from __future__ import annotations
 
+import sys
+
 import numpy as np
 
 import randomtilings_amd as rt
@@ -122,8 +124,12 @@ def equilibrium_samples(ctx: rt.Context, doms, steps: int, per_domain: int, seed
     b = rt.Batch(ctx)
     for ch in chains:
         b.add(ch)
-    b.run(steps, True)
-    b.sync()
+    chunk = 20000  # runs continue one another; a line per chunk shows progress on long runs
+    for s0 in range(0, steps, chunk):
+        b.run(min(chunk, steps - s0), True)
+        b.sync()
+        print(f"# equilibrium samples: {min(steps, s0 + chunk)}/{steps} steps, {len(chains)} chains",
+              file=sys.stderr, flush=True)
     out = [(owner[k], ch.read()) for k, ch in enumerate(chains)]
     b.close()
     for ch in chains:
@@ -157,11 +163,13 @@ def build_dataset(ctx, doms, samples, canvas, average: bool = False):
 def train(net: rt.UNet, X, T, M, epochs: int, batch: int, lr: float, rng) -> list:
     losses = []
     n = len(X)
-    for _ in range(epochs):
+    for e in range(epochs):
         order = rng.permutation(n)
         for s in range(0, n - batch + 1, batch):
             idx = order[s:s + batch]
             losses.append(net.train_step(X[idx], T[idx], M[idx], lr))
+        if (e + 1) % 25 == 0:
+            print(f"# training: epoch {e + 1}/{epochs}, loss {losses[-1]:.4g}", file=sys.stderr, flush=True)
     return losses
 
 
