@@ -82,7 +82,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (independent chains shard "
-              "across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+              "across GPUs without communication)", file=sys.stderr)
         return 2
 
     import randomtilings_amd as rt

@@ -73,7 +73,7 @@ struct PersistArgs {
 };
 
 // Resolve a batched block: the chain owning blockIdx.x, with the copies and schedule offset
-// of this round.
+// of the current block of steps.
 __device__ __forceinline__ void resolve_batch(PersistArgs& a, int& tile, int& ntiles) {
     const int b = blockIdx.x;
     int lo = 0, hi = a.nbatch - 1;
