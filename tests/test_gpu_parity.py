@@ -179,6 +179,30 @@ def test_lozenge_skipped_tiles_parity(gctx, rule, q):
     assert skipped_cfgs >= 8
 
 
+def test_lozenge_inconsistent_tiling_runs_every_tile(gctx):
+    """A hexagon-300 tiling with one edge bit that no longer agrees with its mirror is not a
+    fixed point of the update, so nothing is skipped (every tile runs) and the walk still
+    equals the oracle's, which repairs the bit in its first steps exactly as the sweep does."""
+    from randomtilings_amd import statics as St
+    t0 = St.call("lozenge.MaxTiling", St.call("lozenge.AlmostHexDomain", args=[300, 0])).copy()
+    N = int(round(np.sqrt(t0.size)))
+    t0.reshape(N, N)[N // 2, N // 2] ^= 1
+    m, ok = rt.lozenge_skip_map(t0, 56, 84, 4)
+    assert not ok and m.min() == 1
+    o = O.lozenge_walk(t0, 150, 29, default_table(BIG), rule=rt.RT_LOZENGE_QREF, q=0.999)
+    ch = gctx.chain(rt.RT_LOZENGE, rt.RT_LOZENGE_QREF, t0, [0.999])
+    try:
+        ch.set_path(rt.RT_PATH_PERSISTENT, 0)
+        ch.seed(29)
+        ch.run(150)
+        g = ch.read()
+        tiles, ran = ch.last_grid
+    finally:
+        ch.close()
+    assert ran == tiles
+    assert np.array_equal(g, o)
+
+
 def test_lozenge_scrambled_params(sctx, golden):
     t0 = golden["lozenge_hex20_0_max"]
     for path, cfg in [(rt.RT_PATH_STEPWISE, -1), (rt.RT_PATH_PERSISTENT, 0), (rt.RT_PATH_PERSISTENT, 2)]:
