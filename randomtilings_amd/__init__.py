@@ -73,6 +73,7 @@ SIGNATURES = {
     "rt_tile_config_count": (C.c_int32, [C.c_int]),
     "rt_persist_capacity": (C.c_int32, [C.c_int, C.c_int32, C.c_int, C.c_int]),
     "rt_tile_config": (C.c_int, [C.c_int, C.c_int32, _I32P, _I32P, _I32P, _I32P]),
+    "rt_tile_config_packed": (C.c_int32, [C.c_int, C.c_int32]),
     "rt_chain_last_launches": (C.c_int64, [_P]),
     "rt_chain_last_config": (C.c_int32, [_P]),
     "rt_chain_set_skip": (C.c_int, [_P, C.c_int]),
@@ -174,6 +175,11 @@ def tile_config_count(model: int) -> int:
 def persist_capacity(model: int, idx: int, device: int = 0, uniform: bool = True) -> int:
     """Workgroups of tile configuration idx the device holds at once (0: never persistent)."""
     return int(load().rt_persist_capacity(model, idx, device, int(uniform)))
+
+
+def tile_config_packed(model: int, idx: int) -> bool:
+    """Whether configuration idx runs the packed (four positions per word) sweep."""
+    return bool(load().rt_tile_config_packed(model, idx))
 
 
 def tile_config(model: int, idx: int):

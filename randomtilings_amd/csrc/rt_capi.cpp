@@ -784,16 +784,33 @@ int rt_chain_seed(rt_chain* ch, int mode, int64_t seed) {
 }
 
 
+// Whether the packed sweep can run this chain's rule: every lozenge rule (their intervals are
+// disjoint); for dominoes the rules that split k at one threshold 0 < t < 2^23 per class and
+// row parity (3 -> 12 below t, 12 -> 3 from t on: the CFTP and weighted heat-bath rules).
+static bool packed_rule_ok(const rt_chain* ch) {
+    if (ch->model == RT_LOZENGE) return true;
+    if (ch->model != RT_DOMINO) return false;
+    const FlipRule& f = ch->rp.fr;
+    for (int c = 0; c < 2; c++)
+        for (int p = 0; p < 2; p++) {
+            const uint32_t t = f.ab_hi[c][p];
+            if (f.ab_lo[c][p] != 0 || f.ba_lo[c][p] != t || f.ba_hi[c][p] != (1u << 23) || t == 0 || t >= (1u << 23))
+                return false;
+        }
+    return true;
+}
+
 // Tile configuration for the LDS-resident paths.  *resident is true when the whole tile grid
 // fits on the device at once (one launch runs many blocks of h steps, tiles exchanging
 // borders through flags); otherwise the chain runs "blocked": one launch per block of h
 // steps, the launch boundary being the exchange.
-static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* resident) {
+static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* resident, bool no_packed = false) {
     const int n = persist_config_count(ch->model);
     const bool uni = ch->ctx->uniform;
     auto grid = [&](int idx, bool* res) {
         int tr, tc, h, nt;
         if (!persist_config(ch->model, idx, &tr, &tc, &h, &nt)) return false;
+        if (persist_config_packed(ch->model, idx) && (no_packed || !packed_rule_ok(ch))) return false;
         const int trn = (ch->N + tr - 1) / tr, tcn = (ch->W + tc - 1) / tc;
         const int cap = persist_capacity(ch->model, idx, uni, ch->ctx->device);
         if (cap <= 0) return false;
@@ -1126,6 +1143,10 @@ int rt_tile_config(int model, int32_t idx, int32_t* rows, int32_t* cols, int32_t
     return RT_OK;
 }
 
+int32_t rt_tile_config_packed(int model, int32_t idx) {
+    return model != RT_TRIDIMER && persist_config_packed(model, idx) ? 1 : 0;
+}
+
 int64_t rt_chain_last_launches(rt_chain* ch) { return ch ? ch->last_launches : 0; }
 
 int32_t rt_chain_last_config(rt_chain* ch) { return ch ? ch->last_cfg : -1; }
@@ -1255,7 +1276,11 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         if (!largest) continue;
         int nb = 0, tcn = 0;
         bool res = false;
-        g.cfg = choose_persist_cfg(largest, &nb, &tcn, &res);
+        // one configuration serves the whole group: packed only if every chain's rule allows it
+        bool all_packed_ok = true;
+        for (rt_chain* ch : planar)
+            if (ch->model == m && !packed_rule_ok(ch)) all_packed_ok = false;
+        g.cfg = choose_persist_cfg(largest, &nb, &tcn, &res, !all_packed_ok);
         if (g.cfg < 0) return fail(RT_ERR_ARG, "rt_batch_run: no tile configuration for model %d", m);
         int tr, tc, h, nt;
         persist_config(m, g.cfg, &tr, &tc, &h, &nt);

@@ -17,6 +17,8 @@
 //                 never be more than one block ahead of a neighbour.
 #include "rt_internal.hpp"
 #include "rt_loz4.hpp"
+
+#include <type_traits>
 #include "rt_models.hpp"
 
 #include <algorithm>
@@ -351,7 +353,11 @@ struct Cfg {
     X(32, 32, 8, 256, 0)     \
     X(16, 32, 4, 256, 0)     \
     X(48, 64, 8, 1024, 0)    \
-    X(32, 64, 8, 512, 0)
+    X(32, 64, 8, 512, 0)     \
+    X(64, 84, 8, 1024, 1)    \
+    X(56, 84, 4, 768, 1)     \
+    X(32, 44, 4, 256, 1)     \
+    X(16, 44, 4, 256, 1)
 #define RT_LOZENGE_CFGS(X)  \
     X(64, 84, 8, 1024, 1)   \
     X(64, 84, 4, 1024, 1)   \
@@ -411,11 +417,18 @@ bool persist_config_packed(int model, int idx) {
 
 template <class M, int TR, int TC, int H, int NT, int PACKED>
 static const void* kernel_ptr(bool uni, bool batch) {
-    if constexpr (PACKED != 0) {
+    if constexpr (PACKED != 0 && std::is_same<M, LozengeModel>::value) {
         if (batch)
             return uni ? (const void*)k_persist_loz4_batch<TR, TC, H, NT, true>
                        : (const void*)k_persist_loz4_batch<TR, TC, H, NT, false>;
         return uni ? (const void*)k_persist_loz4<TR, TC, H, NT, true> : (const void*)k_persist_loz4<TR, TC, H, NT, false>;
+    } else if constexpr (PACKED != 0 && std::is_same<M, DominoModel>::value) {
+        if (batch)
+            return uni ? (const void*)k_persist_dom4_batch<TR, TC, H, NT, true>
+                       : (const void*)k_persist_dom4_batch<TR, TC, H, NT, false>;
+        return uni ? (const void*)k_persist_dom4<TR, TC, H, NT, true> : (const void*)k_persist_dom4<TR, TC, H, NT, false>;
+    } else if constexpr (PACKED != 0) {
+        return nullptr;  // no packed kernel for this model (coupled domino pairs)
     } else {
         if (batch)
             return uni ? (const void*)k_persist_batch<M, TR, TC, H, NT, true>

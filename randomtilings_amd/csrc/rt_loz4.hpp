@@ -1,15 +1,17 @@
-// rt_loz4.hpp — persistent lozenge sweep on packed quads (gfx950).
+// rt_loz4.hpp — persistent lozenge and domino sweeps on packed quads (gfx950).
 //
 // Same scheme as the generic persistent kernel (tile + halo in LDS, generators in VGPRs,
 // neighbour-flag exchange), but each thread works on quads: four horizontally adjacent
 // positions of one colour-class row held in one 32-bit LDS word.  All four share the row
-// index, so the update's row-dependent column offsets (lozengekernel.cl:45-48) are uniform
-// over the quad and become byte funnel shifts of neighbouring words, and the update of a
-// quad is a handful of 32-bit mask/shift operations instead of four byte-wise passes.
+// index, so the update's row-dependent column offsets (lozengekernel.cl:45-48,
+// dominokernel.cl:22-31) are uniform over the quad and become byte funnel shifts of
+// neighbouring words, and the update of a quad is a handful of 32-bit mask/shift operations
+// instead of four byte-wise passes.
 //
-// Flip (lozengekernel.cl:21-24 / qlozengekernel.cl:23-29 as thresholds): byte b of the quad
-// toggles between 25 and 38 (25 ^ 38 = 63) when its state and its own generator's draw
-// say so; the toggles of the quad are applied with one xor.
+// Flip (lozengekernel.cl:21-24 / qlozengekernel.cl:23-29, dominokernelCFTP.cl:25-29 as
+// thresholds): byte b of the quad toggles between the model's two flippable states (lozenge
+// 25 and 38, 25 ^ 38 = 63; domino 3 and 12, 3 ^ 12 = 15) when its state and its own
+// generator's draw say so; the toggles of the quad are applied with one xor.
 #pragma once
 
 #include "rt_internal.hpp"
@@ -31,9 +33,28 @@ __device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, int sh) {
     return __builtin_amdgcn_alignbyte(hi, lo, sh);
 }
 
-template <int TR, int TC, int H, int NT, bool UNI>
-__device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, const int ntiles) {
-    constexpr int NC = 3;
+// The two packed models.  Lozenge: three classes, the row index mod 3 selects the update's
+// offsets, one rule for every class.  Domino: two classes, the row parity selects the
+// offsets and (weighted rules) the rule; only rules whose two intervals split k at one
+// threshold t (k < t: 3 -> 12, k >= t: 12 -> 3; dominokernelCFTP.cl's rule and the weighted
+// heat-bath) run packed, with 0 < t < 2^23 (checked on the host).
+struct PackedLozenge {
+    using Model = LozengeModel;
+    static constexpr int NC = 3;
+    static constexpr bool kLozenge = true;
+    __device__ __forceinline__ static int rkey(int gi) { return ((gi % 3) + 3) % 3; }
+};
+struct PackedDomino {
+    using Model = DominoModel;
+    static constexpr int NC = 2;
+    static constexpr bool kLozenge = false;
+    __device__ __forceinline__ static int rkey(int gi) { return gi & 1; }
+};
+
+template <class PM, int TR, int TC, int H, int NT, bool UNI>
+__device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile, const int ntiles) {
+    using M = typename PM::Model;
+    constexpr int NC = PM::NC;
     constexpr int ER = TR + 2 * H, EC = TC + 2 * H;
     static_assert(TC % 4 == 0 && H % 4 == 0, "quads must tile the rows");
     constexpr int EW = EC / 4, TW = TC / 4, HW = H / 4;  // words per ext row, owned row, halo
@@ -72,14 +93,14 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
         if (w < AREAW) {
             const int li = w / EW, lw = w - (w / EW) * EW;
             const int gi = r0 + li;
-            rmod[g] = ((gi % 3) + 3) % 3;
+            rmod[g] = PM::rkey(gi);
             if (li >= H && li < H + TR && lw >= HW && lw < HW + TW) omask |= 1u << g;
 #pragma unroll
             for (int b = 0; b < 4; b++) {
                 const int lj = 4 * lw + b, gj = c0 + lj;
-                if (LozengeModel::has_gen(gi, gj, N, W)) {
+                if (M::has_gen(gi, gj, N, W)) {
                     gbytes[g] |= 0xffu << (8 * b);
-                    const int id = LozengeModel::gen_id(gi, gj, N, W);
+                    const int id = M::gen_id(gi, gj, N, W);
                     const uint4 s = a.gen_in[id];
                     s0[g][b] = s.x;
                     s1[g][b] = s.y;
@@ -92,7 +113,7 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
                         pmt[g][b] = q.z;
                     }
                 }
-                if (li >= 1 && li <= ER - 2 && lj >= 1 && lj <= EC - 2 && LozengeModel::updatable(gi, gj, N, W))
+                if (li >= 1 && li <= ER - 2 && lj >= 1 && lj <= EC - 2 && M::updatable(gi, gj, N, W))
                     ubytes[g] |= 0xffu << (8 * b);
             }
         }
@@ -185,7 +206,7 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     const uint32_t alo = a.rp.fr.ab_lo[0][0], ahi = a.rp.fr.ab_hi[0][0];
     const uint32_t blo = a.rp.fr.ba_lo[0][0], bhi = a.rp.fr.ba_hi[0][0];
     const uint32_t asz = ahi - alo, bsz = bhi - blo;
-    const bool split = alo == 0u && ahi > 0u && ahi <= blo && blo < bhi && bhi == (1u << 23);
+    const bool split = !PM::kLozenge || (alo == 0u && ahi > 0u && ahi <= blo && blo < bhi && bhi == (1u << 23));
     const uint32_t aN = ahi << 9, bL = blo << 9;  // split: t0 < aN <=> k in A, t0 >= bL <=> k in B
     // per quad: byte masks of the bits each update rewrites (recomputed at each use when
     // per-generator parameters take the registers)
@@ -194,6 +215,16 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     for (int g = 0; g < (UNI ? QPT : 1); g++) {
         um2[g] = ubytes[g] & 0x26262626u;
         um3[g] = ubytes[g] & 0x19191919u;
+    }
+    // Domino: the threshold t << 9 of the quad's row parity for class 0 and class 1 (the
+    // weighted rules differ by class and parity; t0 < t << 9 <=> k < t, exact as 0 < t < 2^23)
+    uint32_t dth0[PM::kLozenge ? 1 : QPT], dth1[PM::kLozenge ? 1 : QPT];
+    if constexpr (!PM::kLozenge) {
+#pragma unroll
+        for (int g = 0; g < QPT; g++) {
+            dth0[g] = a.rp.fr.ab_hi[0][rmod[g]] << 9;
+            dth1[g] = a.rp.fr.ab_hi[1][rmod[g]] << 9;
+        }
     }
 
     // The colours of a block (H <= 8 steps) are fetched as one 8-byte word before the
@@ -212,14 +243,17 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     // in A, 38 when in B, 0xff (never a state) otherwise.  SPLIT selects the one-comparison
     // interval tests.  Every slot draws (a slot past the class area holds zero states and its
     // draws are discarded), so no generator state flows through a divergent branch.
-    auto targets = [&](int g, auto split_c) __attribute__((always_inline)) -> uint32_t {
+    auto targets = [&](int g, auto split_c, int c) __attribute__((always_inline)) -> uint32_t {
         constexpr bool SPLIT = decltype(split_c)::value;
         uint32_t x[4];
+        const uint32_t dth = PM::kLozenge ? 0u : (c ? dth1[PM::kLozenge ? 0 : g] : dth0[PM::kLozenge ? 0 : g]);
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const uint32_t t0 = tm_draw_raw(s0[g][b], s1[g][b], s2[g][b], s3[g][b], UNI ? a.m1 : pm1[g][b],
                                             UNI ? a.m2 : pm2[g][b], UNI ? a.tm : pmt[g][b]);
-            if constexpr (SPLIT) {
+            if constexpr (!PM::kLozenge) {
+                x[b] = (t0 < dth) ? 3u : 12u;  // k < t: 3 -> 12; k >= t: 12 -> 3
+            } else if constexpr (SPLIT) {
                 const uint32_t xb = (t0 >= bL) ? 38u : 0xffffffffu;
                 x[b] = (t0 < aN) ? 25u : xb;
             } else {
@@ -232,13 +266,13 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
                                      __builtin_amdgcn_perm(x[1], x[0], 0x0c0c0400u), 0x05040100u);
     };
     // Toggle the bytes of quad g whose state equals its target (exact zero-byte test of
-    // S ^ X; 25 ^ 38 = 63).
+    // S ^ X; 25 ^ 38 = 63, 3 ^ 12 = 15).
     auto apply = [&](uint32_t* Ls, int g, uint32_t S, uint32_t X) __attribute__((always_inline)) {
         const int w = tid + g * NT;
         const uint32_t d = S ^ X;
         const uint32_t t = (d & 0x7f7f7f7fu) + 0x7f7f7f7fu;
         const uint32_t z = RT_BITOP3(t, d, 0x80808080u, ~(A | B) & C);  // 0x80 where S_b == x_b
-        const uint32_t tog = ((z >> 1) - (z >> 7)) & gbytes[g];       // 63 there
+        const uint32_t tog = ((z >> (PM::kLozenge ? 1 : 3)) - (z >> 7)) & gbytes[g];  // 63 / 15 there
         if (AREAW % NT == 0 || w < AREAW) Ls[w] = S ^ tog;
     };
     auto load_states = [&](const uint32_t* Ls, uint32_t* S) __attribute__((always_inline)) {
@@ -249,27 +283,28 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
         }
     };
     // flip with draws made now
-    auto flip = [&](uint32_t* Ls, auto split_c) __attribute__((always_inline)) {
+    auto flip = [&](uint32_t* Ls, auto split_c, int c) __attribute__((always_inline)) {
         uint32_t S[QPT];
         load_states(Ls, S);  // read first, so the LDS latency hides behind the draws
 #pragma unroll
-        for (int g = 0; g < QPT; g++) apply(Ls, g, S[g], targets(g, split_c));
+        for (int g = 0; g < QPT; g++) apply(Ls, g, S[g], targets(g, split_c, c));
     };
     // the targets of the first DA steps of the next full block, drawn during the exchange
     // (more would not fit the register budget; none with per-generator parameters, whose
     // registers leave no room)
     constexpr int DA = UNI ? H / 2 : 0;
     uint32_t XP[DA > 0 ? DA : 1][QPT];
-    auto draw_ahead = [&](auto q0_c, auto q1_c, auto split_c) __attribute__((always_inline)) {
+    auto draw_ahead = [&](auto q0_c, auto q1_c, auto split_c, uint64_t cols) __attribute__((always_inline)) {
 #pragma unroll
         for (int q = decltype(q0_c)::value; q < decltype(q1_c)::value; q++)
 #pragma unroll
-            for (int g = 0; g < QPT; g++) XP[q][g] = targets(g, split_c);
+            for (int g = 0; g < QPT; g++) XP[q][g] = targets(g, split_c, (int)((cols >> (8 * q)) & 0xffu));
     };
-    auto draw_ahead_any = [&](auto q0_c, auto q1_c) __attribute__((always_inline)) {
+    auto draw_ahead_any = [&](auto q0_c, auto q1_c, uint64_t cols) __attribute__((always_inline)) {
         if constexpr (DA > 0) {
-            if (split) draw_ahead(q0_c, q1_c, std::true_type{});
-            else draw_ahead(q0_c, q1_c, std::false_type{});
+            if constexpr (!PM::kLozenge) draw_ahead(q0_c, q1_c, std::true_type{}, cols);
+            else if (split) draw_ahead(q0_c, q1_c, std::true_type{}, cols);
+            else draw_ahead(q0_c, q1_c, std::false_type{}, cols);
         }
     };
 
@@ -278,31 +313,54 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     // quad without branching, and the rewritten bits are merged with three-input selects
     // (a term's stray bits fall outside the bits the final select keeps).
     auto update = [&](uint32_t* Ls, uint32_t* L2, uint32_t* L3, int c) __attribute__((always_inline)) {
-        const int t2 = (2 * c + 1) % 3, t4 = (2 * c + 2) % 3, t8 = (2 * c) % 3;
+        if constexpr (PM::kLozenge) {
+            const int t2 = (2 * c + 1) % 3, t4 = (2 * c + 2) % 3, t8 = (2 * c) % 3;
+    #pragma unroll
+            for (int g = 0; g < QPT; g++) {
+                const int w = tid + g * NT;
+                if (ubytes[g]) {
+                    const int r = rmod[g];
+                    const uint32_t U0 = Ls[w - EW], U1 = Ls[w - EW + 1];
+                    const uint32_t Mm = Ls[w - 1], M0 = Ls[w], Mp = Ls[w + 1];
+                    const uint32_t Dm = Ls[w + EW - 1], D0 = Ls[w + EW];
+                    const uint32_t o2 = L2[w], o3 = L3[w];
+                    const uint32_t up2 = funnel(U1, U0, (r == t2) ? 1 : 0);    // s[i-1][j+p2]
+                    const uint32_t mm4 = (r == t4) ? funnel(M0, Mm, 3) : M0;   // s[i][j-p4]
+                    const uint32_t mp8 = (r == t8) ? funnel(Mp, M0, 1) : M0;   // s[i][j+p8]
+                    const uint32_t dm16 = (r == t2) ? funnel(D0, Dm, 3) : D0;  // s[i+1][j-p16]
+                    // bits 2, 4, 32 of class c+1 and 1, 8, 16 of class c+2
+                    const uint32_t n2 = RT_BITOP3(0x02020202u, up2 >> 3,
+                                                  RT_BITOP3(0x04040404u, mm4 >> 1, D0 << 5, (A & B) | (~A & C)),
+                                                  (A & B) | (~A & C));
+                    const uint32_t n3 = RT_BITOP3(0x01010101u, U0 >> 5,
+                                                  RT_BITOP3(0x08080808u, mp8 << 1, dm16 << 3, (A & B) | (~A & C)),
+                                                  (A & B) | (~A & C));
+                    const uint32_t m2 = UNI ? um2[UNI ? g : 0] : (ubytes[g] & 0x26262626u);
+                    const uint32_t m3 = UNI ? um3[UNI ? g : 0] : (ubytes[g] & 0x19191919u);
+                    L2[w] = RT_BITOP3(m2, n2, o2, (A & B) | (~A & C));
+                    L3[w] = RT_BITOP3(m3, n3, o3, (A & B) | (~A & C));
+                }
+            }
+        } else {
+            // dominokernel.cl:22-31: class 1 - c (L2 here) from class c, every bit rewritten:
+            // (up & 2) >> 1 | (down & 1) << 1 | (s[i][j - offL] & 8) >> 1 | (s[i][j + offR] & 4) << 1
+            // with t = 1 - c, offL = (i + t + 1) & 1, offR = (i + t) & 1
 #pragma unroll
-        for (int g = 0; g < QPT; g++) {
-            const int w = tid + g * NT;
-            if (ubytes[g]) {
-                const int r = rmod[g];
-                const uint32_t U0 = Ls[w - EW], U1 = Ls[w - EW + 1];
-                const uint32_t Mm = Ls[w - 1], M0 = Ls[w], Mp = Ls[w + 1];
-                const uint32_t Dm = Ls[w + EW - 1], D0 = Ls[w + EW];
-                const uint32_t o2 = L2[w], o3 = L3[w];
-                const uint32_t up2 = funnel(U1, U0, (r == t2) ? 1 : 0);    // s[i-1][j+p2]
-                const uint32_t mm4 = (r == t4) ? funnel(M0, Mm, 3) : M0;   // s[i][j-p4]
-                const uint32_t mp8 = (r == t8) ? funnel(Mp, M0, 1) : M0;   // s[i][j+p8]
-                const uint32_t dm16 = (r == t2) ? funnel(D0, Dm, 3) : D0;  // s[i+1][j-p16]
-                // bits 2, 4, 32 of class c+1 and 1, 8, 16 of class c+2
-                const uint32_t n2 = RT_BITOP3(0x02020202u, up2 >> 3,
-                                              RT_BITOP3(0x04040404u, mm4 >> 1, D0 << 5, (A & B) | (~A & C)),
-                                              (A & B) | (~A & C));
-                const uint32_t n3 = RT_BITOP3(0x01010101u, U0 >> 5,
-                                              RT_BITOP3(0x08080808u, mp8 << 1, dm16 << 3, (A & B) | (~A & C)),
-                                              (A & B) | (~A & C));
-                const uint32_t m2 = UNI ? um2[UNI ? g : 0] : (ubytes[g] & 0x26262626u);
-                const uint32_t m3 = UNI ? um3[UNI ? g : 0] : (ubytes[g] & 0x19191919u);
-                L2[w] = RT_BITOP3(m2, n2, o2, (A & B) | (~A & C));
-                L3[w] = RT_BITOP3(m3, n3, o3, (A & B) | (~A & C));
+            for (int g = 0; g < QPT; g++) {
+                const int w = tid + g * NT;
+                if (ubytes[g]) {
+                    const int offR = (rmod[g] + 1 - c) & 1;
+                    const uint32_t U0 = Ls[w - EW], D0 = Ls[w + EW];
+                    const uint32_t Mm = Ls[w - 1], M0 = Ls[w], Mp = Ls[w + 1];
+                    const uint32_t o = L2[w];
+                    const uint32_t lf = offR ? M0 : funnel(M0, Mm, 3);  // s[i][j-offL], offL = 1 - offR
+                    const uint32_t rt = offR ? funnel(Mp, M0, 1) : M0;  // s[i][j+offR]
+                    const uint32_t x8 = (rt << 1) & 0x08080808u;
+                    const uint32_t x4 = RT_BITOP3(0x04040404u, lf >> 1, x8, (A & B) | (~A & C));
+                    const uint32_t x2 = RT_BITOP3(0x02020202u, D0 << 1, x4, (A & B) | (~A & C));
+                    const uint32_t x1 = RT_BITOP3(0x01010101u, U0 >> 1, x2, (A & B) | (~A & C));
+                    L2[w] = RT_BITOP3(ubytes[g], x1, o, (A & B) | (~A & C));
+                }
             }
         }
     };
@@ -313,8 +371,9 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     // still reading, so only then is a barrier needed before the flip.
     auto step = [&](int c, int c_prev, auto flip_c) __attribute__((always_inline)) {
         uint32_t* Ls = L + c * AREAW;
-        uint32_t* L2 = L + (c == 2 ? 0 : c + 1) * AREAW;
-        uint32_t* L3 = L + (c == 0 ? 2 : c - 1) * AREAW;
+        // lozenge: classes c+1 and c+2; domino: L2 is the other class
+        uint32_t* L2 = PM::kLozenge ? L + (c == 2 ? 0 : c + 1) * AREAW : L + (1 - c) * AREAW;
+        uint32_t* L3 = PM::kLozenge ? L + (c == 0 ? 2 : c - 1) * AREAW : nullptr;
         if (c == c_prev) __syncthreads();
         flip_c(Ls);
         __syncthreads();
@@ -335,7 +394,7 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
                     for (int g = 0; g < QPT; g++) apply(Ls, g, S[g], XP[q < DA ? q : 0][g]);
                 });
             } else {
-                step(c, c_prev, [&](uint32_t* Ls) __attribute__((always_inline)) { flip(Ls, split_c); });
+                step(c, c_prev, [&](uint32_t* Ls) __attribute__((always_inline)) { flip(Ls, split_c, c); });
             }
             c_prev = c;
         }
@@ -344,7 +403,7 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     auto run_part = [&](uint64_t colours, int64_t blk, int& c_prev, auto split_c) __attribute__((always_inline)) {
         for (int64_t q = 0; q < blk; q++) {
             const int c = (int)((colours >> (8 * q)) & 0xffu);
-            step(c, c_prev, [&](uint32_t* Ls) __attribute__((always_inline)) { flip(Ls, split_c); });
+            step(c, c_prev, [&](uint32_t* Ls) __attribute__((always_inline)) { flip(Ls, split_c, c); });
             c_prev = c;
         }
     };
@@ -359,16 +418,16 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
     using D1 = std::integral_constant<int, DA / 4>;
     using D3 = std::integral_constant<int, 3 * DA / 4>;
     using DF = std::integral_constant<int, DA>;
-    if (a.nsteps >= H) draw_ahead_any(Z{}, DF{});
+    if (a.nsteps >= H) draw_ahead_any(Z{}, DF{}, colours);
     while (s < a.nsteps) {
         const int64_t blk = (a.nsteps - s < H) ? (a.nsteps - s) : H;
         int c_prev = -1;  // the block starts after a barrier
         if (blk == H) {
-            if (split) run_full(colours, c_prev, std::true_type{});
-            else run_full(colours, c_prev, std::false_type{});
+            if (!PM::kLozenge || split) run_full(colours, c_prev, std::true_type{});
+            else if constexpr (PM::kLozenge) run_full(colours, c_prev, std::false_type{});
         } else {
-            if (split) run_part(colours, blk, c_prev, std::true_type{});
-            else run_part(colours, blk, c_prev, std::false_type{});
+            if (!PM::kLozenge || split) run_part(colours, blk, c_prev, std::true_type{});
+            else if constexpr (PM::kLozenge) run_part(colours, blk, c_prev, std::false_type{});
         }
         s += blk;
         const bool last = s >= a.nsteps;
@@ -380,7 +439,10 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
 #pragma unroll
             for (int g = 0; g < QPT; g++) {
                 const int w = tid + g * NT;
-                if (wmask & (1u << g)) any |= L[w] | L[AREAW + w] | L[2 * AREAW + w];
+                if (wmask & (1u << g)) {
+#pragma unroll
+                    for (int c = 0; c < NC; c++) any |= L[c * AREAW + w];
+                }
             }
             if (any != 0) __hip_atomic_store(a.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -407,11 +469,11 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
             // the next full block's targets are drawn while the published words drain and
             // while the neighbours finish their blocks
             const bool full_next = a.nsteps - s >= H;
-            if (full_next) draw_ahead_any(Z{}, D1{});
+            if (full_next) draw_ahead_any(Z{}, D1{}, next_colours);
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (tid == 0) xstore(a.flags + (size_t)tile * kLine, epoch);
-            if (full_next) draw_ahead_any(D1{}, D3{});
+            if (full_next) draw_ahead_any(D1{}, D3{}, next_colours);
             if (nbr_tile >= 0) {
                 const uint32_t* f = a.flags + (size_t)nbr_tile * kLine;
                 uint32_t polls = 0;
@@ -427,7 +489,7 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
             __syncthreads();
             if constexpr (DA > 0) {
                 halo_issue(buf);
-                if (full_next) draw_ahead_any(D3{}, DF{});
+                if (full_next) draw_ahead_any(D3{}, DF{}, next_colours);
                 halo_land();
             } else {
                 load_ext(buf, true);
@@ -445,7 +507,7 @@ __device__ __forceinline__ void loz4_body(const PersistArgs& a, const int tile, 
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             if (!(gbytes[g] & (0xffu << (8 * b)))) continue;
-            const int id = LozengeModel::gen_id(r0 + li, c0 + 4 * lw + b, N, W);
+            const int id = M::gen_id(r0 + li, c0 + 4 * lw + b, N, W);
             a.gen_out[id] = make_uint4(s0[g][b], s1[g][b], s2[g][b], s3[g][b]);
         }
     }
@@ -455,7 +517,7 @@ template <int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     const bool skip = a.live != nullptr;
-    loz4_body<TR, TC, H, NT, UNI>(a, skip ? a.live[b] : b, skip ? a.ntiles : (int)gridDim.x);
+    packed_body<PackedLozenge, TR, TC, H, NT, UNI>(a, skip ? a.live[b] : b, skip ? a.ntiles : (int)gridDim.x);
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>
@@ -463,7 +525,20 @@ __global__ __launch_bounds__(NT) void k_persist_loz4_batch(PersistArgs a_in) {
     PersistArgs a = a_in;
     int tile, ntiles;
     resolve_batch(a, tile, ntiles);
-    loz4_body<TR, TC, H, NT, UNI>(a, tile, ntiles);
+    packed_body<PackedLozenge, TR, TC, H, NT, UNI>(a, tile, ntiles);
+}
+
+template <int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist_dom4(PersistArgs a) {
+    packed_body<PackedDomino, TR, TC, H, NT, UNI>(a, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+}
+
+template <int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist_dom4_batch(PersistArgs a_in) {
+    PersistArgs a = a_in;
+    int tile, ntiles;
+    resolve_batch(a, tile, ntiles);
+    packed_body<PackedDomino, TR, TC, H, NT, UNI>(a, tile, ntiles);
 }
 
 }  // namespace rt

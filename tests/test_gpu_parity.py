@@ -53,6 +53,9 @@ def paths(model):
 @pytest.mark.parametrize("path,cfg", paths(rt.RT_DOMINO))
 @pytest.mark.parametrize("rule", [rt.RT_DOMINO_LAZY, rt.RT_DOMINO_CFTP])
 def test_domino_walk_parity(gctx, golden, rule, path, cfg):
+    if rule == rt.RT_DOMINO_LAZY and cfg >= 0 and rt.tile_config_packed(rt.RT_DOMINO, cfg):
+        pytest.skip("the packed sweep runs rules that split k at one threshold; dominokernel.cl's "
+                    "overlapping intervals run the byte-per-position kernels")
     for n, steps in [(1, 37), (3, 200), (16, 301), (64, 517)]:
         t0 = golden[f"domino_aztec{n}_min"]
         say("domino", rule, path, cfg, n, steps)
@@ -97,13 +100,22 @@ def test_domino_epochs_parity(gctx, golden):
     assert np.array_equal(g, o)
 
 
-def test_domino_weighted_parity(gctx, golden):
-    t0 = golden["domino_aztec16_min"]
+@pytest.mark.parametrize("cfg", [-1] + [i for i in range(N_CFGS[rt.RT_DOMINO])
+                                        if rt.tile_config_packed(rt.RT_DOMINO, i)])
+def test_domino_weighted_parity(gctx, golden, cfg):
+    """Weighted heat-bath (a threshold per class and row parity), default path and every
+    packed configuration, on Aztec diamonds of order 16 and 64."""
     p12 = [0.5, 0.5, 0.2, 0.8]
-    say("domino weighted")
-    g = gctx.domino_walk(t0, 600, 5, rule=rt.RT_DOMINO_WEIGHTED, p12=p12)
-    o = O.domino_walk(t0, 600, 5, default_table(BIG), rule=2, p12=p12)
-    assert np.array_equal(g, o)
+    for n, steps in [(16, 600), (64, 333)]:
+        t0 = golden[f"domino_aztec{n}_min"]
+        say("domino weighted", cfg, n)
+        gctx.set_path(rt.RT_PATH_PERSISTENT if cfg >= 0 else rt.RT_PATH_AUTO, cfg)
+        try:
+            g = gctx.domino_walk(t0, steps, 5, rule=rt.RT_DOMINO_WEIGHTED, p12=p12)
+        finally:
+            gctx.set_path(rt.RT_PATH_AUTO, -1)
+        o = O.domino_walk(t0, steps, 5, default_table(BIG), rule=2, p12=p12)
+        assert np.array_equal(g, o), n
 
 
 @pytest.mark.parametrize("path,cfg", paths(rt.RT_LOZENGE))
@@ -373,7 +385,7 @@ def test_domino_cftp_bound(gctx, golden):
     assert e.value.code == rt.RT_ERR_NOT_COALESCED
 
 
-@pytest.mark.parametrize("cfg", [0, 1])
+@pytest.mark.parametrize("cfg", [0, 1] + [i for i in range(N_CFGS[rt.RT_DOMINO]) if rt.tile_config_packed(rt.RT_DOMINO, i)])
 def test_domino_blocked_path(gctx, cfg):
     """The blocked path (one launch per block of h steps, the launch boundary being the
     exchange; what a tile grid too large to be resident runs) on an Aztec diamond of order
