@@ -785,11 +785,12 @@ int rt_chain_seed(rt_chain* ch, int mode, int64_t seed) {
 
 
 // Whether the packed sweep can run this chain's rule: every lozenge rule (their intervals are
-// disjoint); for dominoes the rules that split k at one threshold 0 < t < 2^23 per class and
-// row parity (3 -> 12 below t, 12 -> 3 from t on: the CFTP and weighted heat-bath rules).
+// disjoint); for dominoes and coupled domino pairs the rules that split k at one threshold
+// 0 < t < 2^23 per class and row parity (3 -> 12 below t, 12 -> 3 from t on: the CFTP and
+// weighted heat-bath rules).
 static bool packed_rule_ok(const rt_chain* ch) {
     if (ch->model == RT_LOZENGE) return true;
-    if (ch->model != RT_DOMINO) return false;
+    if (ch->model != RT_DOMINO && ch->model != kDominoPair) return false;
     const FlipRule& f = ch->rp.fr;
     for (int c = 0; c < 2; c++)
         for (int p = 0; p < 2; p++) {

@@ -427,8 +427,11 @@ static const void* kernel_ptr(bool uni, bool batch) {
             return uni ? (const void*)k_persist_dom4_batch<TR, TC, H, NT, true>
                        : (const void*)k_persist_dom4_batch<TR, TC, H, NT, false>;
         return uni ? (const void*)k_persist_dom4<TR, TC, H, NT, true> : (const void*)k_persist_dom4<TR, TC, H, NT, false>;
+    } else if constexpr (PACKED != 0 && std::is_same<M, DominoPairModel>::value) {
+        if (batch) return nullptr;  // coupled pairs run alone (coupling from the past)
+        return uni ? (const void*)k_persist_pair2<TR, TC, H, NT, true> : (const void*)k_persist_pair2<TR, TC, H, NT, false>;
     } else if constexpr (PACKED != 0) {
-        return nullptr;  // no packed kernel for this model (coupled domino pairs)
+        return nullptr;  // no packed kernel for this model
     } else {
         if (batch)
             return uni ? (const void*)k_persist_batch<M, TR, TC, H, NT, true>

@@ -40,13 +40,23 @@ __device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, int sh) {
 // heat-bath) run packed, with 0 < t < 2^23 (checked on the host).
 struct PackedLozenge {
     using Model = LozengeModel;
-    static constexpr int NC = 3;
+    static constexpr int NC = 3, QPW = 4;  // classes, positions per 32-bit word
     static constexpr bool kLozenge = true;
     __device__ __forceinline__ static int rkey(int gi) { return ((gi % 3) + 3) % 3; }
 };
 struct PackedDomino {
     using Model = DominoModel;
-    static constexpr int NC = 2;
+    static constexpr int NC = 2, QPW = 4;
+    static constexpr bool kLozenge = false;
+    __device__ __forceinline__ static int rkey(int gi) { return gi & 1; }
+};
+// Coupled domino pairs (coupling from the past): a position is 16 bits, the chain from the
+// maximal tiling in the low byte and the one from the minimal tiling in the high byte, so a
+// word holds two positions.  One draw decides both bytes' flips and the update's bit moves
+// stay inside bytes, so the domino code runs unchanged on bytes, with a position two bytes wide.
+struct PackedDominoPair {
+    using Model = DominoPairModel;
+    static constexpr int NC = 2, QPW = 2;
     static constexpr bool kLozenge = false;
     __device__ __forceinline__ static int rkey(int gi) { return gi & 1; }
 };
@@ -56,8 +66,11 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
     using M = typename PM::Model;
     constexpr int NC = PM::NC;
     constexpr int ER = TR + 2 * H, EC = TC + 2 * H;
-    static_assert(TC % 4 == 0 && H % 4 == 0, "quads must tile the rows");
-    constexpr int EW = EC / 4, TW = TC / 4, HW = H / 4;  // words per ext row, owned row, halo
+    constexpr int QPW = PM::QPW, BPP = 4 / QPW;  // positions per word, bytes per position
+    static_assert(TC % QPW == 0 && H % QPW == 0, "words must tile the rows");
+    constexpr int EW = EC / QPW, TW = TC / QPW, HW = H / QPW;  // words per ext row, owned row, halo
+    // the byte mask of position b of a word
+    auto posmask = [](int b) { return (BPP == 1 ? 0xffu : 0xffffu) << (8 * BPP * b); };
     constexpr int AREAW = ER * EW;                       // words per class
     constexpr int QPT = (AREAW + NT - 1) / NT;           // quads per thread
     constexpr int GUARD = EW + 4;                        // words before and after the classes
@@ -65,18 +78,18 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
     uint32_t* L = lds + GUARD;
 
     const int N = a.N, W = a.W;
-    const int PW = a.P / 4;
+    const int PW = a.P / QPW;
     const size_t CSW = (size_t)N * PW;
     const int tiles_r = ntiles / a.tiles_c;
     const int tr = tile / a.tiles_c, tc = tile - (tile / a.tiles_c) * a.tiles_c;
     const int r0 = tr * TR - H, c0 = tc * TC - H;
-    const int c0w = c0 / 4;
+    const int c0w = c0 / QPW;
     const int tid = threadIdx.x;
 
     // per quad: 4 generators, the byte masks of positions with a generator / updatable /
     // owned, and the row index mod 3
-    uint32_t s0[QPT][4], s1[QPT][4], s2[QPT][4], s3[QPT][4];
-    uint32_t pm1[UNI ? 1 : QPT][UNI ? 1 : 4], pm2[UNI ? 1 : QPT][UNI ? 1 : 4], pmt[UNI ? 1 : QPT][UNI ? 1 : 4];
+    uint32_t s0[QPT][QPW], s1[QPT][QPW], s2[QPT][QPW], s3[QPT][QPW];
+    uint32_t pm1[UNI ? 1 : QPT][UNI ? 1 : QPW], pm2[UNI ? 1 : QPT][UNI ? 1 : QPW], pmt[UNI ? 1 : QPT][UNI ? 1 : QPW];
     uint32_t gbytes[QPT], ubytes[QPT];
     uint32_t omask = 0;
     int rmod[QPT];
@@ -86,7 +99,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
         gbytes[g] = ubytes[g] = 0;
         rmod[g] = 0;
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
+        for (int b = 0; b < QPW; b++) {
             s0[g][b] = s1[g][b] = s2[g][b] = s3[g][b] = 0;
             if (!UNI) pm1[g][b] = pm2[g][b] = pmt[g][b] = 0;
         }
@@ -96,10 +109,10 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
             rmod[g] = PM::rkey(gi);
             if (li >= H && li < H + TR && lw >= HW && lw < HW + TW) omask |= 1u << g;
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int lj = 4 * lw + b, gj = c0 + lj;
+            for (int b = 0; b < QPW; b++) {
+                const int lj = QPW * lw + b, gj = c0 + lj;
                 if (M::has_gen(gi, gj, N, W)) {
-                    gbytes[g] |= 0xffu << (8 * b);
+                    gbytes[g] |= posmask(b);
                     const int id = M::gen_id(gi, gj, N, W);
                     const uint4 s = a.gen_in[id];
                     s0[g][b] = s.x;
@@ -114,7 +127,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
                     }
                 }
                 if (li >= 1 && li <= ER - 2 && lj >= 1 && lj <= EC - 2 && M::updatable(gi, gj, N, W))
-                    ubytes[g] |= 0xffu << (8 * b);
+                    ubytes[g] |= posmask(b);
             }
         }
     }
@@ -245,10 +258,10 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
     // draws are discarded), so no generator state flows through a divergent branch.
     auto targets = [&](int g, auto split_c, int c) __attribute__((always_inline)) -> uint32_t {
         constexpr bool SPLIT = decltype(split_c)::value;
-        uint32_t x[4];
+        uint32_t x[QPW];
         const uint32_t dth = PM::kLozenge ? 0u : (c ? dth1[PM::kLozenge ? 0 : g] : dth0[PM::kLozenge ? 0 : g]);
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
+        for (int b = 0; b < QPW; b++) {
             const uint32_t t0 = tm_draw_raw(s0[g][b], s1[g][b], s2[g][b], s3[g][b], UNI ? a.m1 : pm1[g][b],
                                             UNI ? a.m2 : pm2[g][b], UNI ? a.tm : pmt[g][b]);
             if constexpr (!PM::kLozenge) {
@@ -262,8 +275,12 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
                 x[b] = (k - alo < asz) ? 25u : xb;
             }
         }
-        return __builtin_amdgcn_perm(__builtin_amdgcn_perm(x[3], x[2], 0x0c0c0400u),
-                                     __builtin_amdgcn_perm(x[1], x[0], 0x0c0c0400u), 0x05040100u);
+        if constexpr (QPW == 2) {
+            return __builtin_amdgcn_perm(x[1], x[0], 0x04040000u);  // each target in both bytes
+        } else {
+            return __builtin_amdgcn_perm(__builtin_amdgcn_perm(x[QPW - 1], x[QPW - 2], 0x0c0c0400u),
+                                         __builtin_amdgcn_perm(x[1], x[0], 0x0c0c0400u), 0x05040100u);
+        }
     };
     // Toggle the bytes of quad g whose state equals its target (exact zero-byte test of
     // S ^ X; 25 ^ 38 = 63, 3 ^ 12 = 15).
@@ -353,8 +370,8 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
                     const uint32_t U0 = Ls[w - EW], D0 = Ls[w + EW];
                     const uint32_t Mm = Ls[w - 1], M0 = Ls[w], Mp = Ls[w + 1];
                     const uint32_t o = L2[w];
-                    const uint32_t lf = offR ? M0 : funnel(M0, Mm, 3);  // s[i][j-offL], offL = 1 - offR
-                    const uint32_t rt = offR ? funnel(Mp, M0, 1) : M0;  // s[i][j+offR]
+                    const uint32_t lf = offR ? M0 : funnel(M0, Mm, 4 - BPP);  // s[i][j-offL], offL = 1 - offR
+                    const uint32_t rt = offR ? funnel(Mp, M0, BPP) : M0;    // s[i][j+offR]
                     const uint32_t x8 = (rt << 1) & 0x08080808u;
                     const uint32_t x4 = RT_BITOP3(0x04040404u, lf >> 1, x8, (A & B) | (~A & C));
                     const uint32_t x2 = RT_BITOP3(0x02020202u, D0 << 1, x4, (A & B) | (~A & C));
@@ -505,9 +522,9 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
         const int w = tid + g * NT;
         const int li = w / EW, lw = w - (w / EW) * EW;
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            if (!(gbytes[g] & (0xffu << (8 * b)))) continue;
-            const int id = M::gen_id(r0 + li, c0 + 4 * lw + b, N, W);
+        for (int b = 0; b < QPW; b++) {
+            if (!(gbytes[g] & posmask(b))) continue;
+            const int id = M::gen_id(r0 + li, c0 + QPW * lw + b, N, W);
             a.gen_out[id] = make_uint4(s0[g][b], s1[g][b], s2[g][b], s3[g][b]);
         }
     }
@@ -539,6 +556,11 @@ __global__ __launch_bounds__(NT) void k_persist_dom4_batch(PersistArgs a_in) {
     int tile, ntiles;
     resolve_batch(a, tile, ntiles);
     packed_body<PackedDomino, TR, TC, H, NT, UNI>(a, tile, ntiles);
+}
+
+template <int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist_pair2(PersistArgs a) {
+    packed_body<PackedDominoPair, TR, TC, H, NT, UNI>(a, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 }  // namespace rt

@@ -378,6 +378,24 @@ def test_domino_cftp_parity(gctx, golden, mode, rule, p12):
         assert np.array_equal(g[0], o[0]) and g[1:] == o[1:], (n, g[1:], o[1:])
 
 
+@pytest.mark.parametrize("cfg", [i for i in range(N_CFGS[rt.RT_DOMINO]) if rt.tile_config_packed(rt.RT_DOMINO, i)])
+@pytest.mark.parametrize("rule,p12", [(rt.RT_DOMINO_CFTP, None), (rt.RT_DOMINO_WEIGHTED, [0.6, 0.4, 0.45, 0.55])])
+def test_domino_cftp_packed_configs(gctx, golden, cfg, rule, p12):
+    """Coupled pairs on the packed sweep (two 16-bit positions per word) in every packed
+    configuration: the oracle's sample, replay length and restart count."""
+    for n, seed in [(16, 11), (32, 7)]:
+        tmax, tmin = golden[f"domino_aztec{n}_max"], golden[f"domino_aztec{n}_min"]
+        say("cftp packed", cfg, rule, n)
+        gctx.set_path(rt.RT_PATH_PERSISTENT, cfg)
+        try:
+            g = gctx.domino_cftp(tmax, tmin, seed, rule=rule, p12=p12, mode=rt.RT_CFTP_DOUBLING, max_restarts=9)
+        finally:
+            gctx.set_path(rt.RT_PATH_AUTO, -1)
+        o = O.domino_cftp(tmax, tmin, seed, default_table(BIG), rule=rule, p12=p12, mode=rt.RT_CFTP_DOUBLING,
+                          max_restarts=9)
+        assert np.array_equal(g[0], o[0]) and g[1:] == o[1:], (n, g[1:], o[1:])
+
+
 def test_domino_cftp_bound(gctx, golden):
     tmax, tmin = golden["domino_aztec16_max"], golden["domino_aztec16_min"]
     with pytest.raises(rt.RTError) as e:
