@@ -87,7 +87,19 @@ def build_native(jobs: int = 8) -> Path:
         _run(["g++", "-shared", "-o", str(tmp), *map(str, cxx_objs), "-L", str(LIB), "-lrt_hip",
               f"-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, api)
+        build_cpp_caller()
     return so
+
+
+def build_cpp_caller() -> None:
+    """The C++ caller program the GPU tests run against librandomtilings.so (tests/cpp): it
+    uses the reference-compatible classes the way the reference's example programs do."""
+    src = ROOT / "tests" / "cpp" / "tiler_walks.cpp"
+    if not src.exists():
+        return
+    exe = src.with_suffix("")
+    _run(["g++", "-O2", "-std=gnu++17", str(src), "-L", str(LIB), "-lrandomtilings", "-lrt_hip",
+          "-Wl,-rpath,$ORIGIN/../../randomtilings_amd/lib", "-o", str(exe)])
 
 
 def build_oracle() -> None:
