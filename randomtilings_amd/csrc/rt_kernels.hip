@@ -16,7 +16,7 @@
 //                 inputs within distance H.  Double buffering is race-free because a tile can
 //                 never be more than one block ahead of a neighbour.
 #include "rt_internal.hpp"
-#include "rt_loz4.hpp"
+#include "rt_packed.hpp"
 
 #include <type_traits>
 #include "rt_models.hpp"

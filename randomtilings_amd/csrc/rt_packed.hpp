@@ -1,4 +1,4 @@
-// rt_loz4.hpp — persistent lozenge and domino sweeps on packed quads (gfx950).
+// rt_packed.hpp — persistent lozenge and domino sweeps on packed words (gfx950).
 //
 // Same scheme as the generic persistent kernel (tile + halo in LDS, generators in VGPRs,
 // neighbour-flag exchange), but each thread works on quads: four horizontally adjacent
