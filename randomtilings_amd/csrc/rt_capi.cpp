@@ -784,6 +784,9 @@ int rt_chain_seed(rt_chain* ch, int mode, int64_t seed) {
 }
 
 
+// The parameter table covers the chain (what seeding and autotuning need).
+static bool c_has_params(const rt_ctx* c, const rt_chain* ch) { return c->params && c->nparams >= ch->G; }
+
 // Whether the packed sweep can run this chain's rule: every lozenge rule (their intervals are
 // disjoint); for dominoes and coupled domino pairs the rules that split k at one threshold
 // 0 < t < 2^23 per class and row parity (3 -> 12 below t, 12 -> 3 from t on: the CFTP and
@@ -1538,6 +1541,25 @@ int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* 
     rt_chain* ch = nullptr;
     int rc = rt_chain_create(ctx, kDominoPair, rule, rparams, rparams ? 4 : 0, pair.data(), N, &ch);
     if (rc) return rc;
+    // Coalescence takes many launches of the same shape: unless the caller fixed the path,
+    // time the tile configurations once per (side, parameter mode) on a scratch copy (the
+    // sample does not depend on the configuration) and keep the fastest.
+    if (ch->path == RT_PATH_AUTO && ch->cfg < 0 && c_has_params(ctx, ch)) {
+        bool tuned;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            tuned = g_tuned.count(std::make_tuple((int)kDominoPair, N, ctx->uniform)) != 0;
+        }
+        if (!tuned) {
+            rc = rt_chain_autotune(ch, 1024, nullptr);
+            if (rc) {
+                rt_chain_destroy(ch);
+                return rc;
+            }
+            ch->path = RT_PATH_AUTO;  // the tuned choice is found again through the tuning table
+            ch->cfg = -1;
+        }
+    }
     const size_t bytes = (size_t)N * ch->P * ch->NC * ch->esize;
     void* init = nullptr;
     uint32_t* count = nullptr;
