@@ -162,7 +162,10 @@ int rt_tridimer_walk(rt_ctx* ctx, int32_t* tiling, int32_t N, int64_t steps, int
  * the two agree everywhere.  Both chains run as one coupled device chain (one random number
  * per site and step drives both) and the comparison is a device reduction.  On success
  * `out` holds the sample, *total_steps the steps of the final replay and *restarts the number
- * of restarts (either may be NULL).  RT_ERR_NOT_COALESCED after max_restarts restarts. */
+ * of restarts (either may be NULL).  RT_ERR_NOT_COALESCED after max_restarts restarts.
+ * Unless rt_ctx_set_path fixed a path, the first call for a side times the tile
+ * configurations on a scratch copy (as rt_chain_autotune does) and later calls reuse the
+ * choice; the sample does not depend on it. */
 int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* tmax, const int32_t* tmin,
                    int32_t N, int64_t seed, int mode, int64_t first_steps, int32_t max_restarts, int32_t* out,
                    int64_t* total_steps, int32_t* restarts);
