@@ -808,7 +808,8 @@ static bool packed_rule_ok(const rt_chain* ch) {
 // fits on the device at once (one launch runs many blocks of h steps, tiles exchanging
 // borders through flags); otherwise the chain runs "blocked": one launch per block of h
 // steps, the launch boundary being the exchange.
-static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* resident, bool no_packed = false) {
+static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* resident, bool no_packed = false,
+                              bool blocked = false) {
     const int n = persist_config_count(ch->model);
     const bool uni = ch->ctx->uniform;
     auto grid = [&](int idx, bool* res) {
@@ -826,13 +827,16 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* re
         return true;
     };
     if (ch->cfg >= 0) return grid(ch->cfg, resident) ? ch->cfg : -1;
-    {
+    if (!blocked) {  // tuned choices come from resident single-chain probes
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(std::make_tuple(ch->model, ch->N, uni));
         if (it != g_tuned.end() && grid(it->second, resident)) return it->second;
     }
-    // default: the configuration with the fewest redundant halo positions per owned one,
-    // preferring those whose tile grid is resident
+    // Default.  A resident grid runs every tile at once, one per CU, so a step costs what one
+    // tile costs: the configuration with the least work per tile (tile plus halo, the packed
+    // sweep counted at half per position), the exchange every h steps added as 1/h, wins.  A grid
+    // run blocked (too many tiles, or a batch) streams every tile through HBM per block: there
+    // the fewest redundant halo positions per owned one win.
     int best = -1;
     double best_cost = 1e30;
     for (int i = 0; i < n; i++) {
@@ -840,8 +844,11 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* re
         if (!grid(i, &res)) continue;
         int tr, tc, h, nt;
         persist_config(ch->model, i, &tr, &tc, &h, &nt);
-        const double redund = (double)(tr + 2 * h) * (tc + 2 * h) / ((double)tr * tc);
-        const double cost = redund + 2.0 / h + (res ? 0.0 : 100.0);  // halo compute + barrier cost per step
+        const double ext = (double)(tr + 2 * h) * (tc + 2 * h);
+        const double redund = ext / ((double)tr * tc);
+        const double cost = (res && !blocked)
+                                ? ext * (1.0 + 1.0 / h) * (persist_config_packed(ch->model, i) ? 0.5 : 1.0)
+                                : 1e9 + redund + 2.0 / h;
         if (cost < best_cost) {
             best_cost = cost;
             best = i;
@@ -1284,7 +1291,7 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         bool all_packed_ok = true;
         for (rt_chain* ch : planar)
             if (ch->model == m && !packed_rule_ok(ch)) all_packed_ok = false;
-        g.cfg = choose_persist_cfg(largest, &nb, &tcn, &res, !all_packed_ok);
+        g.cfg = choose_persist_cfg(largest, &nb, &tcn, &res, !all_packed_ok, true);
         if (g.cfg < 0) return fail(RT_ERR_ARG, "rt_batch_run: no tile configuration for model %d", m);
         int tr, tc, h, nt;
         persist_config(m, g.cfg, &tr, &tc, &h, &nt);
