@@ -363,7 +363,9 @@ struct Cfg {
     X(8, 44, 8, 256, 1)      \
     X(16, 28, 8, 256, 1)     \
     X(16, 20, 8, 256, 1)     \
-    X(8, 20, 8, 256, 1)
+    X(8, 20, 8, 256, 1)      \
+    X(16, 44, 8, 512, 1)     \
+    X(24, 28, 8, 512, 1)
 #define RT_LOZENGE_CFGS(X)  \
     X(64, 84, 8, 1024, 1)   \
     X(64, 84, 4, 1024, 1)   \
