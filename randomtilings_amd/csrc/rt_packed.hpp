@@ -1,17 +1,18 @@
 // rt_packed.hpp — persistent lozenge and domino sweeps on packed words (gfx950).
 //
 // Same scheme as the generic persistent kernel (tile + halo in LDS, generators in VGPRs,
-// neighbour-flag exchange), but each thread works on quads: four horizontally adjacent
-// positions of one colour-class row held in one 32-bit LDS word.  All four share the row
-// index, so the update's row-dependent column offsets (lozengekernel.cl:45-48,
-// dominokernel.cl:22-31) are uniform over the quad and become byte funnel shifts of
-// neighbouring words, and the update of a quad is a handful of 32-bit mask/shift operations
-// instead of four byte-wise passes.
+// neighbour-flag exchange), but each thread works on whole 32-bit LDS words of one
+// colour-class row: four horizontally adjacent positions ("quads": lozenge and domino, one
+// byte each) or two (coupled domino pairs, 16 bits each).  All positions of a word share the
+// row index, so the update's row-dependent column offsets (lozengekernel.cl:45-48,
+// dominokernel.cl:22-31) are uniform over the word and become byte funnel shifts of
+// neighbouring words, and the update of a word is a handful of 32-bit mask/shift operations
+// instead of byte-wise passes.
 //
 // Flip (lozengekernel.cl:21-24 / qlozengekernel.cl:23-29, dominokernelCFTP.cl:25-29 as
-// thresholds): byte b of the quad toggles between the model's two flippable states (lozenge
-// 25 and 38, 25 ^ 38 = 63; domino 3 and 12, 3 ^ 12 = 15) when its state and its own
-// generator's draw say so; the toggles of the quad are applied with one xor.
+// thresholds): a byte of the word toggles between the model's two flippable states (lozenge
+// 25 and 38, 25 ^ 38 = 63; domino 3 and 12, 3 ^ 12 = 15) when its state and its position's
+// generator's draw say so; the toggles of the word are applied with one xor.
 #pragma once
 
 #include "rt_internal.hpp"
@@ -33,7 +34,7 @@ __device__ __forceinline__ uint32_t funnel(uint32_t hi, uint32_t lo, int sh) {
     return __builtin_amdgcn_alignbyte(hi, lo, sh);
 }
 
-// The two packed models.  Lozenge: three classes, the row index mod 3 selects the update's
+// The packed models.  Lozenge: three classes, the row index mod 3 selects the update's
 // offsets, one rule for every class.  Domino: two classes, the row parity selects the
 // offsets and (weighted rules) the rule; only rules whose two intervals split k at one
 // threshold t (k < t: 3 -> 12, k >= t: 12 -> 3; dominokernelCFTP.cl's rule and the weighted
@@ -72,7 +73,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
     // the byte mask of position b of a word
     auto posmask = [](int b) { return (BPP == 1 ? 0xffu : 0xffffu) << (8 * BPP * b); };
     constexpr int AREAW = ER * EW;                       // words per class
-    constexpr int QPT = (AREAW + NT - 1) / NT;           // quads per thread
+    constexpr int QPT = (AREAW + NT - 1) / NT;           // words ("quads") per thread
     constexpr int GUARD = EW + 4;                        // words before and after the classes
     __shared__ uint32_t lds[GUARD + NC * AREAW + GUARD];
     uint32_t* L = lds + GUARD;
@@ -86,8 +87,8 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
     const int c0w = c0 / QPW;
     const int tid = threadIdx.x;
 
-    // per quad: 4 generators, the byte masks of positions with a generator / updatable /
-    // owned, and the row index mod 3
+    // per word: a generator per position, the byte masks of positions with a generator /
+    // updatable / owned, and the row key (lozenge: row mod 3; domino: row parity)
     uint32_t s0[QPT][QPW], s1[QPT][QPW], s2[QPT][QPW], s3[QPT][QPW];
     uint32_t pm1[UNI ? 1 : QPT][UNI ? 1 : QPW], pm2[UNI ? 1 : QPT][UNI ? 1 : QPW], pmt[UNI ? 1 : QPT][UNI ? 1 : QPW];
     uint32_t gbytes[QPT], ubytes[QPT];
@@ -251,10 +252,11 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
 
     // Flip decisions.  A draw depends only on its generator, and the state a draw toggles
     // only on the draw (the rule is the same for every class), so the targets of a whole
-    // block can be drawn ahead of its flips.  targets() advances the four generators of quad
-    // g by one draw and returns the four target states packed into one word: 25 when k lies
-    // in A, 38 when in B, 0xff (never a state) otherwise.  SPLIT selects the one-comparison
-    // interval tests.  Every slot draws (a slot past the class area holds zero states and its
+    // block can be drawn ahead of its flips.  targets() advances the generators of word g by
+    // one draw each and returns the target states packed into one word (a pair position's
+    // target in both its bytes): lozenge 25 when k lies in A, 38 when in B, 0xff (never a
+    // state) otherwise; domino 3 below the threshold, 12 from it on.  SPLIT selects the
+    // one-comparison interval tests.  Every slot draws (a slot past the class area holds zero states and its
     // draws are discarded), so no generator state flows through a divergent branch.
     auto targets = [&](int g, auto split_c, int c) __attribute__((always_inline)) -> uint32_t {
         constexpr bool SPLIT = decltype(split_c)::value;
