@@ -111,10 +111,12 @@ int rt_ctx_set_path(rt_ctx* ctx, int path, int32_t tile_config);
 
 /* The integer thresholds on k = u>>9 that replace the reference's float decisions for
  * (model, rule, rparams): 24 words of domino/lozenge transition intervals
- * [class][row parity] (A->B lo, A->B hi, B->A lo, B->A hi) followed by 36 words of the
- * rectangle-triangle rule (gate, 3 band limits, 5x4 cycle and 12 toggle acceptance
- * limits).  Returns the word count (60), writing them when cap >= 60.  Host-only; used
- * to check the derivation against the reference predicates without a GPU. */
+ * [class][row parity] (A->B lo, A->B hi, B->A lo, B->A hi) followed by the
+ * rectangle-triangle rule: 36 words of thresholds (gate, 3 band limits, 5x4 cycle and 12
+ * toggle acceptance limits), then its lookup tables (32 hash keys, 32 indices, 5x4 cycle
+ * and 12 toggle targets).  Returns the word count (156), writing them when cap is at least
+ * that (out may be NULL to ask for the count).  Host-only; used to check the derivation
+ * against the reference predicates without a GPU. */
 int32_t rt_rule_thresholds(int model, int rule, const double* rparams, int32_t nrparams, uint32_t* out,
                            int32_t cap);
 

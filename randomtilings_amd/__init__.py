@@ -160,9 +160,14 @@ def lozenge_skip_map(tiling: np.ndarray, tr: int, tc: int, h: int):
 def rule_thresholds(model: int, rule: int, rparams=None) -> np.ndarray:
     lib = load()
     rp = None if rparams is None else np.ascontiguousarray(rparams, dtype=np.float64)
-    out = np.zeros(60, dtype=np.uint32)
+    # the first call reports the size (FlipRule and RectTriRule words), the second fills it
     n = lib.rt_rule_thresholds(model, rule, None if rp is None else rp.ctypes.data_as(_DP),
-                               0 if rp is None else len(rp), out.ctypes.data_as(_U32P), 60)
+                               0 if rp is None else len(rp), None, 0)
+    if n < 0:
+        _check(n)
+    out = np.zeros(n, dtype=np.uint32)
+    n = lib.rt_rule_thresholds(model, rule, None if rp is None else rp.ctypes.data_as(_DP),
+                               0 if rp is None else len(rp), out.ctypes.data_as(_U32P), n)
     if n < 0:
         _check(n)
     return out

@@ -26,7 +26,17 @@ struct RectTriRule {
     uint32_t band[3];     // rd/0.9f < .25/.5/.75  iff  k < band[b]
     uint32_t cyc_acc[5][4];  // 5-cycle state s, band b: accept iff k < cyc_acc[s][b]
     uint32_t tog_acc[12];    // paired toggles: accept iff k < tog_acc[t]
+    // The 17 states that can move, found by one hash probe instead of 17 comparisons:
+    // slot h = (state * kRtHash mod 2^32) >> 27 holds the state (hkey, 0xffffffff when empty)
+    // and its index (hval: 0-4 the 5-cycle, 5-16 the toggles); the targets per index and band.
+    uint32_t hkey[32];
+    uint32_t hval[32];
+    int32_t cyc_nt[5][4];
+    int32_t tog_nt[12];
 };
+// A multiplier (24-bit, for the 24 x 24-bit multiply) under which the 17 movable
+// rect-triangle states fall in distinct slots of 32.
+constexpr uint32_t kRtHash = 0xc9c201u;
 
 struct RuleParams {
     FlipRule fr;

@@ -73,8 +73,11 @@ struct PersistArgs {
 };
 
 // Resolve a batched block: the chain owning blockIdx.x, with the copies and schedule offset
-// of the current block of steps.
-__device__ __forceinline__ void resolve_batch(PersistArgs& a, int& tile, int& ntiles) {
+// of the current block of steps.  Returns the chain's rule tables where they lie, in the entry
+// in global memory: copying them into the caller's PersistArgs (which the kernel has to copy to
+// change) would make every lane hold that copy in scratch memory, the tables being indexed
+// per lane.
+__device__ __forceinline__ const RuleParams& resolve_batch(PersistArgs& a, int& tile, int& ntiles) {
     const int b = blockIdx.x;
     int lo = 0, hi = a.nbatch - 1;
     while (lo < hi) {
@@ -85,7 +88,6 @@ __device__ __forceinline__ void resolve_batch(PersistArgs& a, int& tile, int& nt
     const BatchEntry& e = a.batch[lo];
     const int64_t s0 = *a.step_ctr;
     const int par = (int)((s0 / a.block_h) & 1);
-    a.rp = e.rp;
     a.cls[0] = e.cls[0];
     a.cls[1] = e.cls[1];
     a.buf = e.buf0 ^ par;
@@ -98,6 +100,7 @@ __device__ __forceinline__ void resolve_batch(PersistArgs& a, int& tile, int& nt
     a.tiles_c = e.tiles_c;
     tile = b - e.tile0;
     ntiles = e.ntiles;
+    return e.rp;
 }
 
 hipError_t launch_init_gens(int64_t G, uint32_t seed, const uint4* params, uint32_t m1, uint32_t m2, uint32_t tm,

@@ -160,7 +160,8 @@ hipError_t launch_stepwise(int model, const StepwiseArgs& a, hipStream_t st) {
 // across XCDs without writing back or invalidating whole L2s), rows being 16-byte aligned.
 
 template <class M, int TR, int TC, int H, int NT, bool UNI>
-__device__ __forceinline__ void persist_body(const PersistArgs& a, const int tile, const int ntiles) {
+__device__ __forceinline__ void persist_body(const PersistArgs& a, const RuleParams& rules, const int tile,
+                                             const int ntiles) {
     using T = typename M::T;
     constexpr int NC = M::NC;
     constexpr int ER = TR + 2 * H, EC = TC + 2 * H;
@@ -187,7 +188,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const int til
     // scratch cannot rely on all its workgroups being resident.
     __shared__ RuleParams srp;
     for (int q = tid; q < (int)(sizeof(RuleParams) / 4); q += NT)
-        reinterpret_cast<uint32_t*>(&srp)[q] = reinterpret_cast<const uint32_t*>(&a.rp)[q];
+        reinterpret_cast<uint32_t*>(&srp)[q] = reinterpret_cast<const uint32_t*>(&rules)[q];
     __syncthreads();
     const RuleParams& rp = srp;
 
@@ -226,7 +227,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const int til
     // load tile + halo of every class (dwords outside the array read as zero; the row
     // padding past W holds zeros)
     auto load_ext = [&](int buf, bool skip_owned) {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.cls[buf]);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>((buf ? a.cls[1] : a.cls[0]));
         for (int q = tid; q < AREAW; q += NT) {
             const int li = q / EW, lw = q - (q / EW) * EW;
             if (skip_owned && li >= H && li < H + TR && lw >= HW && lw < HW + TW) continue;
@@ -281,7 +282,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const int til
         }
         // publish owned positions into the other copy
         {
-            uint32_t* dst = reinterpret_cast<uint32_t*>(a.cls[buf ^ 1]);
+            uint32_t* dst = reinterpret_cast<uint32_t*>((buf ? a.cls[0] : a.cls[1]));
             for (int q = tid; q < TR * TW; q += NT) {
                 const int oi = q / TW, ow = q - (q / TW) * TW;
                 const int gi = tr * TR + oi, gw = tc * TW + ow;
@@ -332,7 +333,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const int til
 
 template <class M, int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist(PersistArgs a) {
-    persist_body<M, TR, TC, H, NT, UNI>(a, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+    persist_body<M, TR, TC, H, NT, UNI>(a, a.rp, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 // batched launches (blocked path): every block runs one tile of one chain of the batch
@@ -340,8 +341,8 @@ template <class M, int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist_batch(PersistArgs a_in) {
     PersistArgs a = a_in;
     int tile, ntiles;
-    resolve_batch(a, tile, ntiles);
-    persist_body<M, TR, TC, H, NT, UNI>(a, tile, ntiles);
+    const RuleParams& rules = resolve_batch(a, tile, ntiles);
+    persist_body<M, TR, TC, H, NT, UNI>(a, rules, tile, ntiles);
 }
 
 // tile configurations per model: {TR, TC, H, NT, packed}; packed selects the quad kernel

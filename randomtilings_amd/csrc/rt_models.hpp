@@ -161,44 +161,27 @@ struct RectTriModel {
     __device__ __forceinline__ static const RectTriRule& step_rule(const RuleParams& rp, int c) { return rp.rt; }
 
     // recttrianglekernel.cl:25-118 with every float decision replaced by a threshold on k.
+    // The state is looked up in the rule's hash table of the 17 states that can move (the
+    // 5-cycle of three-rectangle / all-triangle hexagons, :43-64, and the 12 paired toggles,
+    // :66-112); the band of k selects a 5-cycle state's target (:45-63).
     __device__ __forceinline__ static T flip(const RectTriRule& r, int gi, T s, uint32_t k) {
         if (k >= r.gate) return s;
-        const int b = (k < r.band[0]) ? 0 : (k < r.band[1]) ? 1 : (k < r.band[2]) ? 2 : 3;
-        // 5-cycle of the three-rectangle / all-triangle hexagon states (:43-64)
-        int ci = -1;
-        if (s == 0x122133) ci = 0;
-        else if (s == 0x331221) ci = 1;
-        else if (s == 0x8ebbe8) ci = 2;
-        else if (s == 0xfc99cf) ci = 3;
-        else if (s == 0xa7dd7a) ci = 4;
-        if (ci >= 0) {
-            // band 0..3 targets (the select chains at :45-63)
-            int nt;
-            switch (ci) {
-                case 0: nt = (b == 0) ? 0xa7dd7a : (b == 1) ? 0xfc99cf : (b == 2) ? 0x8ebbe8 : 0x331221; break;
-                case 1: nt = (b == 0) ? 0xa7dd7a : (b == 1) ? 0xfc99cf : (b == 2) ? 0x8ebbe8 : 0x122133; break;
-                case 2: nt = (b == 0) ? 0xa7dd7a : (b == 1) ? 0xfc99cf : (b == 2) ? 0x122133 : 0x331221; break;
-                case 3: nt = (b == 0) ? 0xa7dd7a : (b == 1) ? 0x122133 : (b == 2) ? 0x8ebbe8 : 0x331221; break;
-                default: nt = (b == 0) ? 0x122133 : (b == 1) ? 0xfc99cf : (b == 2) ? 0x8ebbe8 : 0x331221; break;
-            }
-            return (k < r.cyc_acc[ci][b]) ? nt : s;
+        // __umul24 returns a signed int here: shift it as unsigned (an arithmetic shift would
+        // turn products with bit 31 set into negative slots)
+        const uint32_t h = ((uint32_t)__umul24((uint32_t)s, kRtHash) >> 27) & 31u;
+        if ((uint32_t)s != r.hkey[h]) return s;
+        const uint32_t idx = r.hval[h];
+        uint32_t thr;
+        int nt;
+        if (idx < 5) {
+            const int b = (k < r.band[0]) ? 0 : (k < r.band[1]) ? 1 : (k < r.band[2]) ? 2 : 3;
+            thr = r.cyc_acc[idx][b];
+            nt = r.cyc_nt[idx][b];
+        } else {
+            thr = r.tog_acc[idx - 5];
+            nt = r.tog_nt[idx - 5];
         }
-        // paired toggles (:66-112)
-        int ti = -1, nt = s;
-        if (s == 0x47d47a) { ti = 0; nt = 0xd22a33; }
-        else if (s == 0xd22a33) { ti = 1; nt = 0x47d47a; }
-        else if (s == 0xa74d74) { ti = 2; nt = 0x33a22d; }
-        else if (s == 0x33a22d) { ti = 3; nt = 0xa74d74; }
-        else if (s == 0x8eb558) { ti = 4; nt = 0x331eb1; }
-        else if (s == 0x331eb1) { ti = 5; nt = 0x8eb558; }
-        else if (s == 0x855be8) { ti = 6; nt = 0x1be133; }
-        else if (s == 0x1be133) { ti = 7; nt = 0x855be8; }
-        else if (s == 0x6699cf) { ti = 8; nt = 0xcf1221; }
-        else if (s == 0xcf1221) { ti = 9; nt = 0x6699cf; }
-        else if (s == 0xfc9966) { ti = 10; nt = 0x1221fc; }
-        else if (s == 0x1221fc) { ti = 11; nt = 0xfc9966; }
-        if (ti >= 0 && k < r.tog_acc[ti]) return nt;
-        return s;
+        return (k < thr) ? nt : s;
     }
 
     // recttrianglekernel.cl:120-146: updateTiles1 (class c -> c+1) and updateTiles2

@@ -63,7 +63,8 @@ struct PackedDominoPair {
 };
 
 template <class PM, int TR, int TC, int H, int NT, bool UNI>
-__device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile, const int ntiles) {
+__device__ __forceinline__ void packed_body(const PersistArgs& a, const RuleParams& rules, const int tile,
+                                            const int ntiles) {
     using M = typename PM::Model;
     constexpr int NC = PM::NC;
     constexpr int ER = TR + 2 * H, EC = TC + 2 * H;
@@ -138,7 +139,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
         lds[GUARD + NC * AREAW + q] = 0;
     }
     auto load_ext = [&](int buf, bool skip_owned) {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.cls[buf]);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>((buf ? a.cls[1] : a.cls[0]));
         for (int q = tid; q < AREAW; q += NT) {
             const int li = q / EW, lw = q - (q / EW) * EW;
             if (skip_owned && li >= H && li < H + TR && lw >= HW && lw < HW + TW) continue;
@@ -156,7 +157,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
     constexpr int QL = (AREAW + NT - 1) / NT;
     uint32_t hv[QL][NC];
     auto halo_issue = [&](int bufi) __attribute__((always_inline)) {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.cls[bufi]);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>((bufi ? a.cls[1] : a.cls[0]));
 #pragma unroll
         for (int g = 0; g < QL; g++) {
             const int q = tid + g * NT;
@@ -217,8 +218,8 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
     // draw is a function of the draw alone: 25 when k lies in A, 38 when it lies in B, and
     // 0xff (never a state) otherwise.  When A = [0, x) and B = [y, 2^23) are both nonempty
     // ("split": the q-rules) each test is one comparison of the raw output t0.
-    const uint32_t alo = a.rp.fr.ab_lo[0][0], ahi = a.rp.fr.ab_hi[0][0];
-    const uint32_t blo = a.rp.fr.ba_lo[0][0], bhi = a.rp.fr.ba_hi[0][0];
+    const uint32_t alo = rules.fr.ab_lo[0][0], ahi = rules.fr.ab_hi[0][0];
+    const uint32_t blo = rules.fr.ba_lo[0][0], bhi = rules.fr.ba_hi[0][0];
     const uint32_t asz = ahi - alo, bsz = bhi - blo;
     const bool split = !PM::kLozenge || (alo == 0u && ahi > 0u && ahi <= blo && blo < bhi && bhi == (1u << 23));
     const uint32_t aN = ahi << 9, bL = blo << 9;  // split: t0 < aN <=> k in A, t0 >= bL <=> k in B
@@ -236,8 +237,8 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
     if constexpr (!PM::kLozenge) {
 #pragma unroll
         for (int g = 0; g < QPT; g++) {
-            dth0[g] = a.rp.fr.ab_hi[0][rmod[g]] << 9;
-            dth1[g] = a.rp.fr.ab_hi[1][rmod[g]] << 9;
+            dth0[g] = rules.fr.ab_hi[0][rmod[g]] << 9;
+            dth1[g] = rules.fr.ab_hi[1][rmod[g]] << 9;
         }
     }
 
@@ -469,7 +470,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const int tile
         // along the tile's edges (the neighbours' halos read nothing else), after the last
         // block every owned word (the launch's result).
         {
-            uint32_t* dst = reinterpret_cast<uint32_t*>(a.cls[buf ^ 1]);
+            uint32_t* dst = reinterpret_cast<uint32_t*>((buf ? a.cls[0] : a.cls[1]));
             for (int q = tid; q < TR * TW; q += NT) {
                 const int oi = q / TW, ow = q - (q / TW) * TW;
                 const bool band = oi < H || oi >= TR - H || ow < HW || ow >= TW - HW;
@@ -536,33 +537,34 @@ template <int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist_loz4(PersistArgs a) {
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     const bool skip = a.live != nullptr;
-    packed_body<PackedLozenge, TR, TC, H, NT, UNI>(a, skip ? a.live[b] : b, skip ? a.ntiles : (int)gridDim.x);
+    packed_body<PackedLozenge, TR, TC, H, NT, UNI>(a, a.rp, skip ? a.live[b] : b,
+                                                   skip ? a.ntiles : (int)gridDim.x);
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist_loz4_batch(PersistArgs a_in) {
     PersistArgs a = a_in;
     int tile, ntiles;
-    resolve_batch(a, tile, ntiles);
-    packed_body<PackedLozenge, TR, TC, H, NT, UNI>(a, tile, ntiles);
+    const RuleParams& rules = resolve_batch(a, tile, ntiles);
+    packed_body<PackedLozenge, TR, TC, H, NT, UNI>(a, rules, tile, ntiles);
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist_dom4(PersistArgs a) {
-    packed_body<PackedDomino, TR, TC, H, NT, UNI>(a, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+    packed_body<PackedDomino, TR, TC, H, NT, UNI>(a, a.rp, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist_dom4_batch(PersistArgs a_in) {
     PersistArgs a = a_in;
     int tile, ntiles;
-    resolve_batch(a, tile, ntiles);
-    packed_body<PackedDomino, TR, TC, H, NT, UNI>(a, tile, ntiles);
+    const RuleParams& rules = resolve_batch(a, tile, ntiles);
+    packed_body<PackedDomino, TR, TC, H, NT, UNI>(a, rules, tile, ntiles);
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist_pair2(PersistArgs a) {
-    packed_body<PackedDominoPair, TR, TC, H, NT, UNI>(a, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+    packed_body<PackedDominoPair, TR, TC, H, NT, UNI>(a, a.rp, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 }  // namespace rt

@@ -179,6 +179,23 @@ bool recttri_rule(const float* weights16, RectTriRule& r) {
         const float ratio = weight_ratio(w, tog[t][0], tog[t][1]);
         r.tog_acc[t] = first_false(0, r.gate, [&](uint32_t k) { return ratio >= x_of(k); });
     }
+    // the hash table of the movable states and their targets
+    for (int h = 0; h < 32; h++) r.hkey[h] = 0xffffffffu;
+    auto put = [&](int state, uint32_t idx) {
+        const uint32_t h = (uint32_t)((uint32_t)state * kRtHash) >> 27;
+        if (r.hkey[h] != 0xffffffffu) return false;  // kRtHash keeps the 17 states apart
+        r.hkey[h] = (uint32_t)state;
+        r.hval[h] = idx;
+        return true;
+    };
+    for (int s = 0; s < 5; s++) {
+        if (!put(cyc[s], (uint32_t)s)) return false;
+        for (int b = 0; b < 4; b++) r.cyc_nt[s][b] = cyc_target[s][b];
+    }
+    for (int t = 0; t < 12; t++) {
+        if (!put(tog[t][0], 5u + t)) return false;
+        r.tog_nt[t] = tog[t][1];
+    }
     return true;
 }
 

@@ -180,3 +180,27 @@ def test_bad_arguments_raise():
         rt.rule_thresholds(rt.RT_LOZENGE, rt.RT_LOZENGE_QREF, [-1.0])
     with pytest.raises(rt.RTError):
         rt.rule_thresholds(rt.RT_DOMINO, rt.RT_DOMINO_WEIGHTED, [1.5, 0.5, 0.5, 0.5])
+
+
+def test_recttri_state_table():
+    """The rect-triangle rule's hash table (rt_rule_thresholds words 60..155): the 17 states
+    that can move sit at their slot h = (s * 0xc9c201 mod 2^32) >> 27 with their index, the
+    other slots are empty, and the toggle targets pair the 12 toggle states with each other."""
+    w = rt.rule_thresholds(rt.RT_RECTTRI, 0)
+    hkey, hval = w[60:92], w[92:124]
+    cyc_nt, tog_nt = w[124:144].reshape(5, 4), w[144:156]
+    cyc = [0x122133, 0x331221, 0x8ebbe8, 0xfc99cf, 0xa7dd7a]
+    tog = [0x47d47a, 0xd22a33, 0xa74d74, 0x33a22d, 0x8eb558, 0x331eb1, 0x855be8, 0x1be133, 0x6699cf,
+           0xcf1221, 0xfc9966, 0x1221fc]
+    slots = set()
+    for idx, s in enumerate(cyc + tog):
+        h = ((s * 0xc9c201) & 0xffffffff) >> 27
+        assert hkey[h] == s and hval[h] == idx
+        slots.add(h)
+    assert len(slots) == 17
+    assert all(hkey[h] == 0xffffffff for h in range(32) if h not in slots)
+    assert set(cyc_nt.ravel()) <= set(cyc)  # a 5-cycle state moves to another 5-cycle state
+    assert all(cyc_nt[i, b] != cyc[i] for i in range(5) for b in range(4))
+    for t in range(12):
+        partner = tog.index(int(tog_nt[t]))
+        assert tog_nt[partner] == tog[t]
