@@ -836,7 +836,9 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* re
     // tile costs: the configuration with the least work per tile (tile plus halo, the packed
     // sweep counted at half per position), the exchange every h steps added as 1/h, wins.  A grid
     // run blocked (too many tiles, or a batch) streams every tile through HBM per block: there
-    // the fewest redundant halo positions per owned one win.
+    // the fewest redundant halo positions per owned one win, divided by the fraction of 16 waves
+    // per CU the kernel keeps in flight (fewer leave the streaming latency exposed: the
+    // rect-triangle 48 x 48 tiles, 2 waves per SIMD, ran batches at 0.65x the 32 x 32 ones).
     int best = -1;
     double best_cost = 1e30;
     for (int i = 0; i < n; i++) {
@@ -846,9 +848,14 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* re
         persist_config(ch->model, i, &tr, &tc, &h, &nt);
         const double ext = (double)(tr + 2 * h) * (tc + 2 * h);
         const double redund = ext / ((double)tr * tc);
-        const double cost = (res && !blocked)
-                                ? ext * (1.0 + 1.0 / h) * (persist_config_packed(ch->model, i) ? 0.5 : 1.0)
-                                : 1e9 + redund + 2.0 / h;
+        double cost;
+        if (res && !blocked) {
+            cost = ext * (1.0 + 1.0 / h) * (persist_config_packed(ch->model, i) ? 0.5 : 1.0);
+        } else {
+            const int waves = persist_waves_per_cu(ch->model, i, uni, blocked);
+            const double occ = std::max(std::min(1.0, waves / 16.0), 1.0 / 64);
+            cost = 1e9 + (redund + 2.0 / h) / occ;
+        }
         if (cost < best_cost) {
             best_cost = cost;
             best = i;

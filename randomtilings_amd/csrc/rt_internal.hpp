@@ -121,6 +121,7 @@ bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt);
 bool persist_config_packed(int model, int idx);
 int persist_capacity(int model, int idx, bool uni, int device);
 bool persist_uses_scratch(int model, int idx, bool uni);
+int persist_waves_per_cu(int model, int idx, bool uni, bool batch);
 hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st);
 hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t st);
 

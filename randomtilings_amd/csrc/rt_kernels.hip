@@ -498,6 +498,18 @@ int persist_capacity(int model, int idx, bool uni, int device) {
     return std::min(per_cu, 1) * prop.multiProcessorCount;
 }
 
+// Waves per CU that launches of the configuration's kernel (the batched one with `batch`) keep
+// in flight, from the occupancy query: in a blocked run, where workgroups stream their tiles
+// through HBM, this is what hides the load latency.
+int persist_waves_per_cu(int model, int idx, bool uni, bool batch) {
+    const void* k = persist_kernel(model, idx, uni, batch);
+    int tr, tc, h, nt;
+    if (!k || !persist_config(model, idx, &tr, &tc, &h, &nt)) return 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, nt, 0) != hipSuccess) return 0;
+    return per_cu * nt / 64;
+}
+
 // Whether the configuration's kernel needs scratch (private) memory.  Waves that need scratch
 // are launched only as far as the runtime's scratch allocation reaches, which the occupancy
 // figure above does not bound, so such a grid is never treated as co-resident.
