@@ -153,8 +153,10 @@ int rt_recttri_walk(rt_ctx* ctx, const float* weights16, int32_t* tiling, int32_
  * The N x N lozenge-state tiling (N even, states 0..15); triangle and butterfly states are
  * derived on the device as TilingToTriangleFlips / TilingToButterflyFlips do.  Chains of
  * model RT_TRIDIMER: RT_PATH_PERSISTENT = every step in one single-workgroup launch (move
- * lattice N/2 <= 64), RT_PATH_STEPWISE = one tiled launch per step (tile_config selects the
- * tile), RT_PATH_AUTO = the former when it fits. */
+ * lattice N/2 <= 64), RT_PATH_STEPWISE = tiled launches (tile_config selects the tile, and
+ * configurations 3-5 run the tiles of 0-2 two steps per launch with a halo of 12 instead of
+ * 6), RT_PATH_AUTO = the former when it fits, else the tiled launches of the configuration
+ * rt_chain_autotune chose for the lattice (configuration 0 without tuning). */
 int rt_tridimer_walk(rt_ctx* ctx, int32_t* tiling, int32_t N, int64_t steps, int64_t seed);
 
 /* Coupling from the past for the monotone domino rules (RT_DOMINO_CFTP, RT_DOMINO_WEIGHTED),
@@ -267,7 +269,8 @@ int rt_lozenge_skip_map(const int32_t* tiling, int32_t N, int32_t tr, int32_t tc
  * block of h steps covering the tiles of every chain of the model (a chain-index table
  * maps blocks to chains); with use_graph the launch of one block and its step-counter
  * update are captured once as a HIP graph and replayed.  Triangle-dimer chains advance
- * together, one tiled launch per step (replayed the same way).  After rt_batch_run(b, k)
+ * together, one tiled launch per step on 32 x 32 tiles (replayed the same way), or on the
+ * configuration set on the largest of them (rt_chain_set_path; 3-5: two steps per launch).  After rt_batch_run(b, k)
  * every chain is exactly where rt_chain_run(ch, k) would have left it.  Chains stay owned by the caller and must outlive the batch. */
 int rt_batch_create(rt_ctx* ctx, rt_batch** out);
 int rt_batch_destroy(rt_batch* b);

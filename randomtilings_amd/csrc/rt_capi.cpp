@@ -275,12 +275,23 @@ static int td_run(rt_chain* ch, int64_t steps) {
             auto it = g_tuned.find(std::make_tuple(ch->model, ch->N, c->uniform));
             if (it != g_tuned.end()) cfg = it->second;
         }
-        for (int64_t s = 0; s < steps; s++) {
+        // two-step configurations advance pairs of steps per launch, an odd last step on the
+        // one-step kernel of the same tiles
+        const int base = td_config_base(cfg);
+        int64_t s = 0;
+        for (; s < steps; s += 1) {
+            const bool pair = td_config_steps(cfg) == 2 && s + 1 < steps;
             a.cells_in = (const uint4*)ch->cls[ch->cur];
             a.cells_out = (uint4*)ch->cls[ch->cur ^ 1];
             a.gen_in = ch->gens[ch->gcur];
             a.gen_out = ch->gens[ch->gcur ^ 1];
-            HIPCHK(td_launch_step(cfg, a, c->uniform, ch->hsched[s], c->stream));
+            if (pair) {
+                HIPCHK(td_launch_step2(base, a, c->uniform, (uint32_t)ch->hsched[s] | ((uint32_t)ch->hsched[s + 1] << 8),
+                                       c->stream));
+                s++;
+            } else {
+                HIPCHK(td_launch_step(base, a, c->uniform, ch->hsched[s], c->stream));
+            }
             ch->cur ^= 1;
             ch->gcur ^= 1;
             ch->last_launches++;
@@ -300,7 +311,7 @@ static int td_read(rt_chain* ch, int32_t* tiling) {
     return RT_OK;
 }
 
-// Measure every one-step tile configuration on a scratch copy of the chain (its own state is
+// Measure every tile configuration (one and two steps per launch) on a scratch copy of the chain (its own state is
 // not advanced); chains that fit one workgroup always run the single-workgroup kernel.
 static int td_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
     rt_ctx* c = ch->ctx;
@@ -1154,6 +1165,7 @@ int rt_tile_config(int model, int32_t idx, int32_t* rows, int32_t* cols, int32_t
     int tr, tc, h = kHalo, nt;
     const bool ok = model == RT_TRIDIMER ? td_config(idx, &tr, &tc, &nt) : persist_config(model, idx, &tr, &tc, &h, &nt);
     if (!ok) return fail(RT_ERR_ARG, "config %d out of range", idx);
+    if (model == RT_TRIDIMER) h = kHalo * td_config_steps(idx);  // a halo of kHalo per step of a launch
     *rows = tr;
     *cols = tc;
     *halo = h;
@@ -1338,17 +1350,15 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         b->dent_cap = b->hent.size();
     }
     // triangle-dimer entries: one tiled launch per step covers every chain
+    // configuration: the largest chain's own if set, else default 0 (32 x 32 tiles, one step
+    // per launch; tuned single-chain choices are not used, large tiles being slower in batches)
     int td_cfg = 0, td_blocks = 0;
     std::vector<TdBatchEntry> htd;
     if (!tdc.empty()) {
         rt_chain* largest = tdc[0];
         for (rt_chain* ch : tdc)
             if (ch->G > largest->G) largest = ch;
-        {
-            std::lock_guard<std::mutex> lk(g_tune_mu);
-            auto it = g_tuned.find(std::make_tuple((int)RT_TRIDIMER, largest->N, c->uniform));
-            if (it != g_tuned.end() && it->second >= 0) td_cfg = it->second;
-        }
+        if (largest->cfg >= 0) td_cfg = largest->cfg;
         int tr, tc, nt;
         td_config(td_cfg, &tr, &tc, &nt);
         for (size_t k = 0; k < tdc.size(); k++) {
@@ -1453,38 +1463,48 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         a.m2 = c->um2;
         a.tm = c->utm;
         HIPCHK(launch_fill_u32((uint32_t*)b->ctr, 4, 0u, c->stream));
-        if (use_graph) {
-            hipGraph_t graph = nullptr;
-            hipGraphExec_t exec = nullptr;
-            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e1 = td_launch_step_batch(td_cfg, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr,
-                                                 c->stream);
-            hipError_t e2 = launch_add_i64(b->ctr, 1, c->stream);
-            hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
-            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
-                if (graph) hipGraphDestroy(graph);
-                return fail(RT_ERR_HIP, "rt_batch_run: graph capture failed");
-            }
-            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            for (int64_t r = 0; r < steps && e == hipSuccess; r++) e = hipGraphLaunch(exec, c->stream);
-            if (exec) hipGraphExecDestroy(exec);
-            hipGraphDestroy(graph);
-            if (e != hipSuccess) return fail(RT_ERR_HIP, "rt_batch_run: graph replay: %s", hipGetErrorString(e));
-            b->graphs++;
-        } else {
-            for (int64_t r = 0; r < steps; r++) {
-                HIPCHK(td_launch_step_batch(td_cfg, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr,
-                                            c->stream));
-                HIPCHK(launch_add_i64(b->ctr, 1, c->stream));
+        // two-step configurations: pairs of steps, then an odd last step; each kind of launch
+        // (with its counter update) captured once and replayed
+        const int base = td_config_base(td_cfg), spl = td_config_steps(td_cfg);
+        const int64_t pairs = spl == 2 ? steps / 2 : 0, singles = steps - 2 * pairs;
+        for (int kind = 0; kind < 2; kind++) {
+            const bool two = kind == 0;
+            const int64_t count = two ? pairs : singles;
+            if (count == 0) continue;
+            if (use_graph) {
+                hipGraph_t graph = nullptr;
+                hipGraphExec_t exec = nullptr;
+                HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                hipError_t e1 = td_launch_step_batch(base, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr,
+                                                     two, spl, c->stream);
+                hipError_t e2 = launch_add_i64(b->ctr, two ? 2 : 1, c->stream);
+                hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
+                if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+                    if (graph) hipGraphDestroy(graph);
+                    return fail(RT_ERR_HIP, "rt_batch_run: graph capture failed");
+                }
+                hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                for (int64_t r = 0; r < count && e == hipSuccess; r++) e = hipGraphLaunch(exec, c->stream);
+                if (exec) hipGraphExecDestroy(exec);
+                hipGraphDestroy(graph);
+                if (e != hipSuccess) return fail(RT_ERR_HIP, "rt_batch_run: graph replay: %s", hipGetErrorString(e));
+                b->graphs++;
+            } else {
+                for (int64_t r = 0; r < count; r++) {
+                    HIPCHK(td_launch_step_batch(base, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr, two,
+                                                spl, c->stream));
+                    HIPCHK(launch_add_i64(b->ctr, two ? 2 : 1, c->stream));
+                }
             }
         }
-        b->launches += steps;
+        const int64_t launches = pairs + singles;
+        b->launches += launches;
         for (rt_chain* ch : tdc) {
-            if (steps & 1) {
+            if (launches & 1) {
                 ch->cur ^= 1;
                 ch->gcur ^= 1;
             }
-            ch->last_launches = steps;
+            ch->last_launches = launches;
             ch->last_cfg = td_cfg;
             ch->timed = false;
         }

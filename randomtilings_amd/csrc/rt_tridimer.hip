@@ -333,59 +333,61 @@ __device__ __forceinline__ void for_sites(const Region& R, int m, F f) {
     }
 }
 
-// The update kernels of one step after its Rotate (TriangleDimerTiler.cpp:217-338).
+// The update kernels of one step after its Rotate (TriangleDimerTiler.cpp:217-338).  m0: the
+// margin earlier steps of the same launch have already used up (their phases' inputs ran out
+// there), added to every phase's own.
 template <int NT>
-__device__ void replay_updates(const View& V, const Region& R, const Move& mv) {
+__device__ void replay_updates(const View& V, const Region& R, const Move& mv, const int m0 = 0) {
     if (mv.type == 0) {
         const int t = mv.r1, c = mv.r2;
-        for_sites<NT>(R, 1, [&](int i, int j, int gi, int gj) { loz_flipped(V, i, j, gi, gj, t, c); });
+        for_sites<NT>(R, m0 + 1, [&](int i, int j, int gi, int gj) { loz_flipped(V, i, j, gi, gj, t, c); });
         __syncthreads();
-        for_sites<NT>(R, 2, [&](int i, int j, int, int) { loz_update(V, i, j, t); });
+        for_sites<NT>(R, m0 + 2, [&](int i, int j, int, int) { loz_update(V, i, j, t); });
         __syncthreads();
-        for_sites<NT>(R, 3, [&](int i, int j, int, int) {
+        for_sites<NT>(R, m0 + 3, [&](int i, int j, int, int) {
             tri_u_from_loz(V, i, j);
             bfly_from_loz(V, i, j, 0);
             bfly_from_loz(V, i, j, 1);
             bfly_from_loz(V, i, j, 2);
         });
         __syncthreads();
-        for_sites<NT>(R, 4, [&](int i, int j, int, int) { tri_update(V, i, j, 0); });
+        for_sites<NT>(R, m0 + 4, [&](int i, int j, int, int) { tri_update(V, i, j, 0); });
     } else if (mv.type == 1) {
         const int c = mv.r1;
         const bool down = mv.r2 == 1;
-        for_sites<NT>(R, 1, [&](int i, int j, int gi, int gj) { tri_flipped(V, i, j, gi, gj, c, down); });
+        for_sites<NT>(R, m0 + 1, [&](int i, int j, int gi, int gj) { tri_flipped(V, i, j, gi, gj, c, down); });
         __syncthreads();
-        for_sites<NT>(R, 2, [&](int i, int j, int, int) { tri_update(V, i, j, down ? 1 : 0); });
+        for_sites<NT>(R, m0 + 2, [&](int i, int j, int, int) { tri_update(V, i, j, down ? 1 : 0); });
         __syncthreads();
-        for_sites<NT>(R, 3, [&](int i, int j, int, int) { loz_from_tri(V, i, j); });
+        for_sites<NT>(R, m0 + 3, [&](int i, int j, int, int) { loz_from_tri(V, i, j); });
         __syncthreads();
-        for_sites<NT>(R, 4, [&](int i, int j, int, int) {
+        for_sites<NT>(R, m0 + 4, [&](int i, int j, int, int) {
             bfly_from_loz(V, i, j, 0);
             bfly_from_loz(V, i, j, 1);
             bfly_from_loz(V, i, j, 2);
         });
     } else {
         const int t = mv.r1, p1 = mv.p1, p2 = mv.p2;
-        for_sites<NT>(R, 1, [&](int i, int j, int gi, int gj) { bfly_flipped1(V, i, j, gi, gj, t, p1, p2); });
+        for_sites<NT>(R, m0 + 1, [&](int i, int j, int gi, int gj) { bfly_flipped1(V, i, j, gi, gj, t, p1, p2); });
         __syncthreads();
-        for_sites<NT>(R, 2, [&](int i, int j, int gi, int gj) { bfly_flipped21(V, i, j, gi, gj, t, p1); });
+        for_sites<NT>(R, m0 + 2, [&](int i, int j, int gi, int gj) { bfly_flipped21(V, i, j, gi, gj, t, p1); });
         __syncthreads();
-        for_sites<NT>(R, 3, [&](int i, int j, int gi, int gj) { bfly_flipped22(V, i, j, gi, gj, t, p1); });
+        for_sites<NT>(R, m0 + 3, [&](int i, int j, int gi, int gj) { bfly_flipped22(V, i, j, gi, gj, t, p1); });
         __syncthreads();
         // a site's lozenge states depend only on its own (just final) butterfly state
-        for_sites<NT>(R, 4, [&](int i, int j, int gi, int gj) {
+        for_sites<NT>(R, m0 + 4, [&](int i, int j, int gi, int gj) {
             bfly_flipped23(V, i, j, gi, gj, t, p1);
             loz_from_bfly(V, i, j, t);
         });
         __syncthreads();
-        for_sites<NT>(R, 5, [&](int i, int j, int, int) {
+        for_sites<NT>(R, m0 + 5, [&](int i, int j, int, int) {
             if (t != 0) bfly_from_loz(V, i, j, 0);
             if (t != 1) bfly_from_loz(V, i, j, 1);
             if (t != 2) bfly_from_loz(V, i, j, 2);
             tri_u_from_loz(V, i, j);
         });
         __syncthreads();
-        for_sites<NT>(R, 6, [&](int i, int j, int, int) { tri_update(V, i, j, 0); });
+        for_sites<NT>(R, m0 + 6, [&](int i, int j, int, int) { tri_update(V, i, j, 0); });
     }
 }
 
@@ -489,10 +491,11 @@ __global__ __launch_bounds__(NT) void k_td_step(TdArgs a, uint32_t move) {
 }
 
 // Batched: every block runs one tile of one chain; the step index comes from a device
-// counter (one captured launch replays as is), and with it the chain's move and copies.
-template <int TR, int TC, int NT, bool UNI>
-__global__ __launch_bounds__(NT) void k_td_step_batch(TdArgs a0, const TdBatchEntry* __restrict__ batch, int nbatch,
-                                                      const int64_t* __restrict__ step_ctr) {
+// counter (one captured launch replays as is), and with it the chain's moves and copies: the
+// copies flip once per launch, and the launches so far are s / spl (spl: steps per launch of
+// the run's earlier launches).
+__device__ __forceinline__ const TdBatchEntry& td_resolve(TdArgs& a, const TdBatchEntry* __restrict__ batch,
+                                                         int nbatch, int64_t s, int spl, int& tile) {
     const int b = blockIdx.x;
     int lo = 0, hi = nbatch - 1;
     while (lo < hi) {
@@ -501,15 +504,142 @@ __global__ __launch_bounds__(NT) void k_td_step_batch(TdArgs a0, const TdBatchEn
         else hi = mid - 1;
     }
     const TdBatchEntry& e = batch[lo];
-    const int64_t s = *step_ctr;
-    const int par = (int)(s & 1);
-    TdArgs a = a0;
+    const int par = (int)((s / spl) & 1);
     a.M = e.M;
     a.cells_in = e.cells[e.buf0 ^ par];
     a.cells_out = e.cells[e.buf0 ^ par ^ 1];
     a.gen_in = e.gens[e.g0 ^ par];
     a.gen_out = e.gens[e.g0 ^ par ^ 1];
-    td_step_body<TR, TC, NT, UNI>(a, e.sched[s], b - e.tile0);
+    tile = b - e.tile0;
+    return e;
+}
+
+template <int TR, int TC, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_td_step_batch(TdArgs a0, const TdBatchEntry* __restrict__ batch, int nbatch,
+                                                      const int64_t* __restrict__ step_ctr, int spl) {
+    const int64_t s = *step_ctr;
+    TdArgs a = a0;
+    int tile;
+    const TdBatchEntry& e = td_resolve(a, batch, nbatch, s, spl, tile);
+    td_step_body<TR, TC, NT, UNI>(a, e.sched[s], tile);
+}
+
+// ---- S steps per launch, tiles + a halo of S * kHalo -------------------------------------
+// Every step's phases use up at most kHalo of margin at each cut edge, so after S steps the
+// owned tile, S * kHalo inside every cut edge, is exact; the sites in between hold values no
+// owned site reads.  The generators stay in registers between the steps, so the tile's cells
+// and generators cross HBM once per S steps (at S = 2 and 64 x 64 tiles 0.95 region loads per
+// owned site and step instead of 1.41).
+template <int TR, int TC, int NT, bool UNI, int S>
+__device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t moves, const int tile) {
+    constexpr int HH = S * kHalo;
+    constexpr int RR = TR + 2 * HH, RC = TC + 2 * HH;
+    __shared__ uint16_t lds[8 * RR * RC];
+    const int M = a.M;
+    const int tiles_c = (M + TC - 1) / TC;
+    const int ti = tile / tiles_c, tj = tile % tiles_c;
+    const int oi = ti * TR, oj = tj * TC;  // owned tile origin
+    Region R;
+    R.M = M;
+    R.gi0 = max(0, oi - HH);
+    R.gj0 = max(0, oj - HH);
+    const int gi1 = min(M, oi + TR + HH), gj1 = min(M, oj + TC + HH);
+    R.rows = gi1 - R.gi0;
+    R.cols = gj1 - R.gj0;
+    R.cutT = R.gi0 > 0;
+    R.cutL = R.gj0 > 0;
+    R.cutB = gi1 < M;
+    R.cutR = gj1 < M;
+    const View V{lds, RC, RR * RC};
+    const int n = R.rows * R.cols;
+    const int oi1 = min(M, oi + TR), oj1 = min(M, oj + TC);
+    constexpr int SPT = (RR * RC + NT - 1) / NT;  // sites per thread
+    const FastDiv fc((uint32_t)R.cols);
+    uint4 g[SPT];
+    {
+        uint4 cin[SPT];
+#pragma unroll
+        for (int q = 0; q < SPT; q++) {
+            const int idx = threadIdx.x + q * NT;
+            if (idx < n) {
+                const int qi = (int)fc.div((uint32_t)idx);
+                const size_t id = (size_t)(R.gi0 + qi) * M + (R.gj0 + idx - qi * R.cols);
+                cin[q] = a.cells_in[id];
+                g[q] = a.gen_in[id];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < SPT; q++) {
+            const int idx = threadIdx.x + q * NT;
+            if (idx < n) {
+                const int i = (int)fc.div((uint32_t)idx), j = idx - i * R.cols;
+                int v[8];
+                unpack_cell(cin[q], v);
+#pragma unroll
+                for (int p = 0; p < 8; p++) V.s(p, i, j, v[p]);
+            }
+        }
+    }
+    for (int s = 0; s < S; s++) {
+        const Move mv = decode((moves >> (8 * s)) & 0xffu);
+        const int rp = rotate_plane(mv);
+        if (s > 0) __syncthreads();  // the previous step's last phase wrote neighbours' states
+#pragma unroll
+        for (int q = 0; q < SPT; q++) {
+            const int idx = threadIdx.x + q * NT;
+            if (idx < n) {
+                const int i = (int)fc.div((uint32_t)idx), j = idx - i * R.cols;
+                const int gi = R.gi0 + i, gj = R.gj0 + j;
+                uint32_t m1, m2, tm;
+                load_params<UNI>(a, (size_t)gi * M + gj, m1, m2, tm);
+                const uint32_t u = tm_draw_raw(g[q].x, g[q].y, g[q].z, g[q].w, m1, m2, tm);
+                const int v = V.g(rp, i, j);
+                const int w = rotate_value(mv, v, gi, gj, u < 0x80000000u);
+                if (w != v) V.s(rp, i, j, w);
+            }
+        }
+        __syncthreads();
+        replay_updates<NT>(V, R, mv, s * kHalo);
+    }
+#pragma unroll
+    for (int q = 0; q < SPT; q++) {
+        const int idx = threadIdx.x + q * NT;
+        if (idx < n) {
+            const int qi = (int)fc.div((uint32_t)idx);
+            const int gi = R.gi0 + qi, gj = R.gj0 + idx - qi * R.cols;
+            if (gi >= oi && gi < oi1 && gj >= oj && gj < oj1) a.gen_out[(size_t)gi * M + gj] = g[q];
+        }
+    }
+    __syncthreads();
+    const int ow = oj1 - oj, on = (oi1 - oi) * ow;
+    const FastDiv fo((uint32_t)ow);
+    for (int idx = threadIdx.x; idx < on; idx += NT) {
+        const int qi = (int)fo.div((uint32_t)idx);
+        const int gi = oi + qi, gj = oj + idx - qi * ow;
+        const int i = gi - R.gi0, j = gj - R.gj0;
+        uint4 c;
+        c.x = V.g(0, i, j) | (V.g(1, i, j) << 16);
+        c.y = V.g(2, i, j) | (V.g(3, i, j) << 16);
+        c.z = V.g(4, i, j) | (V.g(5, i, j) << 16);
+        c.w = V.g(6, i, j) | (V.g(7, i, j) << 16);
+        a.cells_out[(size_t)gi * M + gj] = c;
+    }
+}
+
+// Two steps per launch: moves holds the two steps' move codes, the first in the low byte.
+template <int TR, int TC, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_td_step2(TdArgs a, uint32_t moves) {
+    td_multi_body<TR, TC, NT, UNI, 2>(a, moves, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+template <int TR, int TC, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_td_step2_batch(TdArgs a0, const TdBatchEntry* __restrict__ batch, int nbatch,
+                                                       const int64_t* __restrict__ step_ctr) {
+    const int64_t s = *step_ctr;
+    TdArgs a = a0;
+    int tile;
+    const TdBatchEntry& e = td_resolve(a, batch, nbatch, s, 2, tile);
+    td_multi_body<TR, TC, NT, UNI, 2>(a, (uint32_t)e.sched[s] | ((uint32_t)e.sched[s + 1] << 8), tile);
 }
 
 // ---- whole domain in one workgroup, every step in one launch ----------------------------
@@ -669,23 +799,40 @@ hipError_t launch_cfg(const TdArgs& a, uint32_t move, hipStream_t st) {
 }
 
 template <int I, bool UNI>
-hipError_t launch_batch_cfg(const TdArgs& a, const TdBatchEntry* batch, int nbatch, int blocks,
-                            const int64_t* step_ctr, hipStream_t st) {
+hipError_t launch_cfg2(const TdArgs& a, uint32_t moves, hipStream_t st) {
     constexpr TdCfg c = kCfgs[I];
-    hipLaunchKernelGGL((k_td_step_batch<c.tr, c.tc, c.nt, UNI>), dim3(blocks), dim3(c.nt), 0, st, a, batch, nbatch,
-                       step_ctr);
+    const int tiles = ((a.M + c.tr - 1) / c.tr) * ((a.M + c.tc - 1) / c.tc);
+    hipLaunchKernelGGL((k_td_step2<c.tr, c.tc, c.nt, UNI>), dim3(tiles), dim3(c.nt), 0, st, a, moves);
+    return hipGetLastError();
+}
+
+template <int I, bool UNI>
+hipError_t launch_batch_cfg(const TdArgs& a, const TdBatchEntry* batch, int nbatch, int blocks,
+                            const int64_t* step_ctr, bool two, int spl, hipStream_t st) {
+    constexpr TdCfg c = kCfgs[I];
+    if (two)
+        hipLaunchKernelGGL((k_td_step2_batch<c.tr, c.tc, c.nt, UNI>), dim3(blocks), dim3(c.nt), 0, st, a, batch,
+                           nbatch, step_ctr);
+    else
+        hipLaunchKernelGGL((k_td_step_batch<c.tr, c.tc, c.nt, UNI>), dim3(blocks), dim3(c.nt), 0, st, a, batch,
+                           nbatch, step_ctr, spl);
     return hipGetLastError();
 }
 
 }  // namespace td
 
-int td_config_count() { return td::kNumCfgs; }
+// Configurations 0 .. kNumCfgs-1 run one step per launch; kNumCfgs .. 2 kNumCfgs-1 run the same
+// tiles two steps per launch (a last odd step on the one-step kernel of the same tile).
+int td_config_count() { return 2 * td::kNumCfgs; }
+int td_config_base(int idx) { return idx % td::kNumCfgs; }
+int td_config_steps(int idx) { return idx >= td::kNumCfgs ? 2 : 1; }
 
 bool td_config(int idx, int* tr, int* tc, int* nt) {
-    if (idx < 0 || idx >= td::kNumCfgs) return false;
-    *tr = td::kCfgs[idx].tr;
-    *tc = td::kCfgs[idx].tc;
-    *nt = td::kCfgs[idx].nt;
+    if (idx < 0 || idx >= td_config_count()) return false;
+    const int b = td_config_base(idx);
+    *tr = td::kCfgs[b].tr;
+    *tc = td::kCfgs[b].tc;
+    *nt = td::kCfgs[b].nt;
     return true;
 }
 
@@ -709,15 +856,28 @@ hipError_t td_launch_small(const TdArgs& a, bool uni, hipStream_t st) {
 }
 
 
+// cfg: a one-step configuration (the tiles).  two: the launch runs two steps (sched[s] and
+// sched[s + 1]); otherwise one.  spl: steps per launch of the run's earlier launches (1 or 2),
+// which fixes the copy parity of step s.
 hipError_t td_launch_step_batch(int cfg, const TdArgs& a, bool uni, const TdBatchEntry* batch, int nbatch, int blocks,
-                                const int64_t* step_ctr, hipStream_t st) {
+                                const int64_t* step_ctr, bool two, int spl, hipStream_t st) {
+    if (spl != 1 && spl != 2) return hipErrorInvalidValue;
     switch (cfg) {
-        case 0: return uni ? td::launch_batch_cfg<0, true>(a, batch, nbatch, blocks, step_ctr, st)
-                           : td::launch_batch_cfg<0, false>(a, batch, nbatch, blocks, step_ctr, st);
-        case 1: return uni ? td::launch_batch_cfg<1, true>(a, batch, nbatch, blocks, step_ctr, st)
-                           : td::launch_batch_cfg<1, false>(a, batch, nbatch, blocks, step_ctr, st);
-        case 2: return uni ? td::launch_batch_cfg<2, true>(a, batch, nbatch, blocks, step_ctr, st)
-                           : td::launch_batch_cfg<2, false>(a, batch, nbatch, blocks, step_ctr, st);
+        case 0: return uni ? td::launch_batch_cfg<0, true>(a, batch, nbatch, blocks, step_ctr, two, spl, st)
+                           : td::launch_batch_cfg<0, false>(a, batch, nbatch, blocks, step_ctr, two, spl, st);
+        case 1: return uni ? td::launch_batch_cfg<1, true>(a, batch, nbatch, blocks, step_ctr, two, spl, st)
+                           : td::launch_batch_cfg<1, false>(a, batch, nbatch, blocks, step_ctr, two, spl, st);
+        case 2: return uni ? td::launch_batch_cfg<2, true>(a, batch, nbatch, blocks, step_ctr, two, spl, st)
+                           : td::launch_batch_cfg<2, false>(a, batch, nbatch, blocks, step_ctr, two, spl, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t td_launch_step2(int cfg, const TdArgs& a, bool uni, uint32_t moves, hipStream_t st) {
+    switch (cfg) {
+        case 0: return uni ? td::launch_cfg2<0, true>(a, moves, st) : td::launch_cfg2<0, false>(a, moves, st);
+        case 1: return uni ? td::launch_cfg2<1, true>(a, moves, st) : td::launch_cfg2<1, false>(a, moves, st);
+        case 2: return uni ? td::launch_cfg2<2, true>(a, moves, st) : td::launch_cfg2<2, false>(a, moves, st);
         default: return hipErrorInvalidValue;
     }
 }

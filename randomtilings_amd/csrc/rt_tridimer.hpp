@@ -42,11 +42,17 @@ inline uint8_t td_move(int type, int r1, int r2, int p1, int p2) {
 
 int td_config_count();
 bool td_config(int idx, int* tr, int* tc, int* nt);
+int td_config_base(int idx);   // the one-step configuration with the same tiles
+int td_config_steps(int idx);  // steps per launch: 1, or 2 for the two-step configurations
 hipError_t td_launch_derive(const int32_t* t, int N, uint4* cells, hipStream_t st);
 hipError_t td_launch_merge(const uint4* cells, int N, int32_t* t, hipStream_t st);
 hipError_t td_launch_small(const TdArgs& a, bool uni, hipStream_t st);
 hipError_t td_launch_step(int cfg, const TdArgs& a, bool uni, uint32_t move, hipStream_t st);
+// Two steps in one launch on the tiles of one-step configuration cfg; moves = first | second << 8.
+hipError_t td_launch_step2(int cfg, const TdArgs& a, bool uni, uint32_t moves, hipStream_t st);
+// cfg: one-step configuration (tiles); two: this launch runs two steps; spl: steps per launch
+// of the run's earlier launches (fixes the copy parity)
 hipError_t td_launch_step_batch(int cfg, const TdArgs& a, bool uni, const TdBatchEntry* batch, int nbatch, int blocks,
-                                const int64_t* step_ctr, hipStream_t st);
+                                const int64_t* step_ctr, bool two, int spl, hipStream_t st);
 
 }  // namespace rt

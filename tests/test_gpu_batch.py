@@ -90,3 +90,37 @@ def test_batch_matches_oracle(bctx, golden):
         b.close()
         c.close()
         d.close()
+
+
+@pytest.mark.parametrize("cfg", range(rt.tile_config_count(rt.RT_TRIDIMER)))
+def test_batch_tridimer_configs(bctx, golden, cfg):
+    """Triangle-dimer chains batched on every tile configuration (3 and up: two steps per
+    launch, an odd last step on the one-step kernel), over odd and even step counts: each chain
+    ends where its own run ends.  The largest chain's configuration serves the batch."""
+    t_big = S.call("tridimer.IceCreamCone", args=[150, 0])  # move lattice 154: several tiles
+    ts = [golden["tridimer_cone22_0"], t_big, golden["tridimer_cone9_2"]]
+    solo, batched = [], []
+    b = rt.Batch(bctx)
+    try:
+        for k, t0 in enumerate(ts):
+            a = bctx.chain(rt.RT_TRIDIMER, 0, t0)
+            c = bctx.chain(rt.RT_TRIDIMER, 0, t0)
+            a.seed(77 + k)
+            c.seed(77 + k)
+            c.set_path(rt.RT_PATH_STEPWISE, cfg)
+            solo.append(a)
+            batched.append(c)
+            b.add(c)
+        for steps in (37, 1, 64, 9):
+            say("batch tridimer", cfg, steps)
+            b.run(steps, True)
+            for a in solo:
+                a.run(steps)
+            b.sync()
+            for k, (a, c) in enumerate(zip(solo, batched)):
+                assert np.array_equal(a.read(), c.read()), (k, steps)
+            assert batched[1].last_config == cfg
+    finally:
+        b.close()
+        for ch in solo + batched:
+            ch.close()

@@ -335,7 +335,9 @@ def test_tridimer_large_cone_tiled(gctx, cfg):
 
 def test_tridimer_scrambled_params(sctx, golden):
     t0 = golden["tridimer_cone14_0"]
-    for path, cfg in [(rt.RT_PATH_AUTO, -1), (rt.RT_PATH_STEPWISE, 0)]:
+    # per-generator parameters on the one-step and two-step tiled kernels (3, 4: two steps)
+    for path, cfg in [(rt.RT_PATH_AUTO, -1), (rt.RT_PATH_STEPWISE, 0), (rt.RT_PATH_STEPWISE, 3),
+                      (rt.RT_PATH_STEPWISE, 4)]:
         say("tridimer scrambled", path, cfg)
         sctx.set_path(path, cfg)
         try:
