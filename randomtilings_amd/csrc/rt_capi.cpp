@@ -847,9 +847,11 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* re
     // tile costs: the configuration with the least work per tile (tile plus halo, the packed
     // sweep counted at half per position), the exchange every h steps added as 1/h, wins.  A grid
     // run blocked (too many tiles, or a batch) streams every tile through HBM per block: there
-    // the fewest redundant halo positions per owned one win, divided by the fraction of 16 waves
-    // per CU the kernel keeps in flight (fewer leave the streaming latency exposed: the
-    // rect-triangle 48 x 48 tiles, 2 waves per SIMD, ran batches at 0.65x the 32 x 32 ones).
+    // the fewest redundant halo positions per owned one win, plus 4/h for the launch, load and
+    // store of every block (batches of 32 lozenge hexagons: halo 8 at 2.3e11 flip attempts/s,
+    // every halo-4 tile at 1.3-1.5e11), divided by the fraction of 16 waves per CU the kernel
+    // keeps in flight (fewer leave the streaming latency exposed: the rect-triangle 48 x 48
+    // tiles, 2 waves per SIMD, ran batches at 0.65x the 32 x 32 ones).
     int best = -1;
     double best_cost = 1e30;
     for (int i = 0; i < n; i++) {
@@ -865,7 +867,7 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* re
         } else {
             const int waves = persist_waves_per_cu(ch->model, i, uni, blocked);
             const double occ = std::max(std::min(1.0, waves / 16.0), 1.0 / 64);
-            cost = 1e9 + (redund + 2.0 / h) / occ;
+            cost = 1e9 + (redund + 4.0 / h) / occ;
         }
         if (cost < best_cost) {
             best_cost = cost;
