@@ -1258,6 +1258,10 @@ int rt_batch_add(rt_batch* b, rt_chain* ch) {
     return RT_OK;
 }
 
+// Launches captured per graph round of a batched run (each at its own step offset), so one
+// step-counter update serves that many blocks.
+constexpr int64_t kRoundBlocks = 8;
+
 int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
     if (!b || steps < 0) return fail(RT_ERR_ARG, "rt_batch_run: bad arguments");
     rt_ctx* c = b->ctx;
@@ -1414,23 +1418,36 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         HIPCHK(launch_fill_u32((uint32_t*)b->ctr, 4, 0u, c->stream));
         a.nsteps = g.h;
         if (use_graph && full > 0) {
-            // one round (h steps of every chain of the group, then the step counter) captured
-            // once and replayed
+            // one round (up to kRoundBlocks blocks of h steps of every chain of the group, each
+            // launch at its step offset, then one step-counter update) captured once and
+            // replayed; the blocks left over run as the same launches directly
+            const int64_t R = std::min<int64_t>(full, kRoundBlocks), rem = full % R;
             hipGraph_t graph = nullptr;
             hipGraphExec_t exec = nullptr;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e1 = launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream);
-            hipError_t e2 = launch_add_i64(b->ctr, g.h, c->stream);
+            hipError_t e1 = hipSuccess;
+            for (int64_t j = 0; j < R; j++) {
+                a.step_off = j * g.h;
+                const hipError_t ej = launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream);
+                if (e1 == hipSuccess) e1 = ej;
+            }
+            hipError_t e2 = launch_add_i64(b->ctr, R * g.h, c->stream);
             hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
             if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
                 if (graph) hipGraphDestroy(graph);
                 return fail(RT_ERR_HIP, "rt_batch_run: graph capture failed");
             }
             hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            for (int64_t r = 0; r < full && e == hipSuccess; r++) e = hipGraphLaunch(exec, c->stream);
+            for (int64_t r = 0; r < full / R && e == hipSuccess; r++) e = hipGraphLaunch(exec, c->stream);
             if (exec) hipGraphExecDestroy(exec);
             hipGraphDestroy(graph);
             if (e != hipSuccess) return fail(RT_ERR_HIP, "rt_batch_run: graph replay: %s", hipGetErrorString(e));
+            for (int64_t j = 0; j < rem; j++) {
+                a.step_off = j * g.h;
+                HIPCHK(launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream));
+            }
+            if (rem > 0) HIPCHK(launch_add_i64(b->ctr, rem * g.h, c->stream));
+            a.step_off = 0;
             b->launches += full;
             b->graphs++;
         } else {
@@ -1473,29 +1490,40 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
             const bool two = kind == 0;
             const int64_t count = two ? pairs : singles;
             if (count == 0) continue;
+            const int k = two ? 2 : 1;  // steps per launch of this kind
             if (use_graph) {
+                // rounds of up to kRoundBlocks launches at their step offsets, one counter update
+                const int64_t R = std::min<int64_t>(count, kRoundBlocks), rem = count % R;
                 hipGraph_t graph = nullptr;
                 hipGraphExec_t exec = nullptr;
                 HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                hipError_t e1 = td_launch_step_batch(base, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr,
-                                                     two, spl, c->stream);
-                hipError_t e2 = launch_add_i64(b->ctr, two ? 2 : 1, c->stream);
+                hipError_t e1 = hipSuccess;
+                for (int64_t j = 0; j < R; j++) {
+                    const hipError_t ej = td_launch_step_batch(base, a, c->uniform, b->dtd, (int)htd.size(), td_blocks,
+                                                               b->ctr, j * k, two, spl, c->stream);
+                    if (e1 == hipSuccess) e1 = ej;
+                }
+                hipError_t e2 = launch_add_i64(b->ctr, R * k, c->stream);
                 hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
                 if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
                     if (graph) hipGraphDestroy(graph);
                     return fail(RT_ERR_HIP, "rt_batch_run: graph capture failed");
                 }
                 hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-                for (int64_t r = 0; r < count && e == hipSuccess; r++) e = hipGraphLaunch(exec, c->stream);
+                for (int64_t r = 0; r < count / R && e == hipSuccess; r++) e = hipGraphLaunch(exec, c->stream);
                 if (exec) hipGraphExecDestroy(exec);
                 hipGraphDestroy(graph);
                 if (e != hipSuccess) return fail(RT_ERR_HIP, "rt_batch_run: graph replay: %s", hipGetErrorString(e));
+                for (int64_t j = 0; j < rem; j++)
+                    HIPCHK(td_launch_step_batch(base, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr, j * k,
+                                                two, spl, c->stream));
+                if (rem > 0) HIPCHK(launch_add_i64(b->ctr, rem * k, c->stream));
                 b->graphs++;
             } else {
                 for (int64_t r = 0; r < count; r++) {
-                    HIPCHK(td_launch_step_batch(base, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr, two,
+                    HIPCHK(td_launch_step_batch(base, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr, 0, two,
                                                 spl, c->stream));
-                    HIPCHK(launch_add_i64(b->ctr, two ? 2 : 1, c->stream));
+                    HIPCHK(launch_add_i64(b->ctr, k, c->stream));
                 }
             }
         }

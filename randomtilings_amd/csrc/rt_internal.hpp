@@ -70,6 +70,7 @@ struct PersistArgs {
     int nbatch;
     int block_h;
     const int64_t* step_ctr;
+    int64_t step_off;  // steps this launch lies past the counter (launches of one captured round)
 };
 
 // Resolve a batched block: the chain owning blockIdx.x, with the copies and schedule offset
@@ -86,7 +87,7 @@ __device__ __forceinline__ const RuleParams& resolve_batch(PersistArgs& a, int& 
         else hi = mid - 1;
     }
     const BatchEntry& e = a.batch[lo];
-    const int64_t s0 = *a.step_ctr;
+    const int64_t s0 = *a.step_ctr + a.step_off;
     const int par = (int)((s0 / a.block_h) & 1);
     a.cls[0] = e.cls[0];
     a.cls[1] = e.cls[1];

@@ -516,8 +516,8 @@ __device__ __forceinline__ const TdBatchEntry& td_resolve(TdArgs& a, const TdBat
 
 template <int TR, int TC, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_td_step_batch(TdArgs a0, const TdBatchEntry* __restrict__ batch, int nbatch,
-                                                      const int64_t* __restrict__ step_ctr, int spl) {
-    const int64_t s = *step_ctr;
+                                                      const int64_t* __restrict__ step_ctr, int64_t off, int spl) {
+    const int64_t s = *step_ctr + off;
     TdArgs a = a0;
     int tile;
     const TdBatchEntry& e = td_resolve(a, batch, nbatch, s, spl, tile);
@@ -634,8 +634,8 @@ __global__ __launch_bounds__(NT) void k_td_step2(TdArgs a, uint32_t moves) {
 
 template <int TR, int TC, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_td_step2_batch(TdArgs a0, const TdBatchEntry* __restrict__ batch, int nbatch,
-                                                       const int64_t* __restrict__ step_ctr) {
-    const int64_t s = *step_ctr;
+                                                       const int64_t* __restrict__ step_ctr, int64_t off) {
+    const int64_t s = *step_ctr + off;
     TdArgs a = a0;
     int tile;
     const TdBatchEntry& e = td_resolve(a, batch, nbatch, s, 2, tile);
@@ -808,14 +808,14 @@ hipError_t launch_cfg2(const TdArgs& a, uint32_t moves, hipStream_t st) {
 
 template <int I, bool UNI>
 hipError_t launch_batch_cfg(const TdArgs& a, const TdBatchEntry* batch, int nbatch, int blocks,
-                            const int64_t* step_ctr, bool two, int spl, hipStream_t st) {
+                            const int64_t* step_ctr, int64_t off, bool two, int spl, hipStream_t st) {
     constexpr TdCfg c = kCfgs[I];
     if (two)
         hipLaunchKernelGGL((k_td_step2_batch<c.tr, c.tc, c.nt, UNI>), dim3(blocks), dim3(c.nt), 0, st, a, batch,
-                           nbatch, step_ctr);
+                           nbatch, step_ctr, off);
     else
         hipLaunchKernelGGL((k_td_step_batch<c.tr, c.tc, c.nt, UNI>), dim3(blocks), dim3(c.nt), 0, st, a, batch,
-                           nbatch, step_ctr, spl);
+                           nbatch, step_ctr, off, spl);
     return hipGetLastError();
 }
 
@@ -857,18 +857,18 @@ hipError_t td_launch_small(const TdArgs& a, bool uni, hipStream_t st) {
 
 
 // cfg: a one-step configuration (the tiles).  two: the launch runs two steps (sched[s] and
-// sched[s + 1]); otherwise one.  spl: steps per launch of the run's earlier launches (1 or 2),
-// which fixes the copy parity of step s.
+// sched[s + 1]); otherwise one.  s = *step_ctr + off.  spl: steps per launch of the run's
+// earlier launches (1 or 2), which fixes the copy parity of step s.
 hipError_t td_launch_step_batch(int cfg, const TdArgs& a, bool uni, const TdBatchEntry* batch, int nbatch, int blocks,
-                                const int64_t* step_ctr, bool two, int spl, hipStream_t st) {
+                                const int64_t* step_ctr, int64_t off, bool two, int spl, hipStream_t st) {
     if (spl != 1 && spl != 2) return hipErrorInvalidValue;
     switch (cfg) {
-        case 0: return uni ? td::launch_batch_cfg<0, true>(a, batch, nbatch, blocks, step_ctr, two, spl, st)
-                           : td::launch_batch_cfg<0, false>(a, batch, nbatch, blocks, step_ctr, two, spl, st);
-        case 1: return uni ? td::launch_batch_cfg<1, true>(a, batch, nbatch, blocks, step_ctr, two, spl, st)
-                           : td::launch_batch_cfg<1, false>(a, batch, nbatch, blocks, step_ctr, two, spl, st);
-        case 2: return uni ? td::launch_batch_cfg<2, true>(a, batch, nbatch, blocks, step_ctr, two, spl, st)
-                           : td::launch_batch_cfg<2, false>(a, batch, nbatch, blocks, step_ctr, two, spl, st);
+        case 0: return uni ? td::launch_batch_cfg<0, true>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st)
+                           : td::launch_batch_cfg<0, false>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st);
+        case 1: return uni ? td::launch_batch_cfg<1, true>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st)
+                           : td::launch_batch_cfg<1, false>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st);
+        case 2: return uni ? td::launch_batch_cfg<2, true>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st)
+                           : td::launch_batch_cfg<2, false>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st);
         default: return hipErrorInvalidValue;
     }
 }

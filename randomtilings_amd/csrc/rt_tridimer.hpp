@@ -50,9 +50,9 @@ hipError_t td_launch_small(const TdArgs& a, bool uni, hipStream_t st);
 hipError_t td_launch_step(int cfg, const TdArgs& a, bool uni, uint32_t move, hipStream_t st);
 // Two steps in one launch on the tiles of one-step configuration cfg; moves = first | second << 8.
 hipError_t td_launch_step2(int cfg, const TdArgs& a, bool uni, uint32_t moves, hipStream_t st);
-// cfg: one-step configuration (tiles); two: this launch runs two steps; spl: steps per launch
-// of the run's earlier launches (fixes the copy parity)
+// cfg: one-step configuration (tiles); the launch's first step is *step_ctr + off; two: it runs
+// two steps; spl: steps per launch of the run's earlier launches (fixes the copy parity)
 hipError_t td_launch_step_batch(int cfg, const TdArgs& a, bool uni, const TdBatchEntry* batch, int nbatch, int blocks,
-                                const int64_t* step_ctr, bool two, int spl, hipStream_t st);
+                                const int64_t* step_ctr, int64_t off, bool two, int spl, hipStream_t st);
 
 }  // namespace rt
