@@ -413,40 +413,11 @@ def bench_c3(args) -> int:
     from randomtilings_amd.params import default_table
 
     args.steps, args.warmup = 1, 0  # the line describes the single timed batched run
-    rng = np.random.default_rng(args.seed)
+    from randomtilings_amd import domains
+
     lo, hi = args.sides
     ctx = rt.Context(0)
-    specs = []
-    for k in range(args.domains):
-        kind = ("aztec", "notched", "tridimer", "recttri")[k % 4]
-        side = int(rng.integers(lo, hi + 1))
-        if kind == "aztec":
-            d = statics.call("domino.AztecDiamond", args=[max(1, side // 2 - 3)])
-            specs.append((rt.RT_DOMINO, rt.RT_DOMINO_CFTP, ctx.extremal_tiling(rt.RT_DOMINO, d, False)[0], kind))
-        elif kind == "notched":
-            while True:
-                m = side - 6
-                d = statics.call("domino.Rectangle", args=[m, m])
-                sd = int(round(np.sqrt(d.size)))
-                d = d.reshape(sd, sd).copy()
-                cut = 0
-                for i in range(m):
-                    step = int(rng.integers(0, 3))
-                    cut = min(m - 1, cut + step) if i < m // 2 else max(0, cut - step)
-                    d[2 + i, 2:2 + 2 * (cut // 2)] = 0
-                try:
-                    t0 = ctx.extremal_tiling(rt.RT_DOMINO, d.ravel(), False)[0]
-                    break
-                except rt.RTError:
-                    side = max(lo, side - 1)
-            specs.append((rt.RT_DOMINO, rt.RT_DOMINO_CFTP, t0, kind))
-        elif kind == "tridimer":
-            n = max(2, side // 2 - 4)
-            while ((n + 1) * (n + 2) // 2) % 2:
-                n += 1
-            specs.append((rt.RT_TRIDIMER, 0, statics.call("tridimer.IceCreamCone", args=[n, 0]), kind))
-        else:
-            specs.append((rt.RT_RECTTRI, 0, statics.call("recttri.maxHex", args=[max(1, (side - 7) // 2)]), kind))
+    specs = domains.mix(ctx, args.domains, (lo, hi), args.seed)
     gmax = max(int(t.size) for _, _, t, _ in specs)
     ctx.load_params(default_table(gmax))
     steps = args.domain_steps

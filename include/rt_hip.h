@@ -40,6 +40,13 @@ extern "C" {
 #define RT_LOZENGE 1
 #define RT_RECTTRI 2
 #define RT_TRIDIMER 3
+#define RT_DOMINO_PAIR 4 /* rt_chain_create only: the two coupled domino chains of coupling from
+                            the past (rt_domino_cftp) as one chain.  Each int of the N x N tiling
+                            holds the chain started from the maximal tiling in bits 0-7 and the
+                            one started from the minimal tiling in bits 8-15; one random number
+                            per site and step drives both, exactly as two Walk calls with the
+                            same seeds (Examples/AztecDiamondCftpEx.cpp:49-60).  Rules:
+                            RT_DOMINO_CFTP, RT_DOMINO_WEIGHTED.  No reference counterpart. */
 
 /* domino rules (the reference selects them by kernel file name) */
 #define RT_DOMINO_LAZY 0     /* dominokernel.cl: toggle 3<->12 iff rd < .8            */
@@ -101,6 +108,14 @@ int rt_load_params(rt_ctx* ctx, const uint32_t* mat3, int64_t n);
  * Replaces XTiler::LoadTinyMT(path, size) (DominoTiler.cpp:109-111,
  * file_reader.cpp:51-70,139-166). */
 int rt_load_params_file(rt_ctx* ctx, const char* path, int64_t n);
+
+/* Host only (no device): parse the first n parameter sets of a tinymtdc-format file into
+ * mat3 (3n words: mat1, mat2, tmat per entry), as tinymt::file_reader::get reads them
+ * (src/TinyMT/file_reader.cpp:51-70,114-166): lines starting with '#' are skipped; an empty
+ * line or the end of the file where an entry is expected, a line with fewer than five commas,
+ * or a field strtoul reports out of range is RT_ERR_IO (the reference throws
+ * std::runtime_error).  rt_load_params_file is this followed by rt_load_params. */
+int rt_parse_params_file(const char* path, int64_t n, uint32_t* mat3);
 
 /* Number of parameter sets currently loaded (0 if none). */
 int64_t rt_params_count(rt_ctx* ctx);
@@ -242,6 +257,13 @@ int64_t rt_chain_state_bytes(rt_chain* ch);
  * used (-1 for the stepwise path). */
 int64_t rt_chain_last_launches(rt_chain* ch);
 int32_t rt_chain_last_config(rt_chain* ch);
+
+/* Resident (persistent) grids are launched cooperatively, so the runtime either guarantees
+ * that all their workgroups are resident at once or refuses the launch; a refused run
+ * continues on the blocked path (same result).  This reports how many launches of this chain
+ * were refused and the hipError_t of the last refusal (0 if none).  Diagnostic; no
+ * counterpart in the reference. */
+int rt_chain_coop_refusals(rt_chain* ch, int64_t* count, int32_t* last_error);
 
 /* Skipped tiles (lozenge chains, packed sweep).  The reference rotates every position of the
  * N x N/3 class arrays (lozengekernel.cl:11-12); positions outside the domain hold code 0,

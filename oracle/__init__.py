@@ -32,6 +32,9 @@ def lib() -> C.CDLL:
         if not p.exists():
             raise RuntimeError(f"{p} missing: run `make -C oracle`")
         L = C.CDLL(str(p))
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_threads.restype = C.c_int
+        L.or_set_threads(default_threads())
         L.or_tinymt_uint32.argtypes = [C.c_uint32] * 4 + [C.c_int, _U32P]
         L.or_tinymt_table_draws.argtypes = [_U32P, C.c_int64, C.c_uint32, C.c_int64, C.c_int, _U32P, _U32P]
         L.or_tinymt_table_draws.restype = C.c_int
@@ -60,6 +63,23 @@ def lib() -> C.CDLL:
     return _or
 
 
+def default_threads() -> int:
+    """Threads the oracle's loops use unless set_threads says otherwise: the cores this
+    process may run on, at most 16."""
+    import os
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def set_threads(n: int) -> int:
+    """Run the oracle's per-kernel loops on n threads (1 = sequential); returns the previous
+    count.  The results do not depend on it."""
+    return int(lib().or_set_threads(int(n)))
+
+
 def ref_available() -> bool:
     return (HERE / "_ref" / "libref.so").exists()
 
@@ -76,8 +96,18 @@ def ref() -> C.CDLL:
         L.ref_write.argtypes = [C.c_char_p, _I32P, C.c_int64, _I32P, C.c_int64, C.c_char_p]
         L.ref_write.restype = C.c_int
         L.ref_tinymt_host.argtypes = [C.c_uint32] * 7 + [C.c_int, _U32P]
+        L.ref_tinymt_file_read.argtypes = [C.c_char_p, C.c_int64, _U32P]
+        L.ref_tinymt_file_read.restype = C.c_int
         _ref = L
     return _ref
+
+
+def ref_tinymt_file_read(path, n: int):
+    """The reference's tinymt::file_reader::get called n times: an (n, 3) uint32 array, or None
+    when the reader threw."""
+    out = np.zeros((n, 3), dtype=np.uint32)
+    rc = ref().ref_tinymt_file_read(str(path).encode(), n, out.ctypes.data_as(_U32P))
+    return None if rc else out
 
 
 def _arr(a):

@@ -18,6 +18,7 @@
 #include "TriangleDimer/TriangleDimerTiler.h"
 #include "common/common.h"
 #include "TinyMT/tinymt32.h"
+#include "TinyMT/file_reader.h"
 
 static int64_t put(const std::vector<int>& v, int32_t* out, int64_t cap) {
     if ((int64_t)v.size() > cap) return -2;
@@ -117,6 +118,19 @@ void ref_tinymt_host(uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3, uint32_
         tinymt32_next_state(&r);
         out[i] = tinymt32_temper(&r);
     }
+}
+
+// The reference's own parameter-file reader (src/TinyMT/file_reader.cpp:51-70,114-166):
+// n calls of tinymt::file_reader::get into out (3n words).  Returns 0, or -1 when the reader
+// threw (missing file, format error, comma not found).
+int ref_tinymt_file_read(const char* path, int64_t n, uint32_t* out) {
+    try {
+        tinymt::file_reader fr(path);
+        for (int64_t i = 0; i < n; i++) fr.get(&out[3 * i], &out[3 * i + 1], &out[3 * i + 2]);
+    } catch (const std::exception&) {
+        return -1;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -18,6 +18,7 @@
 //     C++-standard 10000th-output anchors.
 //   * Rotate/Update rules: legal-tiling invariants after every step, exact stationary
 //     laws by brute force on tiny domains, conservation laws.
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <cmath>
@@ -29,7 +30,18 @@
 #include <omp.h>
 #endif
 
+// Every loop over one kernel's NDRange below may run its rows on several threads: within one
+// launch a kernel reads one array and writes another, or writes only its own site and
+// generator, so the result does not depend on the order (or the threads) that visit the
+// sites.  or_set_threads chooses the thread count (1 = a plain sequential loop).
+// Small loops stay sequential (a parallel region costs more than they do).
+#define OR_PRAGMA(x) _Pragma(#x)
+#define OR_PAR_ROWS(work) \
+    OR_PRAGMA(omp parallel for schedule(static) if (g_threads > 1 && (int64_t)(work) >= 32768) num_threads(g_threads))
+
 namespace {
+
+int g_threads = 1;
 
 // ---------------------------------------------------------------------------------
 // TinyMT32 (src/TinyMT/tinymt32.clh)
@@ -110,7 +122,9 @@ struct Table {
     // InitTinyMT kernel: tinymt32_init(&tiny, seed+id) over NDRange(count); the uint
     // kernel argument plus a size_t id wraps modulo 2^32 (e.g. dominokernel.cl:33-40).
     void init(uint32_t seed, int64_t count) {
-        for (int64_t id = 0; id < count && id < (int64_t)g.size(); id++)
+        const int64_t n = std::min<int64_t>(count, (int64_t)g.size());
+        OR_PAR_ROWS(n)
+        for (int64_t id = 0; id < n; id++)
             tm_init(g[id], (uint32_t)(seed + (uint64_t)id));
     }
 };
@@ -130,6 +144,7 @@ struct DominoRule {
 void domino_rotate(const DominoRule& R, std::vector<TM>& gens, uint8_t* a, int N, int t) {
     const int W = N / 2;
     const int G0 = N - 2, G1 = W - 2;
+    OR_PAR_ROWS((int64_t)G0 * G1)
     for (int g0 = 0; g0 < G0; g0++) {
         for (int g1 = 0; g1 < G1; g1++) {
             const size_t id = (size_t)g1 + (size_t)G1 * g0;  // tinymt.clh:23-30
@@ -162,6 +177,7 @@ void domino_rotate(const DominoRule& R, std::vector<TM>& gens, uint8_t* a, int N
 // dominokernel.cl:22-31 (identical in dominokernelCFTP.cl:33-42)
 void domino_update(uint8_t* dst, const uint8_t* ref, int N, int t) {
     const int W = N / 2;
+    OR_PAR_ROWS((int64_t)N * W)
     for (int i = 1; i <= N - 2; i++) {
         for (int j = 1; j <= W - 2; j++) {
             dst[i * W + j] = (uint8_t)((ref[(i - 1) * W + j] & 2) / 2 + 2 * (ref[(i + 1) * W + j] & 1) +
@@ -207,6 +223,7 @@ void domino_steps(const DominoRule& R, std::vector<TM>& gens, std::vector<uint8_
 // ---------------------------------------------------------------------------------
 void lozenge_rotate(int rule, double q, std::vector<TM>& gens, uint8_t* a, int N) {
     const int M = N / 3;
+    OR_PAR_ROWS((int64_t)N * M)
     for (int i = 0; i < N; i++) {
         for (int j = 0; j < M; j++) {
             const size_t id = (size_t)j + (size_t)M * i;
@@ -240,6 +257,7 @@ void lozenge_rotate(int rule, double q, std::vector<TM>& gens, uint8_t* a, int N
 // lozengekernel.cl:36-54
 void lozenge_update(const uint8_t* t1, uint8_t* t2, uint8_t* t3, int N, int t) {
     const int M = N / 3;
+    OR_PAR_ROWS((int64_t)N * M)
     for (int i = 1; i < N - 1; i++) {
         const int p2 = (i % 3 == (2 * t + 1) % 3), p4 = (i % 3 == (2 * t + 2) % 3);
         const int p8 = (i % 3 == (2 * t) % 3), p16 = (i % 3 == (2 * t + 1) % 3);
@@ -271,6 +289,7 @@ float rt_ratio(const float* w, int a, int b) {  // recttrianglekernel.cl:157-160
 
 void recttri_flip(const float* w, std::vector<TM>& gens, int32_t* a, int N) {  // recttrianglekernel.cl:25-118
     const int M = N / 3;
+    OR_PAR_ROWS((int64_t)N * M)
     for (int i = 0; i < N; i++) {
         for (int j = 0; j < M; j++) {
             const size_t id = (size_t)j + (size_t)M * i;
@@ -327,6 +346,7 @@ void recttri_flip(const float* w, std::vector<TM>& gens, int32_t* a, int N) {  /
 
 void recttri_update1(const int32_t* t1, int32_t* t2, int N, int t) {  // recttrianglekernel.cl:120-132
     const int M = N / 3;
+    OR_PAR_ROWS((int64_t)N * M)
     for (int i = 1; i < N - 1; i++) {
         const int p1 = (i % 3 == t), p2 = (i % 3 == (t + 1) % 3);
         for (int j = 1; j < M - 1; j++) {
@@ -339,6 +359,7 @@ void recttri_update1(const int32_t* t1, int32_t* t2, int N, int t) {  // recttri
 
 void recttri_update2(const int32_t* t1, int32_t* t2, int N, int t) {  // recttrianglekernel.cl:134-146
     const int M = N / 3;
+    OR_PAR_ROWS((int64_t)N * M)
     for (int i = 1; i < N - 1; i++) {
         const int p1 = (i % 3 == (t + 2) % 3), p2 = (i % 3 == (t + 1) % 3);
         for (int j = 1; j < M - 1; j++) {
@@ -780,6 +801,13 @@ void td_step(TdArrays& s, std::vector<TM>& g, int type, int r1, int r2, int p1, 
 }  // namespace
 
 extern "C" {
+
+// Threads the per-kernel loops may use (1: sequential).  Returns the previous count.
+int or_set_threads(int n) {
+    const int old = g_threads;
+    g_threads = n < 1 ? 1 : n;
+    return old;
+}
 
 // ---- TinyMT known-answer helpers ----------------------------------------------------
 // init(seed) then n tinymt32_uint32 outputs

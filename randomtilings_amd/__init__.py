@@ -21,7 +21,7 @@ RT_OK, RT_ERR_ARG, RT_ERR_PARAMS, RT_ERR_HIP, RT_ERR_IO, RT_ERR_DEVICE, RT_ERR_U
 RT_ERR_NOT_COALESCED = -7
 RT_CFTP_REFERENCE, RT_CFTP_DOUBLING = 0, 1
 # models
-RT_DOMINO, RT_LOZENGE, RT_RECTTRI, RT_TRIDIMER = 0, 1, 2, 3
+RT_DOMINO, RT_LOZENGE, RT_RECTTRI, RT_TRIDIMER, RT_DOMINO_PAIR = 0, 1, 2, 3, 4
 # rules
 RT_DOMINO_LAZY, RT_DOMINO_CFTP, RT_DOMINO_WEIGHTED = 0, 1, 2
 RT_LOZENGE_UNIFORM, RT_LOZENGE_QREF, RT_LOZENGE_QHEAT = 0, 1, 2
@@ -44,6 +44,7 @@ SIGNATURES = {
     "rt_destroy": (C.c_int, [_P]),
     "rt_load_params": (C.c_int, [_P, _U32P, C.c_int64]),
     "rt_load_params_file": (C.c_int, [_P, C.c_char_p, C.c_int64]),
+    "rt_parse_params_file": (C.c_int, [C.c_char_p, C.c_int64, _U32P]),
     "rt_params_count": (C.c_int64, [_P]),
     "rt_ctx_set_path": (C.c_int, [_P, C.c_int, C.c_int32]),
     "rt_ctx_mark": (C.c_int, [_P, C.c_int32]),
@@ -76,6 +77,7 @@ SIGNATURES = {
     "rt_tile_config_packed": (C.c_int32, [C.c_int, C.c_int32]),
     "rt_chain_last_launches": (C.c_int64, [_P]),
     "rt_chain_last_config": (C.c_int32, [_P]),
+    "rt_chain_coop_refusals": (C.c_int, [_P, _I64P, _I32P]),
     "rt_chain_set_skip": (C.c_int, [_P, C.c_int]),
     "rt_chain_last_grid": (C.c_int, [_P, _I32P, _I32P]),
     "rt_lozenge_skip_map": (C.c_int, [_I32P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_uint8), _I32P,
@@ -155,6 +157,14 @@ def lozenge_skip_map(tiling: np.ndarray, tr: int, tc: int, h: int):
     _check(load().rt_lozenge_skip_map(_i32(t), N, tr, tc, h, m.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(run),
                                       C.byref(ok)))
     return m.reshape((N + tr - 1) // tr, (W + tc - 1) // tc), bool(ok.value)
+
+
+def parse_params_file(path, n: int) -> np.ndarray:
+    """Host only: the first n (mat1, mat2, tmat) entries of a tinymtdc-format file, as an
+    (n, 3) uint32 array (rt_parse_params_file; raises RTError RT_ERR_IO on a format error)."""
+    out = np.zeros((n, 3), dtype=np.uint32)
+    _check(load().rt_parse_params_file(str(path).encode(), n, out.ctypes.data_as(_U32P)))
+    return out
 
 
 def rule_thresholds(model: int, rule: int, rparams=None) -> np.ndarray:
@@ -390,6 +400,14 @@ class Chain:
     @property
     def last_config(self) -> int:
         return int(self._lib.rt_chain_last_config(self.h))
+
+    @property
+    def coop_refusals(self):
+        """(resident launches the runtime refused, hipError_t of the last refusal): such runs
+        continued on the blocked path."""
+        n, e = C.c_int64(), C.c_int32()
+        _check(self._lib.rt_chain_coop_refusals(self.h, C.byref(n), C.byref(e)))
+        return n.value, e.value
 
     def set_skip(self, enable: bool) -> None:
         """Skip (default) or run the tiles whose cells never change (lozenge, packed sweep)."""

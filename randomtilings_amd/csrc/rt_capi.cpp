@@ -112,6 +112,8 @@ struct rt_chain {
     bool consistent = false;
     std::vector<Runs> runs;
     int32_t last_tiles = 0, last_running = 0;
+    int64_t coop_refused = 0;  // resident launches the runtime refused (continued blocked)
+    int coop_error = 0;        // the hipError_t of the last refusal
 };
 
 // The tiles a persistent run of configuration cfg skips, or nullptr when every tile runs.
@@ -494,38 +496,42 @@ int rt_load_params(rt_ctx* c, const uint32_t* mat3, int64_t n) {
     return RT_OK;
 }
 
-// file_reader.cpp:114-166: the hex fields after the 3rd, 4th and 5th comma of each
-// non-'#' line are mat1, mat2, tmat.  Read sequentially (the reference reopens the file
-// and seeks for every entry).
-int rt_load_params_file(rt_ctx* c, const char* path, int64_t n) {
-    if (!c || !path || n <= 0) return fail(RT_ERR_ARG, "rt_load_params_file: bad args");
+// file_reader.cpp:51-70,114-166: entry i is the i-th line that does not start with '#'; its
+// mat1, mat2 and tmat are the hexadecimal numbers (strtoul base 16) after the 3rd, 4th and 5th
+// comma.  An empty line where an entry is expected (also the end of the file), a missing comma
+// or a number strtoul reports out of range (errno) is a format error, as in the reference.
+// Read sequentially (the reference reopens the file and seeks for every entry).
+int rt_parse_params_file(const char* path, int64_t n, uint32_t* mat3) {
+    if (!path || n <= 0 || !mat3) return fail(RT_ERR_ARG, "rt_parse_params_file: bad args");
     std::ifstream f(path);
     if (!f) return fail(RT_ERR_IO, "parameter file not found: %s", path);
-    std::vector<uint32_t> m;
-    m.reserve((size_t)n * 3);
     std::string line;
-    int64_t got = 0;
-    while (got < n && std::getline(f, line)) {
-        if (!line.empty() && line[0] == '#') continue;
-        if (line.empty()) return fail(RT_ERR_IO, "file format error at entry %lld", (long long)got);
-        uint32_t v[3];
+    for (int64_t got = 0; got < n; got++) {
+        bool have = false;
+        while ((have = (bool)std::getline(f, line)) && !line.empty() && line[0] == '#') {
+        }
+        if (!have || line.empty())
+            return fail(RT_ERR_IO, "file format error at entry %lld of %s (%s)", (long long)got, path,
+                        have ? "empty line" : "end of file");
+        errno = 0;
         for (int k = 0; k < 3; k++) {
             int commas = 0;
             size_t pos = 0;
             for (; pos < line.size(); pos++)
                 if (line[pos] == ',' && ++commas == 3 + k) break;
             if (pos >= line.size()) return fail(RT_ERR_IO, "comma not found in line: %s", line.c_str());
-            errno = 0;
-            v[k] = (uint32_t)strtoul(line.c_str() + pos + 1, nullptr, 16);
+            mat3[3 * got + k] = (uint32_t)strtoul(line.c_str() + pos + 1, nullptr, 16);
         }
-        m.push_back(v[0]);
-        m.push_back(v[1]);
-        m.push_back(v[2]);
-        got++;
+        if (errno != 0) return fail(RT_ERR_IO, "file format error: %s", line.c_str());
     }
-    if (got < n)
-        return fail(RT_ERR_IO, "parameter file %s holds %lld entries, %lld requested", path, (long long)got,
-                    (long long)n);
+    return RT_OK;
+}
+
+int rt_load_params_file(rt_ctx* c, const char* path, int64_t n) {
+    if (!c || !path || n <= 0) return fail(RT_ERR_ARG, "rt_load_params_file: bad args");
+    std::vector<uint32_t> m((size_t)n * 3);
+    const int rc = rt_parse_params_file(path, n, m.data());
+    if (rc != RT_OK) return rc;
     return rt_load_params(c, m.data(), n);
 }
 
@@ -920,6 +926,7 @@ int rt_chain_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
     int rc = rt_chain_seed(t, RT_SEED_WALK, 12345);
     int chosen = -1;
     double best_ms = 1e30;
+    std::string last_fault;
     const int n = persist_config_count(ch->model);
     for (int i = 0; i < n && rc == RT_OK; i++) {
         t->path = RT_PATH_PERSISTENT;
@@ -936,6 +943,23 @@ int rt_chain_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
             if (rc == RT_OK) rc = rt_chain_sync(t);
             if (rc == RT_OK && m < ms) ms = m;
         }
+        if (rc == RT_ERR_DEVICE) {
+            // This configuration raised the device error flag (e.g. a bounded wait expired):
+            // it is not used.  The flag is sticky, so clear it, the exchange flags and the
+            // block epoch, and restore the chain's state before probing the next one.
+            last_fault = g_err;
+            hipError_t er = hipStreamSynchronize(c->stream);
+            if (er == hipSuccess) er = hipMemsetAsync(t->misc, 0, 64, c->stream);
+            if (er == hipSuccess) er = hipMemsetAsync(t->bar, 0, kBarBytes, c->stream);
+            for (int b = 0; b < 2 && er == hipSuccess; b++)
+                er = hipMemcpyAsync(t->cls[b], ch->cls[ch->cur], bytes, hipMemcpyDeviceToDevice, c->stream);
+            if (er == hipSuccess) er = hipStreamSynchronize(c->stream);
+            t->epoch = 0;
+            t->cur = 0;
+            rc = er == hipSuccess ? rt_chain_seed(t, RT_SEED_WALK, 12345)
+                                  : fail(RT_ERR_HIP, "autotune reset: %s", hipGetErrorString(er));
+            continue;
+        }
         if (rc == RT_OK && ms < best_ms) {
             best_ms = ms;
             chosen = i;
@@ -943,7 +967,9 @@ int rt_chain_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
     }
     rt_chain_destroy(t);
     if (rc != RT_OK) return rc;
-    if (chosen < 0) return fail(RT_ERR_ARG, "no persistent tile configuration fits this chain");
+    if (chosen < 0)
+        return fail(last_fault.empty() ? RT_ERR_ARG : RT_ERR_DEVICE, "no persistent tile configuration %s%s",
+                    last_fault.empty() ? "fits this chain" : "ran without a device error: ", last_fault.c_str());
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[std::make_tuple(ch->model, ch->N, c->uniform)] = chosen;
@@ -1022,8 +1048,11 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         ch->last_running = nblocks;
         HIPCHK(hipEventRecord(ch->ev0, c->stream));
         // one launch per chunk keeps single kernels short; chunk is a multiple of h.  A grid
-        // that is not resident runs one block of h steps per launch.
-        const int64_t chunk = resident ? (int64_t)h * 2048 : (int64_t)h;
+        // that is not resident runs one block of h steps per launch.  Resident grids launch
+        // cooperatively: the runtime then guarantees that every workgroup is resident at once
+        // (the neighbour waits rely on it), or refuses the launch, in which case the chain
+        // continues on the blocked path (the same kernel, one block per launch, no waits).
+        int64_t chunk = resident ? (int64_t)h * 2048 : (int64_t)h;
         for (int64_t s0 = 0; s0 < steps; s0 += chunk) {
             const int64_t ns = std::min<int64_t>(chunk, steps - s0);
             PersistArgs a{};
@@ -1050,7 +1079,21 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
             a.live = runs ? runs->d_list : nullptr;
             a.live_map = runs ? runs->d_map : nullptr;
             a.ntiles = ch->last_tiles;
-            HIPCHK(launch_persist(ch->model, cfg, c->uniform, a, nblocks, c->stream));
+            const hipError_t el = launch_persist(ch->model, cfg, c->uniform, a, nblocks, c->stream, resident);
+            if (el != hipSuccess && resident) {
+                // Refused before anything ran: the grid cannot be resident
+                // (hipErrorCooperativeLaunchTooLarge), or the runtime cannot give this process
+                // a cooperative queue now (hipErrorIllegalState was seen while another process
+                // held one).  Redo this chunk, and the rest of the run, on the blocked path.
+                (void)hipGetLastError();
+                ch->coop_error = (int)el;
+                resident = false;
+                chunk = h;
+                s0 -= chunk;
+                ch->coop_refused++;
+                continue;
+            }
+            HIPCHK(el);
             ch->gcur ^= 1;
             const int64_t nblk = (ns + h - 1) / h;  // each block of h steps publishes once
             ch->epoch += (uint32_t)(nblk - 1);
@@ -1182,6 +1225,13 @@ int32_t rt_tile_config_packed(int model, int32_t idx) {
 int64_t rt_chain_last_launches(rt_chain* ch) { return ch ? ch->last_launches : 0; }
 
 int32_t rt_chain_last_config(rt_chain* ch) { return ch ? ch->last_cfg : -1; }
+
+int rt_chain_coop_refusals(rt_chain* ch, int64_t* count, int32_t* last_error) {
+    if (!ch) return fail(RT_ERR_ARG, "rt_chain_coop_refusals: null chain");
+    if (count) *count = ch->coop_refused;
+    if (last_error) *last_error = ch->coop_error;
+    return RT_OK;
+}
 
 int32_t rt_chain_tile_configs(rt_chain* ch) { return ch ? rt_tile_config_count(ch->model) : 0; }
 

@@ -123,7 +123,11 @@ bool persist_config_packed(int model, int idx);
 int persist_capacity(int model, int idx, bool uni, int device);
 bool persist_uses_scratch(int model, int idx, bool uni);
 int persist_waves_per_cu(int model, int idx, bool uni, bool batch);
-hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st);
+// coop: launch cooperatively, so that the runtime refuses a grid whose workgroups cannot all
+// be resident at once (hipErrorCooperativeLaunchTooLarge) instead of queuing tiles that would
+// wait on tiles that never start.
+hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st,
+                          bool coop = false);
 hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t st);
 
 }  // namespace rt

@@ -521,11 +521,13 @@ bool persist_uses_scratch(int model, int idx, bool uni) {
     return fa.localSizeBytes > 0;
 }
 
-hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st) {
+hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st,
+                          bool coop) {
     const void* k = persist_kernel(model, idx, uni, a.batch != nullptr);
     int tr, tc, h, nt;
     if (!k || !persist_config(model, idx, &tr, &tc, &h, &nt)) return hipErrorInvalidValue;
     void* args[] = {(void*)&a};
+    if (coop) return hipLaunchCooperativeKernel(k, dim3(nblocks), dim3(nt), args, 0, st);
     return hipLaunchKernel(k, dim3(nblocks), dim3(nt), args, 0, st);
 }
 

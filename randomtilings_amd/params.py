@@ -44,15 +44,8 @@ def write_tinymtdc(path: str, table: np.ndarray) -> None:
 
 
 def read_tinymtdc(path: str, n: int) -> np.ndarray:
-    out = []
-    with open(path) as f:
-        for line in f:
-            if line.startswith("#"):
-                continue
-            if len(out) == n:
-                break
-            fields = line.rstrip("\n").split(",")
-            out.append([int(fields[3], 16), int(fields[4], 16), int(fields[5], 16)])
-    if len(out) < n:
-        raise ValueError(f"{path}: {len(out)} entries, {n} requested")
-    return np.asarray(out, dtype=np.uint32)
+    """The first n (mat1, mat2, tmat) entries of a tinymtdc file, read by the library's own
+    reader (rt_parse_params_file, the one LoadTinyMT uses)."""
+    from . import parse_params_file
+
+    return parse_params_file(path, n)
