@@ -159,6 +159,19 @@ int rt_domino_walk_epochs(rt_ctx* ctx, int rule, const double* rparams, int32_t*
  * q is used by RT_LOZENGE_QREF / RT_LOZENGE_QHEAT (qlozengekernel.cl:3 hard-codes 0.99). */
 int rt_lozenge_walk(rt_ctx* ctx, int rule, double q, int32_t* tiling, int32_t N, int64_t steps, int64_t seed);
 
+/* rt_lozenge_walk that allocates nothing: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, semantics and result as rt_lozenge_walk; the device
+ * memory it needs (two state copies, two generator copies, exchange flags, error word, colour
+ * schedule) lies in a file-scope array of 64 MiB that each device the library runs on holds
+ * until the process ends, its address taken once per device by rt_create.  Limit: the layout
+ * must fit that array (about 12.7 * N^2 + 1.6 MiB bytes: N <= 2271, e.g. the hexagon
+ * a = b = c = 1000 at N = 2007); a larger N is RT_ERR_ARG.  Tile skipping is off.  Calls on
+ * one device share the array (each clears what must start at zero): calls that overlap on one
+ * device race.  The parameter table must have been loaded beforehand (rt_load_params). */
+int rt_lozenge_walk_noalloc(rt_ctx* ctx, int rule, double q, int32_t* tiling, int32_t N, int64_t steps,
+                            int64_t seed);
+
 /* RectTriangleTiler::Walk(tiling&, int steps, long seed) — RectTriangleTiler.cpp:17-92.
  * weights16: NULL for the reference's compile-time weights (recttrianglekernel.cl:3-13,
  * all 1.0), or the 16-entry per-digit weight table. */

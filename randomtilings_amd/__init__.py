@@ -54,6 +54,7 @@ SIGNATURES = {
     "rt_domino_walk": (C.c_int, [_P, C.c_int, _DP, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_domino_walk_epochs": (C.c_int, [_P, C.c_int, _DP, _I32P, C.c_int32, _I64P, _I64P, C.c_int32]),
     "rt_lozenge_walk": (C.c_int, [_P, C.c_int, C.c_double, _I32P, C.c_int32, C.c_int64, C.c_int64]),
+    "rt_lozenge_walk_noalloc": (C.c_int, [_P, C.c_int, C.c_double, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_recttri_walk": (C.c_int, [_P, _FP, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_tridimer_walk": (C.c_int, [_P, _I32P, C.c_int32, C.c_int64, C.c_int64]),
     "rt_extremal_tiling": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, C.c_int, _I32P, _I32P]),
@@ -277,6 +278,14 @@ class Context:
         t = np.ascontiguousarray(t, dtype=np.int32).copy()
         N = int(round(np.sqrt(t.size)))
         _check(self._lib.rt_lozenge_walk(self.h, rule, q, _i32(t), N, steps, seed))
+        return t
+
+    def lozenge_walk_noalloc(self, t: np.ndarray, steps: int, seed: int, rule: int = RT_LOZENGE_UNIFORM,
+                             q: float = 0.99) -> np.ndarray:
+        """lozenge_walk through rt_lozenge_walk_noalloc (no device allocation; N <= 2271)."""
+        t = np.ascontiguousarray(t, dtype=np.int32).copy()
+        N = int(round(np.sqrt(t.size)))
+        _check(self._lib.rt_lozenge_walk_noalloc(self.h, rule, q, _i32(t), N, steps, seed))
         return t
 
     def recttri_walk(self, t: np.ndarray, steps: int, seed: int, weights16=None) -> np.ndarray:

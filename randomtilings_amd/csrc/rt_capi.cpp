@@ -114,6 +114,9 @@ struct rt_chain {
     int32_t last_tiles = 0, last_running = 0;
     int64_t coop_refused = 0;  // resident launches the runtime refused (continued blocked)
     int coop_error = 0;        // the hipError_t of the last refusal
+    // Device memory borrowed from the caller's scratch (rt_lozenge_walk_noalloc): nothing of it
+    // is allocated or freed here, and there are no events.
+    bool borrowed = false;
 };
 
 // The tiles a persistent run of configuration cfg skips, or nullptr when every tile runs.
@@ -244,6 +247,9 @@ static int td_run(rt_chain* ch, int64_t steps) {
     HIPCHK(hipEventRecord(ch->ev0, c->stream));
     if (small) {
         if (ch->dsched_cap < steps) {
+            if (ch->borrowed)
+                return fail(RT_ERR_ARG, "rt_chain_run: %lld steps exceed the schedule buffer (%lld)", (long long)steps,
+                            (long long)ch->dsched_cap);
             if (ch->dsched) {
                 HIPCHK(hipStreamSynchronize(c->stream));
                 HIPCHK(hipFree(ch->dsched));
@@ -300,8 +306,8 @@ static int td_run(rt_chain* ch, int64_t steps) {
         }
         ch->last_cfg = cfg;
     }
-    HIPCHK(hipEventRecord(ch->ev1, c->stream));
-    ch->timed = true;
+    if (ch->ev1) HIPCHK(hipEventRecord(ch->ev1, c->stream));
+    ch->timed = ch->ev1 != nullptr;
     return RT_OK;
 }
 
@@ -406,6 +412,7 @@ int rt_create(int device, rt_ctx** out) {
     HIPCHK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(RT_ERR_ARG, "rt_create: device %d of %d", device, n);
     HIPCHK(hipSetDevice(device));
+    noalloc_setup(device);  // the allocation-free entry point's scratch address on this device
     rt_ctx* c = new rt_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -987,7 +994,8 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
     rt_ctx* c = ch->ctx;
     HIPCHK(hipSetDevice(c->device));
     if (ch->sched_pending) {  // the previous upload may still be reading hsched
-        HIPCHK(hipEventSynchronize(ch->ev_sched));
+        if (ch->ev_sched) HIPCHK(hipEventSynchronize(ch->ev_sched));
+        else HIPCHK(hipStreamSynchronize(c->stream));
         ch->sched_pending = false;
     }
     ch->hsched.resize((size_t)std::max<int64_t>(steps, 1));
@@ -1009,7 +1017,7 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         return fail(RT_ERR_ARG, "no persistent tile configuration fits this chain on the device");
 
     if (cfg < 0) {
-        HIPCHK(hipEventRecord(ch->ev0, c->stream));
+        if (ch->ev0) HIPCHK(hipEventRecord(ch->ev0, c->stream));
         StepwiseArgs a{};
         a.rp = ch->rp;
         a.cls = ch->cls[ch->cur];
@@ -1028,6 +1036,9 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         ch->last_cfg = -1;
     } else {
         if (ch->dsched_cap < steps) {
+            if (ch->borrowed)
+                return fail(RT_ERR_ARG, "rt_chain_run: %lld steps exceed the schedule buffer (%lld)", (long long)steps,
+                            (long long)ch->dsched_cap);
             if (ch->dsched) {
                 HIPCHK(hipStreamSynchronize(c->stream));
                 HIPCHK(hipFree(ch->dsched));
@@ -1038,7 +1049,7 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
             ch->dsched_cap = steps;
         }
         HIPCHK(hipMemcpyAsync(ch->dsched, ch->hsched.data(), (size_t)steps, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(hipEventRecord(ch->ev_sched, c->stream));
+        if (ch->ev_sched) HIPCHK(hipEventRecord(ch->ev_sched, c->stream));
         ch->sched_pending = true;
         int tr, tc, h, nt;
         persist_config(ch->model, cfg, &tr, &tc, &h, &nt);
@@ -1046,7 +1057,7 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         if (runs) HIPCHK(upload_runs(*runs));
         ch->last_tiles = ((ch->N + tr - 1) / tr) * tiles_c;
         ch->last_running = nblocks;
-        HIPCHK(hipEventRecord(ch->ev0, c->stream));
+        if (ch->ev0) HIPCHK(hipEventRecord(ch->ev0, c->stream));
         // one launch per chunk keeps single kernels short; chunk is a multiple of h.  A grid
         // that is not resident runs one block of h steps per launch.  Resident grids launch
         // cooperatively: the runtime then guarantees that every workgroup is resident at once
@@ -1102,8 +1113,8 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
         }
         ch->last_cfg = cfg;
     }
-    HIPCHK(hipEventRecord(ch->ev1, c->stream));
-    ch->timed = true;
+    if (ch->ev1) HIPCHK(hipEventRecord(ch->ev1, c->stream));
+    ch->timed = ch->ev1 != nullptr;
     return RT_OK;
 }
 
@@ -1795,6 +1806,79 @@ int rt_domino_walk_epochs(rt_ctx* ctx, int rule, const double* rparams, int32_t*
 
 int rt_lozenge_walk(rt_ctx* ctx, int rule, double q, int32_t* tiling, int32_t N, int64_t steps, int64_t seed) {
     return walk(ctx, RT_LOZENGE, rule, &q, 1, tiling, N, &steps, &seed, 1, RT_SEED_WALK);
+}
+
+// rt_lozenge_walk without allocating: the chain's state copies, generators, exchange flags,
+// error word and colour schedule are carved out of this device's file-scope scratch array
+// (rt_noalloc.hip), zeroed where they must start at zero by a fill kernel, and the chain runs
+// on the same kernels as rt_lozenge_walk.  Tile skipping is off (it would upload per-chain
+// tile lists); the result does not depend on it.  Calls on one device share the array:
+// overlapping calls race.
+int rt_lozenge_walk_noalloc(rt_ctx* c, int rule, double q, int32_t* tiling, int32_t N, int64_t steps,
+                            int64_t seed) {
+    if (!c || !tiling) return fail(RT_ERR_ARG, "rt_lozenge_walk_noalloc: null argument");
+    if (steps < 0) return fail(RT_ERR_ARG, "rt_lozenge_walk_noalloc: negative steps");
+    if (N < 6 || N % 3) return fail(RT_ERR_ARG, "tiling side %d must be a multiple of 3 and >= 6", N);
+    for (int64_t i = 0; i < (int64_t)N * N; i++)
+        if (tiling[i] < 0 || tiling[i] > 255)
+            return fail(RT_ERR_ARG, "lozenge vertex state %d out of the 8-bit range", tiling[i]);
+    std::string why;
+    uint8_t* base = (uint8_t*)noalloc_scratch(c->device, why);
+    if (!base) return fail(RT_ERR_HIP, "rt_lozenge_walk_noalloc: no scratch array: %s", why.c_str());
+    rt_chain ch;
+    ch.ctx = c;
+    ch.model = RT_LOZENGE;
+    ch.rule = rule;
+    ch.N = N;
+    ch.NC = 3;
+    ch.W = N / 3;
+    ch.esize = 1;
+    ch.G = (int64_t)N * ch.W;
+    ch.P = (ch.W + 15) / 16 * 16;
+    ch.path = c->default_path;
+    ch.cfg = c->default_cfg;
+    ch.skip = false;
+    ch.borrowed = true;
+    if (!lozenge_rule(rule, q, ch.rp.fr)) return fail(RT_ERR_ARG, "invalid lozenge rule %d or q", rule);
+    // layout: error word, exchange flags, schedule, two state copies, two generator copies
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    constexpr int64_t kSched = (int64_t)1 << 20;  // schedule bytes per run; longer walks run in pieces
+    const size_t cls_bytes = (size_t)N * ch.P * 3, gen_bytes = sizeof(uint4) * (size_t)ch.G;
+    size_t off = 0;
+    const size_t o_misc = off;
+    off += up(64);
+    const size_t o_bar = off;
+    off += up(kBarBytes);
+    const size_t o_sched = off;
+    off += up((size_t)kSched + 16);
+    size_t o_cls[2], o_gen[2];
+    for (int b = 0; b < 2; b++) o_cls[b] = off, off += up(cls_bytes);
+    for (int b = 0; b < 2; b++) o_gen[b] = off, off += up(gen_bytes);
+    if (off > kNoAllocScratchBytes)
+        return fail(RT_ERR_ARG,
+                    "rt_lozenge_walk_noalloc: side %d needs %zu bytes of device scratch, the array holds %zu "
+                    "(rt_lozenge_walk has no limit)",
+                    N, off, kNoAllocScratchBytes);
+    ch.misc = (uint32_t*)(base + o_misc);
+    ch.bar = (uint32_t*)(base + o_bar);
+    ch.dsched = base + o_sched;
+    ch.dsched_cap = kSched;
+    for (int b = 0; b < 2; b++) {
+        ch.cls[b] = base + o_cls[b];
+        ch.gens[b] = (uint4*)(base + o_gen[b]);
+    }
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<uint8_t> h;
+    split(&ch, tiling, h);
+    for (int b = 0; b < 2; b++) HIPCHK(hipMemcpyAsync(ch.cls[b], h.data(), cls_bytes, hipMemcpyHostToDevice, c->stream));
+    // the previous call's flags, epochs and error word are still in the array
+    HIPCHK(launch_fill_u32(ch.misc, 16, 0u, c->stream));
+    HIPCHK(launch_fill_u32(ch.bar, (int)(kBarBytes / sizeof(uint32_t)), 0u, c->stream));
+    int rc = rt_chain_seed(&ch, RT_SEED_WALK, seed);
+    for (int64_t s0 = 0; s0 < steps && rc == RT_OK; s0 += kSched) rc = rt_chain_run(&ch, std::min(kSched, steps - s0));
+    if (rc == RT_OK) rc = rt_chain_read(&ch, tiling);  // synchronizes; reports the device error word
+    if (rc != RT_OK) (void)hipStreamSynchronize(c->stream);  // leave nothing queued on the array
+    return rc;
 }
 
 int rt_tridimer_walk(rt_ctx* ctx, int32_t* tiling, int32_t N, int64_t steps, int64_t seed) {

@@ -130,4 +130,11 @@ hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, in
                           bool coop = false);
 hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t st);
 
+// The allocation-free entry point's device memory (rt_noalloc.hip): a file-scope array of
+// kNoAllocScratchBytes per device.  noalloc_setup takes its device address once per device
+// (with that device current); noalloc_scratch returns it, or nullptr with the reason.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+void noalloc_setup(int device);
+void* noalloc_scratch(int device, std::string& why);
+
 }  // namespace rt
