@@ -139,33 +139,60 @@ def main() -> int:
     bytes_per_launch = 2.0 * state_bytes * steps_per_launch
     achieved = bytes_per_launch / launch_s / 1e9
 
-    traffic = None
+    # HBM traffic is measured by rocprofv3 (FETCH_SIZE / WRITE_SIZE) in a separate profiling run
+    # of this same workload; it is attached only when this run used the profiled kernel
+    # configuration on the profiled hexagon, else null.
+    cfg_used = list(ch.tile_configs()[ch.last_config]) if ch.last_config >= 0 else None
+    traffic, traffic_src = None, None
     tf = ROOT / "profiles" / "traffic_lozenge_hex1000.json"
     if tf.exists():
         try:
             tj = json.loads(tf.read_text())
-            traffic = tj.get("bytes_per_chain_step")
-            if traffic is not None:
-                traffic = float(traffic) * steps_per_launch
+            if tj.get("hexagon") == args.hex and tj.get("tile_config") == cfg_used:
+                traffic = float(tj["bytes_per_chain_step"]) * steps_per_launch
+                traffic_src = f"rocprofv3 FETCH_SIZE/WRITE_SIZE run of this workload and kernel ({tf.name})"
         except Exception:
             traffic = None
+
+    # Executed work: the reference rotates every work-item of the N x N/3 class arrays; the
+    # packed sweep does not run tiles whose cells can never change (outside the hexagon).
+    executed = attempts
+    if cfg_used is not None and rt.tile_config_packed(rt.RT_LOZENGE, ch.last_config):
+        tr, tc, th = cfg_used[:3]
+        m, ok = rt.lozenge_skip_map(t0, tr, tc, th)
+        if ok and ch.last_grid[1] < ch.last_grid[0]:
+            W = N // 3
+            executed = 0
+            for (ti, tj_), run in np.ndenumerate(m):
+                if run:
+                    executed += (min(N, (ti + 1) * tr) - ti * tr) * (min(W, (tj_ + 1) * tc) - tj_ * tc)
+    exec_frac = executed / attempts
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         import oracle
 
-        cpu_s = oracle.lozenge_time(t0, args.cpu_steps, args.seed, params, rule=rt.RT_LOZENGE_QREF, q=args.q)
+        # all cores the process may use (the oracle's per-kernel loops over a colour class run on
+        # threads; the result does not depend on it), and one core
+        cores = oracle.default_threads()
+        oracle.set_threads(cores)
+        cpu_all = oracle.lozenge_time(t0, 4 * args.cpu_steps, args.seed, params, rule=rt.RT_LOZENGE_QREF, q=args.q)
+        oracle.set_threads(1)
+        cpu_one = oracle.lozenge_time(t0, args.cpu_steps, args.seed, params, rule=rt.RT_LOZENGE_QREF, q=args.q)
+        oracle.set_threads(cores)
         # parity on the same sample: the GPU walk of the same steps must match the oracle bit for bit
         g = ctx.lozenge_walk(t0, args.cpu_steps, args.seed, rule=rt.RT_LOZENGE_QREF, q=args.q)
         o = oracle.lozenge_walk(t0, args.cpu_steps, args.seed, params, rule=rt.RT_LOZENGE_QREF, q=args.q)
         cpu = {
-            "value": args.cpu_steps * attempts / cpu_s,
+            "value": 4 * args.cpu_steps * attempts / cpu_all,
             "unit": "flip attempts/s",
-            "cores": 1,
+            "cores": cores,
             "kind": "port",
-            "sample": f"oracle (sequential C++ restatement) LozengeTiler Walk of {args.cpu_steps} steps on the same "
-                      f"N={N} hexagon, q-rule q={args.q}, host split/merge included; GPU result bit-identical: "
-                      f"{bool(np.array_equal(g, o))}",
+            "single_core_value": args.cpu_steps * attempts / cpu_one,
+            "sample": f"oracle (sequential C++ restatement, per-kernel loops on {cores} threads) LozengeTiler Walk "
+                      f"of {4 * args.cpu_steps} steps on the same N={N} hexagon, q-rule q={args.q}, host split/merge "
+                      f"included; single_core_value: {args.cpu_steps} steps on one thread; GPU result of "
+                      f"{args.cpu_steps} steps bit-identical: {bool(np.array_equal(g, o))}",
         }
 
     line = {
@@ -196,6 +223,9 @@ def main() -> int:
             "tiles_run": ch.last_grid[1],
             "in_domain_sites_per_chain_step": in_domain,
             "in_domain_attempts_per_s": in_domain * args.steps * S / wall,
+            # the work the GPU executed: positions of the tiles that ran
+            "executed_attempts_per_chain_step": executed,
+            "executed_attempts_per_s": executed * args.steps * S / wall,
         },
         "roofline": {
             "bound": "hbm",
@@ -204,6 +234,15 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            # achieved counts algorithmic bytes: a read and a write of the whole packed state
+            # per chain step (every Rotate work-item, as the reference counts them);
+            # frac_executed counts only the state of the tiles that ran
+            "basis": "algorithmic bytes, 2 x packed state per chain step",
+            "frac_executed": achieved * exec_frac / HBM_PEAK_GBS,
+            "traffic_source": traffic_src,
+            # what the profile of this kernel found binding (the state is LDS-resident; HBM sees
+            # only the edge-band exchange): VALU issue and LDS/barrier latency
+            "limiter": "valu issue + latency (SQ counters, profiles/lozenge_hex1000_qref_persist.md)",
         },
         "cpu_baseline": cpu,
         "gpu_kernel_ms_per_chain_step": gpu_ms / (args.steps * S),

@@ -81,6 +81,7 @@ typedef struct rt_ctx rt_ctx;
 typedef struct rt_chain rt_chain;
 typedef struct rt_batch rt_batch;
 typedef struct rt_unet rt_unet;
+typedef struct rt_observer rt_observer;
 
 /* Thread-local description of the last error on this thread. */
 const char* rt_last_error(void);
@@ -296,6 +297,25 @@ int rt_chain_set_skip(rt_chain* ch, int enable);
 int rt_chain_last_grid(rt_chain* ch, int32_t* tiles, int32_t* running);
 int rt_lozenge_skip_map(const int32_t* tiling, int32_t N, int32_t tr, int32_t tc, int32_t h, uint8_t* map,
                         int32_t* running, int32_t* consistent);
+
+/* ---- observables on the device (no counterpart in the reference) ---------------------------
+ * Height-function statistics of domino chains computed where the chains live, without reading
+ * the tilings back.  The heights are DominoTiler::TilingToHeightfunc's (src/Domino/
+ * DominoTiler.cpp:423-461: 0 at the first boundary vertex in row-major order, +-1 along
+ * uncrossed edges) for the domain's tilings.  An observer holds, for one domain, its M x M
+ * squares (`domain`), its minimal height function `hmin` ((M+1) x (M+1), INT32_MAX outside,
+ * e.g. from rt_extremal_tiling: only its boundary heights are used, which every tiling shares),
+ * the vertices the statistics cover (`mask`, NULL: every vertex of the domain) and an optional
+ * reference field `ref256` (256 x a height field, e.g. a long-run mean, rounded; NULL: none).
+ * rt_observe computes, for each of n domino chains of side M+1 on the observer's context, in
+ * one launch (a segmented row scan per row, wave64 reductions): out[4k+0] = masked vertices,
+ * out[4k+1] = sum of their heights, out[4k+2] = sum of squares, out[4k+3] = sum of
+ * |256 h - ref256| (0 without a reference), all exact integers; with heights != NULL also
+ * every chain's height function (n x (M+1)^2, INT32_MAX outside).  Synchronizes. */
+int rt_observer_create(rt_ctx* ctx, const int32_t* domain, int32_t M, const int32_t* hmin, const uint8_t* mask,
+                       const int32_t* ref256, rt_observer** out);
+int rt_observer_destroy(rt_observer* obs);
+int rt_observe(rt_observer* obs, rt_chain* const* chains, int32_t n, int64_t* out, int32_t* heights);
 
 /* ---- batches of independent chains (no counterpart in the reference: its callers loop
  * over tilers) ---------------------------------------------------------------------------

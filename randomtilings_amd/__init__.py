@@ -96,6 +96,9 @@ SIGNATURES = {
     "rt_nn_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _FP, _FP, _FP]),
     "rt_nn_gemm_ex": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, _FP, _FP, _FP]),
     "rt_nn_gemm_time": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "rt_observer_create": (C.c_int, [_P, _I32P, C.c_int32, _I32P, C.POINTER(C.c_uint8), _I32P, C.POINTER(_P)]),
+    "rt_observer_destroy": (C.c_int, [_P]),
+    "rt_observe": (C.c_int, [_P, C.POINTER(_P), C.c_int32, _I64P, _I32P]),
     "rt_batch_create": (C.c_int, [_P, C.POINTER(_P)]),
     "rt_batch_destroy": (C.c_int, [_P]),
     "rt_batch_add": (C.c_int, [_P, _P]),
@@ -436,6 +439,47 @@ class Chain:
             _check(self._lib.rt_chain_tile_config_info(self.h, i, *[C.byref(x) for x in v]))
             out.append(tuple(x.value for x in v))
         return out
+
+
+class Observer:
+    """Height-function statistics of domino chains on one domain, computed on the device
+    (rt_observer_*, rt_observe)."""
+
+    def __init__(self, ctx: Context, domain: np.ndarray, hmin: np.ndarray, mask=None, ref256=None):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        d = np.ascontiguousarray(domain, dtype=np.int32)
+        M = int(round(np.sqrt(d.size)))
+        self.N = M + 1
+        hm = np.ascontiguousarray(hmin, dtype=np.int32)
+        mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        rf = None if ref256 is None else np.ascontiguousarray(ref256, dtype=np.int32)
+        h = _P()
+        _check(self._lib.rt_observer_create(ctx.h, _i32(d), M, _i32(hm),
+                                            None if mk is None else mk.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                            None if rf is None else _i32(rf), C.byref(h)))
+        self.h = h
+
+    def observe(self, chains, heights: bool = False):
+        """(n x 4 int64 array of count, sum h, sum h^2, sum |256 h - ref256| per chain,
+        n x N^2 heights or None)."""
+        n = len(chains)
+        arr = (_P * n)(*[ch.h for ch in chains])
+        out = np.zeros((n, 4), np.int64)
+        hs = np.zeros((n, self.N * self.N), np.int32) if heights else None
+        _check(self._lib.rt_observe(self.h, arr, n, out.ctypes.data_as(_I64P), None if hs is None else _i32(hs)))
+        return out, hs
+
+    def close(self) -> None:
+        if self.h:
+            self._lib.rt_observer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

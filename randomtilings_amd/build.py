@@ -34,7 +34,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-I", str(ROOT / "include"), "-I", str(SRC)]
 
 HIP_UNITS = ["rt_kernels.hip", "rt_tridimer.hip", "rt_heights.hip", "rt_mfma.hip", "rt_nnops.hip", "rt_capi.cpp",
-             "rt_rules.cpp", "rt_unet.cpp", "rt_skip.cpp", "rt_noalloc.hip"]
+             "rt_rules.cpp", "rt_unet.cpp", "rt_skip.cpp", "rt_noalloc.hip", "rt_observe.hip"]
 CXX_UNITS = [
     "common/common.cpp", "TinyMT/file_reader.cpp", "Domino/DominoTiler.cpp",
     "Lozenge/LozengeTiler.cpp", "RectTriangle/RectTriangleTiler.cpp",

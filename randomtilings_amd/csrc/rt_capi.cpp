@@ -1743,6 +1743,106 @@ int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* 
     return rc;
 }
 
+// ---- observables on the device -----------------------------------------------------------
+
+struct rt_observer {
+    rt_ctx* ctx = nullptr;
+    int N = 0;
+    uint8_t* faces = nullptr;  // M x M squares of the domain
+    int32_t* hmin = nullptr;   // N x N, INT32_MAX outside
+    uint8_t* mask = nullptr;   // N x N vertices the statistics cover
+    int32_t* ref = nullptr;    // N x N reference field x 256, or null
+    const uint8_t** dchains = nullptr;
+    unsigned long long* dout = nullptr;
+    int32_t* dheights = nullptr;
+    int cap = 0, hcap = 0;
+};
+
+extern "C" {
+
+int rt_observer_create(rt_ctx* ctx, const int32_t* domain, int32_t M, const int32_t* hmin, const uint8_t* mask,
+                       const int32_t* ref256, rt_observer** out) {
+    if (!ctx || !domain || !hmin || !out || M < 2) return fail(RT_ERR_ARG, "rt_observer_create: bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int N = M + 1;
+    const size_t NN = (size_t)N * N, MM = (size_t)M * M;
+    std::vector<uint8_t> f(MM), m(NN);
+    for (size_t k = 0; k < MM; k++) f[k] = domain[k] != 0;
+    for (size_t k = 0; k < NN; k++) m[k] = mask ? (mask[k] != 0) : (hmin[k] != INT32_MAX);
+    rt_observer* o = new rt_observer();
+    o->ctx = ctx;
+    o->N = N;
+    hipError_t e = hipMalloc(&o->faces, MM);
+    if (e == hipSuccess) e = hipMalloc(&o->hmin, NN * 4);
+    if (e == hipSuccess) e = hipMalloc(&o->mask, NN);
+    if (e == hipSuccess && ref256) e = hipMalloc(&o->ref, NN * 4);
+    if (e == hipSuccess) e = hipMemcpy(o->faces, f.data(), MM, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(o->hmin, hmin, NN * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(o->mask, m.data(), NN, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ref256) e = hipMemcpy(o->ref, ref256, NN * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        rt_observer_destroy(o);
+        return fail(RT_ERR_HIP, "rt_observer_create: %s", hipGetErrorString(e));
+    }
+    *out = o;
+    return RT_OK;
+}
+
+int rt_observer_destroy(rt_observer* o) {
+    if (!o) return RT_OK;
+    hipSetDevice(o->ctx->device);
+    hipStreamSynchronize(o->ctx->stream);
+    for (void* p : {(void*)o->faces, (void*)o->hmin, (void*)o->mask, (void*)o->ref, (void*)o->dchains,
+                    (void*)o->dout, (void*)o->dheights})
+        if (p) hipFree(p);
+    delete o;
+    return RT_OK;
+}
+
+int rt_observe(rt_observer* o, rt_chain* const* chains, int32_t n, int64_t* out, int32_t* heights) {
+    if (!o || !chains || n <= 0 || !out) return fail(RT_ERR_ARG, "rt_observe: bad arguments");
+    rt_ctx* c = o->ctx;
+    std::vector<const uint8_t*> ptrs((size_t)n);
+    int P = -1;
+    for (int k = 0; k < n; k++) {
+        const rt_chain* ch = chains[k];
+        if (!ch || ch->model != RT_DOMINO || ch->N != o->N || ch->ctx != c)
+            return fail(RT_ERR_ARG, "rt_observe: chain %d is not a domino chain of side %d on this context", k, o->N);
+        if (P >= 0 && ch->P != P) return fail(RT_ERR_ARG, "rt_observe: chains of different pitch");
+        P = ch->P;
+        ptrs[k] = (const uint8_t*)ch->cls[ch->cur];
+    }
+    HIPCHK(hipSetDevice(c->device));
+    if (o->cap < n) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (o->dchains) HIPCHK(hipFree(o->dchains));
+        if (o->dout) HIPCHK(hipFree(o->dout));
+        o->dchains = nullptr;
+        o->dout = nullptr;
+        HIPCHK(hipMalloc(&o->dchains, sizeof(void*) * n));
+        HIPCHK(hipMalloc(&o->dout, sizeof(unsigned long long) * 4 * n));
+        o->cap = n;
+    }
+    const size_t NN = (size_t)o->N * o->N;
+    if (heights && o->hcap < n) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (o->dheights) HIPCHK(hipFree(o->dheights));
+        o->dheights = nullptr;
+        HIPCHK(hipMalloc(&o->dheights, NN * 4 * n));
+        o->hcap = n;
+    }
+    HIPCHK(hipMemcpyAsync(o->dchains, ptrs.data(), sizeof(void*) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_fill_u32((uint32_t*)o->dout, 8 * n, 0u, c->stream));
+    HIPCHK(launch_domino_observe(o->dchains, n, o->N, P, o->faces, o->hmin, o->mask, o->ref,
+                                 heights ? o->dheights : nullptr, o->dout, c->stream));
+    HIPCHK(hipMemcpyAsync(out, o->dout, sizeof(int64_t) * 4 * n, hipMemcpyDeviceToHost, c->stream));
+    if (heights) HIPCHK(hipMemcpyAsync(heights, o->dheights, NN * 4 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return RT_OK;
+}
+
+}  // extern "C"
+
 // ---- height functions on the device -----------------------------------------------------
 
 static int domain_of(int model, const int32_t* domain, int32_t M, std::vector<int>& d) {

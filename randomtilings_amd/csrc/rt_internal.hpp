@@ -130,6 +130,13 @@ hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, in
                           bool coop = false);
 hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t st);
 
+// Domino height observables (rt_observe.hip): chains points at nchains device pointers to
+// class arrays (two classes of N rows, pitch P bytes); out: 4 uint64 per chain (count, sum of
+// heights, sum of squares, sum of |256 h - ref256|) over the masked vertices, added to.
+hipError_t launch_domino_observe(const void* chains, int nchains, int N, int P, const uint8_t* faces,
+                                 const int32_t* hmin, const uint8_t* mask, const int32_t* ref256, int32_t* heights,
+                                 unsigned long long* out, hipStream_t st);
+
 // The allocation-free entry point's device memory (rt_noalloc.hip): a file-scope array of
 // kNoAllocScratchBytes per device.  noalloc_setup takes its device address once per device
 // (with that device current); noalloc_scratch returns it, or nullptr with the reason.
