@@ -53,6 +53,8 @@ def main() -> int:
                          "c4: coupling from the past on an Aztec diamond (--aztec); tridimer: triangle-dimer cone")
     ap.add_argument("--aztec", type=int, default=256, help="Aztec diamond order of the c4 workload")
     ap.add_argument("--p12", nargs=4, default=None, help="c4: weighted heat-bath probabilities (else p = 1/2)")
+    ap.add_argument("--pair-steps", type=int, default=0,
+                    help="c4: only time this many coupled steps at --aztec (no coalescence)")
     ap.add_argument("--placement-n", type=int, default=16, help="c4: Aztec order of the placement check")
     ap.add_argument("--placement-samples", type=int, default=200, help="c4: exact samples for the placement check")
     ap.add_argument("--cone", type=int, default=1000, help="cone side of the tridimer workload")
@@ -60,12 +62,23 @@ def main() -> int:
     ap.add_argument("--domains", type=int, default=64, help="c3: number of domains")
     ap.add_argument("--sides", type=int, nargs=2, default=[64, 512], help="c3: domain side range")
     ap.add_argument("--domain-steps", type=int, default=1000, help="c3: chain steps per domain")
+    ap.add_argument("--kinds", nargs="+", default=["aztec", "notched", "tridimer", "recttri"],
+                    help="c3: domain kinds of the mix (aztec, notched, tridimer, recttri)")
     ap.add_argument("--canvas", type=int, default=64, help="c5: largest vertex grid side (training canvas)")
     ap.add_argument("--train-domains", type=int, default=96, help="c5: training domains")
     ap.add_argument("--samples-per-domain", type=int, default=16, help="c5: chain samples averaged per domain")
     ap.add_argument("--eq-steps", type=int, default=30000, help="c5: chain steps per training sample")
     ap.add_argument("--epochs", type=int, default=200, help="c5: training epochs")
-    ap.add_argument("--horizon", type=int, default=40000, help="c5: chain steps of each evaluation trajectory")
+    ap.add_argument("--horizon", type=int, default=20000, help="c5: least chain steps of each evaluation trajectory")
+    ap.add_argument("--horizon-n2", type=float, default=16.0,
+                    help="c5: most chain steps of an evaluation trajectory in units of N^2 (it ends earlier once "
+                         "the chains from the minimal and maximal tilings agree, and runs as long again)")
+    ap.add_argument("--held", type=int, default=4, help="c5: held-out domains")
+    ap.add_argument("--train-max-side", type=int, default=0, help="c5: largest training tiling side (0: 2 x canvas)")
+    ap.add_argument("--held-big", type=int, default=0, help="c5: tiling side of one extra large held-out Aztec diamond")
+    ap.add_argument("--big-horizon-n2", type=float, default=3.0, help="c5: horizon cap of the large one, in N^2")
+    ap.add_argument("--held-sides", type=int, nargs=2, default=[96, 250], help="c5: held-out tiling side range")
+    ap.add_argument("--eval-chains", type=int, default=16, help="c5: chains per start in the evaluation")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -340,6 +353,38 @@ def bench_c4(args) -> int:
     tmax, tmin = statics.call("domino.MaxTiling", d), statics.call("domino.MinTiling", d)
     ctx = rt.Context(0)
     ctx.load_params(default_table(tmax.size // 2))
+    if args.pair_steps > 0:
+        # time per coupled step at this order (no coalescence): the coupled pair chain
+        # (RT_DOMINO_PAIR) on its autotuned configuration, one epoch of --pair-steps steps
+        ch = ctx.chain(rt.RT_DOMINO_PAIR, rule, tmax | (tmin << 8), p12)
+        cfg = ch.autotune(256)
+        ch.seed(args.seed, mode=rt.RT_SEED_EPOCH)
+        ch.run(max(1, args.pair_steps // 8))
+        ch.sync()
+        ctx.sync()
+        ctx.mark(0)
+        w0 = time.perf_counter()
+        ch.run(args.pair_steps)
+        ctx.mark(1)
+        ch.sync()
+        wall = time.perf_counter() - w0
+        gpu_ms = ctx.elapsed_ms(0, 1)
+        att = 2 * ch.attempts_per_step  # both chains
+        extra = {"aztec_order": n, "tiling_side": int(round(np.sqrt(tmax.size))),
+                 "rule": "dominokernelCFTP.cl (p = 1/2)" if p12 is None else f"weighted heat-bath p12={p12}",
+                 "coupled_steps": args.pair_steps, "us_per_coupled_step": gpu_ms * 1e3 / args.pair_steps,
+                 "tile_config": list(ch.tile_configs()[cfg]), "packed": rt.tile_config_packed(rt.RT_DOMINO, cfg),
+                 "flip_attempts_per_s_both_chains": att * args.pair_steps / (gpu_ms / 1e3),
+                 "state_bytes": ch.state_bytes}
+        args.steps, args.warmup = 1, 0
+        line = _line(f"microseconds per coupled step (CFTP pair, domino Aztec diamond n={n})",
+                     gpu_ms * 1e3 / args.pair_steps, "us", args, wall * 1e3, "u8x2",
+                     f"coupled max/min domino chains, Aztec diamond n={n}, {args.pair_steps} steps", extra,
+                     higher=False)
+        print(json.dumps(line), flush=True)
+        ch.close()
+        ctx.close()
+        return 0
     times, steps, restarts = [], [], []
     for k in range(args.warmup + args.steps):
         w0 = time.perf_counter()
@@ -456,7 +501,7 @@ def bench_c3(args) -> int:
 
     lo, hi = args.sides
     ctx = rt.Context(0)
-    specs = domains.mix(ctx, args.domains, (lo, hi), args.seed)
+    specs = domains.mix(ctx, args.domains, (lo, hi), args.seed, tuple(args.kinds))
     gmax = max(int(t.size) for _, _, t, _ in specs)
     ctx.load_params(default_table(gmax))
     steps = args.domain_steps
@@ -511,8 +556,9 @@ def bench_c3(args) -> int:
              "batch_graphs": b.last_graphs, "bit_identical_batch_vs_individual": bool(same)}
     line = _line("aggregate flip attempts/s (batched domains)", att / batch_s, "flip attempts/s", args,
                  batch_s * 1e3, "u8/i32/u16",
-                 f"{args.domains} domains, sides U[{lo},{hi}], Aztec/notched-rectangle dominoes, triangle-dimer cones, "
-                 f"rect-triangle hexagons, {steps} steps each", extra)
+                 f"{args.domains} domains, sides U[{lo},{hi}], kinds {'/'.join(args.kinds)} (Aztec diamonds, "
+                 f"notched domino rectangles, triangle-dimer hexagons, rect-triangle hexagons), {steps} steps each",
+                 extra)
     if not args.no_cpu:
         # the oracle on one domain of each kind (the largest), a bounded number of steps each
         import oracle
@@ -520,7 +566,7 @@ def bench_c3(args) -> int:
                 rt.RT_TRIDIMER: lambda t, n, sd, p: oracle.tridimer_walk(t, n, sd, p),
                 rt.RT_RECTTRI: lambda t, n, sd, p: oracle.recttri_walk(t, n, sd, p)}
         cpu_att, cpu_s, picked = 0, 0.0, []
-        for kind in ("aztec", "notched", "tridimer", "recttri"):
+        for kind in sorted({sp[3] for sp in specs}):
             k = max((i for i in range(len(specs)) if specs[i][3] == kind), key=lambda i: specs[i][2].size)
             m, _, t, _ = specs[k]
             cs = 20
@@ -541,20 +587,31 @@ def bench_c3(args) -> int:
 
 
 def bench_c5(args) -> int:
-    """Learned warm start: train the U-Net (4 levels, 32 base channels, bf16 matrix cores) on
-    the mean of --samples-per-domain chain-equilibrated height functions of random domino domains (Aztec diamonds and notched
-    squares fitting a --canvas vertex grid), then compare chain steps to equilibrium (mean
-    interior height within 1% of the max-min range of its long-run value) from the projected
-    prediction and from the minimal tiling on held-out domains."""
+    """Learned warm start on 2-periodic weighted domino chains.  Training: --train-domains
+    random domains (Aztec diamonds and notched squares fitting a --canvas vertex grid), each
+    with edge weights a, b ~ U[0.3, 1] (weighted heat-bath, p12 = 1/2, 1/2, a^2/(a^2+b^2),
+    b^2/(a^2+b^2)); the target of each is the mean height fraction of --samples-per-domain
+    chains run --eq-steps steps from the minimal tiling.  The U-Net (4 levels, 32 base channels,
+    bf16 matrix cores) is trained on the GPU.  Evaluation on --held held-out weighted domains
+    with tiling sides in --held-sides (predicted on features sampled down to the canvas):
+    chain steps to equilibrium from the minimal tiling (plain), from the midpoint
+    (h_min + h_max) / 2 projected and mean-matched exactly as the prediction is (midpoint), and
+    from the projected prediction (learned), on two observables: the mean interior height
+    fraction (which the projection is matched to) and the mean |h - h_bar| (h_bar the long-run
+    mean field; not targeted).  The line's value is plain / learned steps on the second."""
     import randomtilings_amd as rt
     from randomtilings_amd import warmstart as ws
     from randomtilings_amd.params import default_table
 
     rng = np.random.default_rng(args.seed)
     ctx = rt.Context(0)
-    ctx.load_params(default_table(args.canvas * args.canvas))
+    lo_side, hi_side = args.held_sides
+    ctx.load_params(default_table(max(2 * args.canvas, args.train_max_side, hi_side + 8, args.held_big + 8) ** 2))
     t0 = time.perf_counter()
-    doms = [ws.Domain(ctx, d) for _, d in ws.domain_bank(rng, args.train_domains, args.canvas)]
+    # training domains up to --train-max-side (default twice the canvas): the larger ones are
+    # seen sampled down, as the held-out domains are
+    tmax_side = args.train_max_side or 2 * args.canvas
+    doms = [ws.Domain(ctx, d, ws.periodic_weights(rng)) for _, d in ws.domain_bank(rng, args.train_domains, tmax_side)]
     samples = ws.equilibrium_samples(ctx, doms, args.eq_steps, args.samples_per_domain, args.seed)
     X, T, M = ws.build_dataset(ctx, doms, samples, args.canvas, average=True)
     data_s = time.perf_counter() - t0
@@ -562,22 +619,31 @@ def bench_c5(args) -> int:
     t0 = time.perf_counter()
     losses = ws.train(net, X, T, M, args.epochs, 16, 2e-3, rng)
     train_s = time.perf_counter() - t0
-    every = max(100, args.horizon // 800)  # at most ~800 readings of the ensemble per trajectory
+    hrng = np.random.default_rng(args.seed + 99)
+    held = [ws.Domain(ctx, d, ws.periodic_weights(hrng))
+            for _, d in ws.domain_bank(hrng, args.held, hi_side, min_side=lo_side)]
+    if args.held_big:  # one large Aztec diamond, with a shorter horizon cap
+        held.append(ws.Domain(ctx, ws.aztec(max(4, (args.held_big - 6) // 2)), ws.periodic_weights(hrng)))
     results = []
-    held = [ws.Domain(ctx, d) for _, d in ws.domain_bank(np.random.default_rng(args.seed + 99), 4, args.canvas)]
     for k, dom in enumerate(held):
-        print(f"# held-out domain {k + 1}/{len(held)}: N = {dom.N}", file=sys.stderr, flush=True)
+        n2 = args.big_horizon_n2 if (args.held_big and k == len(held) - 1) else args.horizon_n2
+        horizon = int(max(args.horizon, n2 * dom.N * dom.N))
+        every = max(100, horizon // 400)
+        horizon = (horizon // every) * every
+        print(f"# held-out domain {k + 1}/{len(held)}: N = {dom.N}, p12 = {dom.p12}, horizon {horizon}",
+              file=sys.stderr, flush=True)
         ws.warm_tiling(ctx, net, dom, args.canvas)  # warm-up of the inference path
         w0 = time.perf_counter()
         tw = ws.warm_tiling(ctx, net, dom, args.canvas)  # U-Net forward, projection, tiling read-off
         prep_s = time.perf_counter() - w0
-        plain = ws.trajectory(ctx, dom, dom.tmin, args.horizon, every, 77 + k)
-        warm = ws.trajectory(ctx, dom, tw, args.horizon, every, 77 + k)
-        eq = 0.5 * (plain[len(plain) // 2:].mean() + warm[len(warm) // 2:].mean())
-        sp = ws.steps_to_equilibrium(plain, eq, every)
-        sw = ws.steps_to_equilibrium(warm, eq, every)
+        tm = ws.midpoint_tiling(ctx, dom)
+        chains = args.eval_chains if dom.N <= 520 else max(4, args.eval_chains // 4)
+        w0 = time.perf_counter()
+        cmp = ws.compare_starts(ctx, dom, {"plain": dom.tmin, "midpoint": tm, "learned": tw}, horizon, every,
+                                77 + k, chains)
+        eval_s = time.perf_counter() - w0
         # wall clock of one chain on this domain: seconds per chain step
-        ch = ctx.chain(rt.RT_DOMINO, rt.RT_DOMINO_CFTP, dom.tmin)
+        ch = dom.chain(ctx, dom.tmin)
         ch.seed(5)
         ch.run(2000)
         ch.sync()
@@ -586,29 +652,41 @@ def bench_c5(args) -> int:
         ch.sync()
         step_s = (time.perf_counter() - w0) / 20000
         ch.close()
-        r = ws.predicted_fraction(net, dom, args.canvas)
-        inn = dom.inner
-        pred_obs = float(((dom.hmin[inn] + r[inn] * (dom.hmax[inn] - dom.hmin[inn])).mean() - dom.hmin[inn].mean()) /
-                         (dom.hmax[inn].mean() - dom.hmin[inn].mean()))
-        results.append({"N": dom.N, "plain_steps": sp, "warm_steps": sw, "eq_fraction": eq,
-                        "predicted_fraction": pred_obs,
-                        "warm_start_fraction": float(warm[0]), "plain_start_fraction": float(plain[0]),
-                        "seconds_per_chain_step": step_s, "warm_start_prep_seconds": prep_s,
-                        "plain_seconds_to_eq": sp * step_s, "warm_seconds_to_eq": prep_s + sw * step_s})
-    # a warm start already in the band reads 0 steps; the trajectory is sampled every `every`
-    # steps, so the warm count is floored at that resolution and the ratio is a lower bound
-    ratio = float(np.mean([r["plain_steps"] for r in results]) /
-                  max(float(every), np.mean([r["warm_steps"] for r in results])))
+        cmp.update({"N": dom.N, "p12": dom.p12, "horizon": horizon, "every": every, "chains_per_start": chains,
+                    "sampled_down_by": max(1, -(-dom.N // args.canvas)),
+                    "seconds_per_chain_step": step_s, "learned_prep_seconds": prep_s, "eval_seconds": eval_s})
+        print(f"# {json.dumps(cmp)}", file=sys.stderr, flush=True)
+        results.append(cmp)
+
+    # domains whose brackets never met within the horizon have no trustworthy equilibrium value:
+    # they are reported but left out of the ratios
+    scored = [r for r in results if r["settled"]] or results
+
+    def ratio(a, b, obs):
+        # a start already in the band reads 0 steps; readings are `every` steps apart, so the
+        # denominator is floored at that resolution and the ratio is a lower bound
+        num = np.mean([r[a][obs] for r in scored])
+        den = max(np.mean([r["every"] for r in scored]), np.mean([r[b][obs] for r in scored]))
+        return float(num / den)
+
+    summary = {f"{a}_over_{b}_{obs}": ratio(a, b, f"steps_{obs}")
+               for a in ("plain", "midpoint") for b in ("learned",) for obs in ("height", "deviation")}
+    summary["plain_over_midpoint_deviation"] = ratio("plain", "midpoint", "steps_deviation")
+    summary["scored_domains"] = len(scored)
+    wall = float(np.mean([r["plain"]["steps_deviation"] * r["seconds_per_chain_step"] for r in scored]) /
+                 np.mean([r["learned_prep_seconds"] + r["learned"]["steps_deviation"] * r["seconds_per_chain_step"]
+                          for r in scored]))
     extra = {"canvas": args.canvas, "train_domains": args.train_domains, "train_examples": int(len(X)),
-             "chain_samples_averaged_per_domain": args.samples_per_domain, "eq_steps_per_sample": args.eq_steps, "unet": "4 levels x 32 base channels, bf16 MFMA, fp32 accumulate",
-             "params": net.param_count, "epochs": args.epochs, "final_loss": float(np.mean(losses[-10:])),
-             "first_loss": float(losses[0]), "data_seconds": data_s, "train_seconds": train_s,
-             "held_out": results, "steps_plain_over_warm": ratio,
-             "wall_clock_plain_over_warm": float(np.mean([r["plain_seconds_to_eq"] for r in results]) /
-                                                 np.mean([r["warm_seconds_to_eq"] for r in results]))}
-    line = _line("sweeps-to-equilibrium, plain / warm (domino, learned warm start)", ratio, "x", args,
-                 train_s * 1e3, "bf16/fp32", "warm start from the projected U-Net prediction vs the minimal tiling",
-                 extra)
+             "weights": "2-periodic, a, b ~ U[0.3, 1], p12 = (1/2, 1/2, a^2/(a^2+b^2), b^2/(a^2+b^2))",
+             "chain_samples_averaged_per_domain": args.samples_per_domain, "eq_steps_per_sample": args.eq_steps,
+             "unet": "4 levels x 32 base channels, bf16 MFMA, fp32 accumulate", "params": net.param_count,
+             "epochs": args.epochs, "final_loss": float(np.mean(losses[-10:])), "first_loss": float(losses[0]),
+             "data_seconds": data_s, "train_seconds": train_s, "held_out": results, **summary,
+             "wall_clock_plain_over_learned_deviation": wall}
+    line = _line("sweeps-to-equilibrium, plain / learned warm start (domino, weighted, |h - h_bar| observable)",
+                 summary["plain_over_learned_deviation"], "x", args, train_s * 1e3, "bf16/fp32",
+                 f"warm start from the projected U-Net prediction vs the minimal tiling and the midpoint, held-out "
+                 f"sides {lo_side}-{hi_side}", extra)
     print(json.dumps(line), flush=True)
     net.close()
     ctx.close()

@@ -320,13 +320,16 @@ int rt_observe(rt_observer* obs, rt_chain* const* chains, int32_t n, int64_t* ou
 /* ---- batches of independent chains (no counterpart in the reference: its callers loop
  * over tilers) ---------------------------------------------------------------------------
  * A batch runs many seeded chains of one context together: the domino, lozenge and
- * rectangle-triangle chains of each model advance in lockstep, one blocked launch per
- * block of h steps covering the tiles of every chain of the model (a chain-index table
- * maps blocks to chains); with use_graph the launch of one block and its step-counter
- * update are captured once as a HIP graph and replayed.  Triangle-dimer chains advance
- * together, one tiled launch per step on 32 x 32 tiles (replayed the same way), or on the
- * configuration set on the largest of them (rt_chain_set_path; 3-5: two steps per launch).  After rt_batch_run(b, k)
- * every chain is exactly where rt_chain_run(ch, k) would have left it.  Chains stay owned by the caller and must outlive the batch. */
+ * rectangle-triangle chains of each model advance in lockstep, one blocked launch per block of
+ * h steps covering the tiles of every chain of the model (a chain-index table maps blocks to
+ * chains); triangle-dimer chains advance together, one tiled launch per step on 32 x 32 tiles,
+ * or on the configuration set on the largest of them (rt_chain_set_path; 3-5: two steps per
+ * launch).  With use_graph, one round of 32 steps of every model (its launches at their offsets
+ * from a device step counter, then one counter update) is captured as a HIP graph and replayed;
+ * the graph is kept by the batch and replayed by later runs as long as its chains, their tile
+ * configurations and the parameter table are unchanged (rt_batch_last_graphs is then 0).
+ * After rt_batch_run(b, k) every chain is exactly where rt_chain_run(ch, k) would have left it.
+ * Chains stay owned by the caller and must outlive the batch. */
 int rt_batch_create(rt_ctx* ctx, rt_batch** out);
 int rt_batch_destroy(rt_batch* b);
 int rt_batch_add(rt_batch* b, rt_chain* ch);
@@ -334,7 +337,8 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph);
 int rt_batch_sync(rt_batch* b);
 /* GPU milliseconds of the most recent rt_batch_run (HIP events on the context stream). */
 int rt_batch_elapsed_ms(rt_batch* b, double* ms);
-/* Kernel launches (graph replays included) and graphs instantiated by the last run. */
+/* Kernel launches (those of graph replays included) and graphs instantiated by the last run
+ * (0 when it replayed the graph kept from an earlier run). */
 int64_t rt_batch_last_launches(rt_batch* b);
 int32_t rt_batch_last_graphs(rt_batch* b);
 

@@ -1272,7 +1272,11 @@ struct rt_batch {
     std::vector<BatchEntry> hent;
     std::vector<rt_chain*> hent_chain;  // the chain of each entry
     int64_t launches = 0;
-    int graphs = 0;
+    int graphs = 0;          // graphs instantiated by the last run (0: the kept one was replayed)
+    int64_t replays = 0;     // graph replays of the last run
+    hipGraphExec_t exec = nullptr;  // one round of every group, kept across runs
+    std::vector<int64_t> graph_key;  // what the kept graph was captured for
+    int64_t graph_launches = 0;      // kernel launches in one replay
     bool timed = false;
 };
 
@@ -1303,6 +1307,7 @@ int rt_batch_destroy(rt_batch* b) {
     if (b->dent) hipFree(b->dent);
     if (b->dtd) hipFree(b->dtd);
     if (b->ctr) hipFree(b->ctr);
+    if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
     if (b->ev_up) hipEventDestroy(b->ev_up);
@@ -1319,9 +1324,10 @@ int rt_batch_add(rt_batch* b, rt_chain* ch) {
     return RT_OK;
 }
 
-// Launches captured per graph round of a batched run (each at its own step offset), so one
-// step-counter update serves that many blocks.
-constexpr int64_t kRoundBlocks = 8;
+// Steps per captured round of a batched run: a multiple of every planar block height (4, 8) and
+// of the triangle-dimer steps per launch (1, 2), so every group's launches of one round cover
+// exactly these steps and one step-counter update serves them all.
+constexpr int64_t kRoundSteps = 32;
 
 int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
     if (!b || steps < 0) return fail(RT_ERR_ARG, "rt_batch_run: bad arguments");
@@ -1335,6 +1341,7 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
     }
     b->launches = 0;
     b->graphs = 0;
+    b->replays = 0;
     b->timed = false;
     if (steps == 0 || b->chains.empty()) return RT_OK;
     // planar chains are grouped by model and run together; triangle-dimer chains run their
@@ -1463,7 +1470,13 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
     HIPCHK(hipEventRecord(b->ev_up, c->stream));
     b->up_pending = true;
     HIPCHK(hipEventRecord(b->ev0, c->stream));
-    for (const Group& g : groups) {
+    // Every group advances in rounds of kRoundSteps steps (each block height h and each
+    // triangle-dimer launch length divides it): one round of every group plus one update of the
+    // shared step counter is captured once as a HIP graph and replayed steps / kRoundSteps
+    // times; the graph is kept and replayed by later runs of the same batch as long as its
+    // chains, tile configurations, device tables and parameters are unchanged.  The steps left
+    // over run as direct launches at their offsets from the counter.
+    auto planar_args = [&](const Group& g) {
         PersistArgs a{};
         a.params = c->uniform ? nullptr : c->params;
         a.m1 = c->um1;
@@ -1475,57 +1488,90 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         a.step_ctr = b->ctr;
         a.err = (int*)(b->ctr + 1);
         a.spin_limit = 1;
-        const int64_t full = steps / g.h, rest = steps % g.h;
-        HIPCHK(launch_fill_u32((uint32_t*)b->ctr, 4, 0u, c->stream));
         a.nsteps = g.h;
-        if (use_graph && full > 0) {
-            // one round (up to kRoundBlocks blocks of h steps of every chain of the group, each
-            // launch at its step offset, then one step-counter update) captured once and
-            // replayed; the blocks left over run as the same launches directly
-            const int64_t R = std::min<int64_t>(full, kRoundBlocks), rem = full % R;
-            hipGraph_t graph = nullptr;
-            hipGraphExec_t exec = nullptr;
-            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e1 = hipSuccess;
-            for (int64_t j = 0; j < R; j++) {
-                a.step_off = j * g.h;
-                const hipError_t ej = launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream);
-                if (e1 == hipSuccess) e1 = ej;
+        return a;
+    };
+    TdArgs ta{};
+    ta.params = c->uniform ? nullptr : c->params;
+    ta.m1 = c->um1;
+    ta.m2 = c->um2;
+    ta.tm = c->utm;
+    const int td_base = tdc.empty() ? 0 : td_config_base(td_cfg), td_spl = tdc.empty() ? 1 : td_config_steps(td_cfg);
+    // launches of one group covering `n` steps from offset `off` (planar: blocks of h and a
+    // partial last block; triangle dimers: pairs of steps when the configuration runs two, then
+    // single steps)
+    auto run_span = [&](int64_t off, int64_t n, int64_t* nl) -> hipError_t {
+        for (const Group& g : groups) {
+            PersistArgs a = planar_args(g);
+            for (int64_t j = 0; j * g.h < n; j++) {
+                a.step_off = off + j * g.h;
+                a.nsteps = std::min<int64_t>(g.h, n - j * g.h);
+                const hipError_t e = launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream);
+                if (e != hipSuccess) return e;
+                ++*nl;
             }
-            hipError_t e2 = launch_add_i64(b->ctr, R * g.h, c->stream);
-            hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
+        }
+        if (!htd.empty()) {
+            const int64_t pairs = td_spl == 2 ? n / 2 : 0;
+            for (int64_t j = 0; j < n - pairs; j++) {  // pairs first, then single steps
+                const bool two = j < pairs;
+                const int64_t o = two ? 2 * j : 2 * pairs + (j - pairs);
+                const hipError_t e = td_launch_step_batch(td_base, ta, c->uniform, b->dtd, (int)htd.size(), td_blocks,
+                                                          b->ctr, off + o, two, td_spl, c->stream);
+                if (e != hipSuccess) return e;
+                ++*nl;
+            }
+        }
+        return hipSuccess;
+    };
+    HIPCHK(launch_fill_u32((uint32_t*)b->ctr, 4, 0u, c->stream));  // step counter and error word
+    bool round_ok = true;  // every block height divides the round
+    for (const Group& g : groups) round_ok = round_ok && kRoundSteps % g.h == 0;
+    const int64_t replays = use_graph && round_ok ? steps / kRoundSteps : 0;
+    if (replays > 0) {
+        std::vector<int64_t> key{(int64_t)(intptr_t)b->dent, (int64_t)(intptr_t)b->dtd, (int64_t)c->uniform,
+                                 (int64_t)(intptr_t)c->params, (int64_t)c->um1, (int64_t)c->um2, (int64_t)c->utm,
+                                 (int64_t)td_cfg, (int64_t)td_blocks, (int64_t)htd.size()};
+        for (const Group& g : groups)
+            for (int64_t v : {(int64_t)g.model, (int64_t)g.cfg, (int64_t)g.first, (int64_t)g.count, (int64_t)g.blocks})
+                key.push_back(v);
+        if (!b->exec || key != b->graph_key) {
+            if (b->exec) HIPCHK(hipGraphExecDestroy(b->exec));
+            b->exec = nullptr;
+            hipGraph_t graph = nullptr;
+            int64_t nl = 0;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t e1 = run_span(0, kRoundSteps, &nl);
+            const hipError_t e2 = launch_add_i64(b->ctr, kRoundSteps, c->stream);
+            const hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
             if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
                 if (graph) hipGraphDestroy(graph);
                 return fail(RT_ERR_HIP, "rt_batch_run: graph capture failed");
             }
-            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            for (int64_t r = 0; r < full / R && e == hipSuccess; r++) e = hipGraphLaunch(exec, c->stream);
-            if (exec) hipGraphExecDestroy(exec);
+            const hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
             hipGraphDestroy(graph);
-            if (e != hipSuccess) return fail(RT_ERR_HIP, "rt_batch_run: graph replay: %s", hipGetErrorString(e));
-            for (int64_t j = 0; j < rem; j++) {
-                a.step_off = j * g.h;
-                HIPCHK(launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream));
+            if (e != hipSuccess) {
+                b->exec = nullptr;
+                return fail(RT_ERR_HIP, "rt_batch_run: graph instantiation: %s", hipGetErrorString(e));
             }
-            if (rem > 0) HIPCHK(launch_add_i64(b->ctr, rem * g.h, c->stream));
-            a.step_off = 0;
-            b->launches += full;
+            b->graph_key = key;
+            b->graph_launches = nl;
             b->graphs++;
-        } else {
-            for (int64_t r = 0; r < full; r++) {
-                HIPCHK(launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream));
-                HIPCHK(launch_add_i64(b->ctr, g.h, c->stream));
-            }
-            b->launches += full;
         }
-        if (rest > 0) {
-            a.nsteps = rest;
-            HIPCHK(launch_persist(g.model, g.cfg, c->uniform, a, g.blocks, c->stream));
-            b->launches++;
-        }
-        const int64_t rounds = full + (rest > 0 ? 1 : 0);
+        for (int64_t r = 0; r < replays; r++) HIPCHK(hipGraphLaunch(b->exec, c->stream));
+        b->launches += replays * b->graph_launches;
+        b->replays = replays;
+    }
+    {
+        int64_t nl = 0;
+        HIPCHK(run_span(0, steps - replays * kRoundSteps, &nl));
+        b->launches += nl;
+    }
+    // copy flips: a planar chain flips its copies once per block, a triangle-dimer chain once
+    // per launch
+    for (const Group& g : groups) {
+        const int64_t rounds = (steps + g.h - 1) / g.h;
         for (size_t k = g.first; k < g.first + g.count; k++) {
-            // each chain of the group advanced by `rounds` copy flips
             rt_chain* ch = b->hent_chain[k];
             if (rounds & 1) {
                 ch->cur ^= 1;
@@ -1537,59 +1583,7 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         }
     }
     if (!htd.empty()) {
-        TdArgs a{};
-        a.params = c->uniform ? nullptr : c->params;
-        a.m1 = c->um1;
-        a.m2 = c->um2;
-        a.tm = c->utm;
-        HIPCHK(launch_fill_u32((uint32_t*)b->ctr, 4, 0u, c->stream));
-        // two-step configurations: pairs of steps, then an odd last step; each kind of launch
-        // (with its counter update) captured once and replayed
-        const int base = td_config_base(td_cfg), spl = td_config_steps(td_cfg);
-        const int64_t pairs = spl == 2 ? steps / 2 : 0, singles = steps - 2 * pairs;
-        for (int kind = 0; kind < 2; kind++) {
-            const bool two = kind == 0;
-            const int64_t count = two ? pairs : singles;
-            if (count == 0) continue;
-            const int k = two ? 2 : 1;  // steps per launch of this kind
-            if (use_graph) {
-                // rounds of up to kRoundBlocks launches at their step offsets, one counter update
-                const int64_t R = std::min<int64_t>(count, kRoundBlocks), rem = count % R;
-                hipGraph_t graph = nullptr;
-                hipGraphExec_t exec = nullptr;
-                HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                hipError_t e1 = hipSuccess;
-                for (int64_t j = 0; j < R; j++) {
-                    const hipError_t ej = td_launch_step_batch(base, a, c->uniform, b->dtd, (int)htd.size(), td_blocks,
-                                                               b->ctr, j * k, two, spl, c->stream);
-                    if (e1 == hipSuccess) e1 = ej;
-                }
-                hipError_t e2 = launch_add_i64(b->ctr, R * k, c->stream);
-                hipError_t e3 = hipStreamEndCapture(c->stream, &graph);
-                if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
-                    if (graph) hipGraphDestroy(graph);
-                    return fail(RT_ERR_HIP, "rt_batch_run: graph capture failed");
-                }
-                hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-                for (int64_t r = 0; r < count / R && e == hipSuccess; r++) e = hipGraphLaunch(exec, c->stream);
-                if (exec) hipGraphExecDestroy(exec);
-                hipGraphDestroy(graph);
-                if (e != hipSuccess) return fail(RT_ERR_HIP, "rt_batch_run: graph replay: %s", hipGetErrorString(e));
-                for (int64_t j = 0; j < rem; j++)
-                    HIPCHK(td_launch_step_batch(base, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr, j * k,
-                                                two, spl, c->stream));
-                if (rem > 0) HIPCHK(launch_add_i64(b->ctr, rem * k, c->stream));
-                b->graphs++;
-            } else {
-                for (int64_t r = 0; r < count; r++) {
-                    HIPCHK(td_launch_step_batch(base, a, c->uniform, b->dtd, (int)htd.size(), td_blocks, b->ctr, 0, two,
-                                                spl, c->stream));
-                    HIPCHK(launch_add_i64(b->ctr, k, c->stream));
-                }
-            }
-        }
-        const int64_t launches = pairs + singles;
-        b->launches += launches;
+        const int64_t pairs = td_spl == 2 ? steps / 2 : 0, launches = steps - pairs;
         for (rt_chain* ch : tdc) {
             if (launches & 1) {
                 ch->cur ^= 1;

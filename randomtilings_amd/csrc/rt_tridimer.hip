@@ -10,11 +10,12 @@
 //
 // A reference step is one Rotate kernel plus three to nine update kernels, each reading the
 // previous one's output at distance <= 1.  Here a step is one kernel: every workgroup loads
-// its tile plus a halo of kHalo sites into LDS, draws every loaded site's TinyMT number,
-// applies the rotation, and replays the update kernels as LDS phases separated by workgroup
-// barriers.  Phase p only computes sites at least p away from a halo edge that cuts the
-// domain, which is exactly the region whose inputs are still valid; kHalo = 6 covers the
-// longest chain (butterfly moves).  Generators are double-buffered: halo copies read the
+// its tile plus a halo into LDS, draws every loaded site's TinyMT number, applies the
+// rotation, and replays the update kernels as LDS phases separated by workgroup barriers.
+// Phase p only computes sites at least p away from a halo edge that cuts the domain, which is
+// exactly the region whose inputs are still valid.  The halo is as wide as the step's chain of
+// phases: 4 for lozenge and triangle moves, kHalo = 6 for butterfly moves (the move type of
+// a step is known when its block starts).  Generators are double-buffered: halo copies read the
 // previous states while owners write the new ones.
 //
 // Domains up to 64 x 64 sites run every step in one launch of one workgroup (the whole
@@ -68,6 +69,11 @@ __device__ __forceinline__ int rotate_value(const Move& m, int v, int gi, int gj
     if (v == 85) return par ? 170 : 85;
     return v;
 }
+
+// Distance a step's update chain reaches: its phases (one Rotate plus the re-derivations of
+// replay_updates) each read neighbours at distance <= 1; lozenge and triangle moves take four
+// phases, butterfly moves six (kHalo).
+__device__ __forceinline__ int move_reach(const Move& m) { return m.type == 2 ? kHalo : 4; }
 
 // ---- update kernels, per site (local i, j; global gi, gj) -------------------------------
 
@@ -415,11 +421,15 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     const int tiles_c = (M + TC - 1) / TC;
     const int ti = tile / tiles_c, tj = tile % tiles_c;
     const int oi = ti * TR, oj = tj * TC;  // owned tile origin
+    // the region: the owned tile and a halo as wide as this step's move reaches (kHalo at most,
+    // what the LDS holds)
+    const Move mv = decode(move);
+    const int hw = move_reach(mv);
     Region R;
     R.M = M;
-    R.gi0 = max(0, oi - kHalo);
-    R.gj0 = max(0, oj - kHalo);
-    const int gi1 = min(M, oi + TR + kHalo), gj1 = min(M, oj + TC + kHalo);
+    R.gi0 = max(0, oi - hw);
+    R.gj0 = max(0, oj - hw);
+    const int gi1 = min(M, oi + TR + hw), gj1 = min(M, oj + TC + hw);
     R.rows = gi1 - R.gi0;
     R.cols = gj1 - R.gj0;
     R.cutT = R.gi0 > 0;
@@ -427,7 +437,6 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     R.cutB = gi1 < M;
     R.cutR = gj1 < M;
     const View V{lds, RC, RR * RC};
-    const Move mv = decode(move);
     const int rp = rotate_plane(mv);
     const int n = R.rows * R.cols;
     const int oi1 = min(M, oi + TR), oj1 = min(M, oj + TC);
@@ -524,21 +533,25 @@ __global__ __launch_bounds__(NT) void k_td_step_batch(TdArgs a0, const TdBatchEn
     td_step_body<TR, TC, NT, UNI>(a, e.sched[s], tile);
 }
 
-// ---- S steps per launch, tiles + a halo of S * kHalo -------------------------------------
-// Every step's phases use up at most kHalo of margin at each cut edge, so after S steps the
-// owned tile, S * kHalo inside every cut edge, is exact; the sites in between hold values no
-// owned site reads.  The generators stay in registers between the steps, so the tile's cells
+// ---- S steps per launch, tiles + a halo of up to S * kHalo --------------------------------
+// Every step's phases use up its move's reach (4, or kHalo for butterfly moves) of margin at
+// each cut edge, so with a halo of the summed reaches the owned tile is exact after S steps;
+// the sites in between hold values no owned site reads.  The generators stay in registers between the steps, so the tile's cells
 // and generators cross HBM once per S steps (at S = 2 and 64 x 64 tiles 0.95 region loads per
 // owned site and step instead of 1.41).
 template <int TR, int TC, int NT, bool UNI, int S>
 __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t moves, const int tile) {
-    constexpr int HH = S * kHalo;
-    constexpr int RR = TR + 2 * HH, RC = TC + 2 * HH;
+    constexpr int HMAX = S * kHalo;
+    constexpr int RR = TR + 2 * HMAX, RC = TC + 2 * HMAX;
     __shared__ uint16_t lds[8 * RR * RC];
     const int M = a.M;
     const int tiles_c = (M + TC - 1) / TC;
     const int ti = tile / tiles_c, tj = tile % tiles_c;
     const int oi = ti * TR, oj = tj * TC;  // owned tile origin
+    // the halo: the sum of the S steps' reaches (each step's phases use up its own)
+    int HH = 0;
+#pragma unroll
+    for (int s = 0; s < S; s++) HH += move_reach(decode((moves >> (8 * s)) & 0xffu));
     Region R;
     R.M = M;
     R.gi0 = max(0, oi - HH);
@@ -580,6 +593,7 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
             }
         }
     }
+    int used = 0;  // margin the earlier steps' phases used up
     for (int s = 0; s < S; s++) {
         const Move mv = decode((moves >> (8 * s)) & 0xffu);
         const int rp = rotate_plane(mv);
@@ -599,7 +613,8 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
             }
         }
         __syncthreads();
-        replay_updates<NT>(V, R, mv, s * kHalo);
+        replay_updates<NT>(V, R, mv, used);
+        used += move_reach(mv);
     }
 #pragma unroll
     for (int q = 0; q < SPT; q++) {

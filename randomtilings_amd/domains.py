@@ -1,5 +1,6 @@
 """Domain mixes for batched runs: Aztec diamonds, domino rectangles with staircase notches
-(tileability decided by the device maximal/minimal construction), triangle-dimer regions and
+(tileability decided by the device maximal/minimal construction), triangle-dimer hexagons
+(covered by TriangleDimerTiler::Hexagon, sides chosen so a dimer cover exists) and
 rectangle-triangle hexagons, drawn from a seed.
 
 The reference builds one domain per tiler and loops over them in the caller
@@ -41,6 +42,23 @@ def notched_rectangle(ctx, side: int, rng: np.random.Generator, lo: int = 8):
             side = max(lo, side - 1)
 
 
+def hexagon_vertices(x: int, y: int, z: int) -> int:
+    """Vertices of the triangle-dimer hexagon with sides x, y, z (TriangleDimerTiler::Hexagon):
+    a top row of x + 1, y rows growing by one, z rows shrinking by one."""
+    return sum(x + 1 + min(r, y) - max(0, r - y) for r in range(y + z + 1))
+
+
+def tridimer_hexagon_sides(side: int):
+    """Sides (x, y, z) of a nearly regular triangle-dimer hexagon whose tiling side is about
+    `side` (4 m + 12 for sides near m) and whose vertex count is even (a regular hexagon's,
+    3 m^2 + 3 m + 1, never is)."""
+    m = max(2, (side - 12) // 4)
+    for xyz in ((m + 1, m, m), (m, m, m - 1), (m, m, m + 1), (m + 1, m, m + 1), (m, m + 1, m)):
+        if hexagon_vertices(*xyz) % 2 == 0:
+            return xyz
+    raise ValueError(side)
+
+
 def domain(ctx, kind: str, side: int, rng: np.random.Generator):
     """(model, rule, start tiling) of one domain of the mix with a tiling side near `side`."""
     import randomtilings_amd as rt
@@ -51,23 +69,20 @@ def domain(ctx, kind: str, side: int, rng: np.random.Generator):
     if kind == "notched":
         return rt.RT_DOMINO, rt.RT_DOMINO_CFTP, notched_rectangle(ctx, side, rng)
     if kind == "tridimer":
-        n = max(2, side // 2 - 4)
-        while ((n + 1) * (n + 2) // 2) % 2:  # the cone's vertex count must be even
-            n += 1
-        return rt.RT_TRIDIMER, 0, statics.call("tridimer.IceCreamCone", args=[n, 0])
+        return rt.RT_TRIDIMER, 0, statics.call("tridimer.Hexagon", args=list(tridimer_hexagon_sides(side)))
     if kind == "recttri":
         return rt.RT_RECTTRI, 0, statics.call("recttri.maxHex", args=[max(1, (side - 7) // 2)])
     raise ValueError(kind)
 
 
-def mix(ctx, count: int, sides, seed: int):
-    """`count` domains, kinds cycling through KINDS, tiling sides uniform in [lo, hi] drawn
+def mix(ctx, count: int, sides, seed: int, kinds=KINDS):
+    """`count` domains, kinds cycling through `kinds`, tiling sides uniform in [lo, hi] drawn
     from `seed`: a list of (model, rule, tiling, kind)."""
     rng = np.random.default_rng(seed)
     lo, hi = sides
     out = []
     for k in range(count):
-        kind = KINDS[k % len(KINDS)]
+        kind = kinds[k % len(kinds)]
         side = int(rng.integers(lo, hi + 1))
         m, r, t = domain(ctx, kind, side, rng)
         out.append((m, r, t, kind))

@@ -7,6 +7,7 @@
 // the SVG documents (:772-905).
 #include "TriangleDimerTiler.h"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -150,6 +151,91 @@ tiling TriangleDimerTiler::IceCreamCone(int N, int M) {
                 }
                 if (j == last && i % 4 == (2 + N % 2 + 2 * k) % 4) f.R(i, j) = 1;
             }
+        }
+    }
+    return f.states();
+}
+
+// Rows r = 0 .. y + z of the hexagon, at lattice row a = 2 + r: the first y + 1 rows start one
+// vertex further left each row and end at the same column, the last z rows start at the same
+// column and end one vertex further left each row.  A vertex (a, b) has the lower neighbours
+// (a + 1, b) (edge l[a][b]) and (a + 1, b - 1) (edge r[a][b - 1]).  Scanning the rows top down,
+// each row's vertices are paired horizontally except at most one, at an even offset from the
+// row's first vertex, which is matched to a lower neighbour at an even offset in the next row
+// (of its two lower neighbours, which differ by one, one has it; a vertex near the middle of a
+// row has both, and rows change by at most one vertex at either end).  The scan covers every
+// vertex iff their number is even.
+tiling TriangleDimerTiler::Hexagon(int x, int y, int z) {
+    if (x < 0 || y < 0 || z < 0 || z > x + y) {
+        std::cout << "Bad hexagon sides." << std::endl;
+        return tiling(16 * 16, 0);
+    }
+    const int rows = y + z + 1;
+    std::vector<int> lo(rows), hi(rows);  // lattice columns of each row's first and last vertex
+    const int E = std::max(rows, x + y + 1) + 5;
+    long count = 0;
+    for (int r = 0; r < rows; ++r) {
+        lo[r] = 2 + y - std::min(r, y);
+        hi[r] = 2 + y + x - std::max(0, r - y);
+        count += hi[r] - lo[r] + 1;
+    }
+    if (count % 2 != 0) {
+        std::cout << "No dimer cover exists." << std::endl;
+        return tiling((size_t)4 * E * E, 0);
+    }
+    Dimers f(E);
+    int carry = -1;  // column of the vertex of the previous row still to be matched in this row
+    for (int r = 0; r < rows; ++r) {
+        const int a = 2 + r;
+        int start = lo[r];  // first vertex of the run still to pair horizontally
+        int skip = -1;      // the vertex the carry took
+        if (carry >= 0) {
+            // lower neighbours of (a - 1, carry): (a, carry) and (a, carry - 1); take the one
+            // at an even offset from this row's first vertex
+            const int v = ((carry - lo[r]) % 2 == 0) ? carry : carry - 1;
+            if (v < lo[r] || v > hi[r]) {
+                std::cout << "No dimer cover exists." << std::endl;
+                return tiling((size_t)4 * E * E, 0);
+            }
+            if (v == carry) f.L(a - 1, carry) = 1;
+            else f.R(a - 1, v) = 1;
+            skip = v;
+        }
+        carry = -1;
+        // pair [start, skip) (of even length: skip lies at an even offset) and (skip, hi]; an
+        // odd run leaves one vertex, near its middle at an even offset from its first vertex,
+        // for the next row
+        bool ok = true;
+        auto pair_run = [&](int from, int to) {
+            const int n = to - from + 1;
+            if (n <= 0) return;
+            int keep = -1;
+            if (n % 2 != 0) {
+                if (r + 1 == rows) {
+                    ok = false;
+                    return;
+                }
+                keep = from + 2 * ((n - 1) / 4);
+            }
+            for (int b = from; b <= to;) {
+                if (b == keep) {
+                    b += 1;
+                    continue;
+                }
+                f.H(a, b) = 1;
+                b += 2;
+            }
+            if (keep >= 0) carry = keep;
+        };
+        if (skip >= 0) {
+            pair_run(start, skip - 1);
+            pair_run(skip + 1, hi[r]);
+        } else {
+            pair_run(start, hi[r]);
+        }
+        if (!ok) {
+            std::cout << "No dimer cover exists." << std::endl;
+            return tiling((size_t)4 * E * E, 0);
         }
     }
     return f.states();

@@ -39,6 +39,13 @@ class TriangleDimerTiler {
     // A triangle of side N edges pointing down with a triangle of side M cut from its top
     // left corner, covered by a fixed dimer configuration (tiling side 2N + 8).
     static tiling IceCreamCone(int N, int M);
+    // A hexagon of the triangular lattice with sides x, y, z, x + y - z, z, y (a top row of
+    // x + 1 vertices, y rows growing by one, then z rows shrinking by one) covered by dimers,
+    // as a tiling of side 2 (max(y + z + 1, x + y + 1) + 5).  A hexagon with an odd number of
+    // vertices has no dimer cover: the message the reference prints for a cone without one is
+    // printed and an all-zero tiling returned.  Not in the reference (its only region is the
+    // cone).
+    static tiling Hexagon(int x, int y, int z);
     // A single butterfly of orientation 0 (H), 1 (L) or 2 (R) in a 16 x 16 tiling.
     static tiling TestButterfly(int type);
 

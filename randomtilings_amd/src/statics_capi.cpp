@@ -51,6 +51,7 @@ extern "C" int64_t rt_static_call(const char* name, const int32_t* in, int64_t n
 #endif
 #ifdef RT_HAVE_TRIDIMER
     if (f == "tridimer.IceCreamCone") return emit(TriangleDimerTiler::IceCreamCone(arg(0), arg(1)), out, cap);
+    if (f == "tridimer.Hexagon") return emit(TriangleDimerTiler::Hexagon(arg(0), arg(1), arg(2)), out, cap);
     if (f == "tridimer.TestButterfly") return emit(TriangleDimerTiler::TestButterfly(arg(0)), out, cap);
     if (f == "tridimer.TilingToTriangleFlips") return emit(TriangleDimerTiler::TilingToTriangleFlips(a), out, cap);
     if (f == "tridimer.TilingToButterflyFlips") return emit(TriangleDimerTiler::TilingToButterflyFlips(a), out, cap);

@@ -317,6 +317,25 @@ def test_tridimer_walk_parity(gctx, golden, path, cfg):
         assert tridimer_legal(g, tridimer_covered(t0))
 
 
+@pytest.mark.parametrize("path,cfg", TD_PATHS)
+def test_tridimer_hexagon_parity(gctx, path, cfg):
+    """Triangle-dimer hexagons (TriangleDimerTiler::Hexagon covers) on every path: a small one
+    (one workgroup on the AUTO path) and one spanning several tiles."""
+    from randomtilings_amd import statics as S
+    from tests.tilings import tridimer_covered, tridimer_legal
+    for xyz, steps in [((8, 3, 11), 301), ((41, 40, 40), 150)]:
+        t0 = S.call("tridimer.Hexagon", args=list(xyz))
+        assert t0.any()
+        say("tridimer hexagon", path, cfg, xyz)
+        gctx.set_path(path, cfg)
+        try:
+            g = gctx.tridimer_walk(t0, steps, 515)
+        finally:
+            gctx.set_path(rt.RT_PATH_AUTO, -1)
+        assert np.array_equal(g, O.tridimer_walk(t0, steps, 515, default_table(BIG))), xyz
+        assert tridimer_legal(g, tridimer_covered(t0))
+
+
 @pytest.mark.parametrize("cfg", range(N_TD))
 def test_tridimer_large_cone_tiled(gctx, cfg):
     """A cone whose move lattice spans several tiles of every configuration (M = 154): the
