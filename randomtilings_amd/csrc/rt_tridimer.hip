@@ -325,17 +325,62 @@ struct Region {
     bool cutT, cutB, cutL, cutR;
 };
 
+// Each thread takes one column of a band of NT / w rows and steps down by the band, so the
+// site index is divided once per phase, not once per site; the update range is folded into
+// the loop limits.  (A region wider than NT, which no tile configuration has, falls back to a
+// flat index.)
 template <int NT, class F>
 __device__ __forceinline__ void for_sites(const Region& R, int m, F f) {
-    const int i0 = R.cutT ? m : 0, i1 = R.rows - (R.cutB ? m : 0);
-    const int j0 = R.cutL ? m : 0, j1 = R.cols - (R.cutR ? m : 0);
-    const int w = j1 - j0, n = (i1 - i0) * w;
+    const int i0 = max(R.cutT ? m : 0, 1 - R.gi0), i1 = min(R.rows - (R.cutB ? m : 0), R.M - 1 - R.gi0);
+    const int j0 = max(R.cutL ? m : 0, 1 - R.gj0), j1 = min(R.cols - (R.cutR ? m : 0), R.M - 1 - R.gj0);
+    const int w = j1 - j0;
+    if (w <= 0 || i1 <= i0) return;
+    if (w <= NT) {
+        const int band = NT / w;
+        const int r = (int)threadIdx.x / w;
+        if (r >= band) return;
+        const int j = j0 + (int)threadIdx.x - r * w, gj = R.gj0 + j;
+        for (int i = i0 + r; i < i1; i += band) f(i, j, R.gi0 + i, gj);
+        return;
+    }
+    const int n = (i1 - i0) * w;
     const FastDiv fw((uint32_t)w);
     for (int idx = threadIdx.x; idx < n; idx += NT) {
         const int qi = (int)fw.div((uint32_t)idx);
         const int i = i0 + qi, j = j0 + idx - qi * w;
-        const int gi = R.gi0 + i, gj = R.gj0 + j;
-        if (gi > 0 && gj > 0 && gi < R.M - 1 && gj < R.M - 1) f(i, j, gi, gj);
+        f(i, j, R.gi0 + i, R.gj0 + j);
+    }
+}
+
+// Like for_sites, over the sites with (a * gi + b * gj) mod q == want only (a, b in {0, 1}, q in
+// {2, 3}): the colour classes the Rotate-adjacent phases touch.  Rows are walked with the
+// residue's first column and a stride of q, so threads are not spent on sites the phase skips.
+template <int NT, class F>
+__device__ __forceinline__ void for_sites_mod(const Region& R, int m, int a, int b, int q, int want, F f) {
+    const int i0 = max(R.cutT ? m : 0, 1 - R.gi0), i1 = min(R.rows - (R.cutB ? m : 0), R.M - 1 - R.gi0);
+    const int j0 = max(R.cutL ? m : 0, 1 - R.gj0), j1 = min(R.cols - (R.cutR ? m : 0), R.M - 1 - R.gj0);
+    const int w = j1 - j0;
+    if (w <= 0 || i1 <= i0) return;
+    if (b == 0) {  // a condition on rows only: every column, rows of one residue
+        const int band = NT / w;
+        const int r = (int)threadIdx.x / w;
+        if (r >= band) return;
+        const int j = j0 + (int)threadIdx.x - r * w, gj = R.gj0 + j;
+        int i = i0;
+        while (i < i1 && ((a * (R.gi0 + i)) % q) != want) i++;
+        for (i += r * q; i < i1; i += band * q) f(i, j, R.gi0 + i, gj);
+        return;
+    }
+    const int wc = (w + q - 1) / q;  // columns of one residue in a row, at most
+    const int band = NT / wc;
+    const int r = (int)threadIdx.x / wc;
+    if (r >= band) return;
+    const int k = (int)threadIdx.x - r * wc;
+    for (int i = i0 + r; i < i1; i += band) {
+        const int gi = R.gi0 + i;
+        // first column j >= j0 with (a gi + gj) mod q == want
+        const int j = j0 + (((want - a * gi - (R.gj0 + j0)) % q) + q) % q + q * k;
+        if (j < j1) f(i, j, gi, R.gj0 + j);
     }
 }
 
@@ -346,7 +391,9 @@ template <int NT>
 __device__ void replay_updates(const View& V, const Region& R, const Move& mv, const int m0 = 0) {
     if (mv.type == 0) {
         const int t = mv.r1, c = mv.r2;
-        for_sites<NT>(R, m0 + 1, [&](int i, int j, int gi, int gj) { loz_flipped(V, i, j, gi, gj, t, c); });
+        // UpdateLozengesFlipped rewrites the class ((1 - (t & 1)) gi + ((1 + t) / 2) gj) mod 2 != c
+        for_sites_mod<NT>(R, m0 + 1, 1 - (t & 1), (1 + t) / 2, 2, 1 - c,
+                          [&](int i, int j, int gi, int gj) { loz_flipped(V, i, j, gi, gj, t, c); });
         __syncthreads();
         for_sites<NT>(R, m0 + 2, [&](int i, int j, int, int) { loz_update(V, i, j, t); });
         __syncthreads();
@@ -374,11 +421,16 @@ __device__ void replay_updates(const View& V, const Region& R, const Move& mv, c
         });
     } else {
         const int t = mv.r1, p1 = mv.p1, p2 = mv.p2;
-        for_sites<NT>(R, m0 + 1, [&](int i, int j, int gi, int gj) { bfly_flipped1(V, i, j, gi, gj, t, p1, p2); });
+        // the slice classes bslice(t) = (b1 gi + b2 gj) mod 3 these phases rewrite
+        const int sa = t == 1 ? 0 : 1, sb = t == 0 ? 0 : 1;
+        for_sites_mod<NT>(R, m0 + 1, sa, sb, 3, p1,
+                          [&](int i, int j, int gi, int gj) { bfly_flipped1(V, i, j, gi, gj, t, p1, p2); });
         __syncthreads();
-        for_sites<NT>(R, m0 + 2, [&](int i, int j, int gi, int gj) { bfly_flipped21(V, i, j, gi, gj, t, p1); });
+        for_sites_mod<NT>(R, m0 + 2, sa, sb, 3, t == 0 ? (p1 + 1) % 3 : (p1 + 2) % 3,
+                          [&](int i, int j, int gi, int gj) { bfly_flipped21(V, i, j, gi, gj, t, p1); });
         __syncthreads();
-        for_sites<NT>(R, m0 + 3, [&](int i, int j, int gi, int gj) { bfly_flipped22(V, i, j, gi, gj, t, p1); });
+        for_sites_mod<NT>(R, m0 + 3, sa, sb, 3, t == 0 ? (p1 + 2) % 3 : (p1 + 1) % 3,
+                          [&](int i, int j, int gi, int gj) { bfly_flipped22(V, i, j, gi, gj, t, p1); });
         __syncthreads();
         // a site's lozenge states depend only on its own (just final) butterfly state
         for_sites<NT>(R, m0 + 4, [&](int i, int j, int gi, int gj) {
