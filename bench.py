@@ -62,6 +62,7 @@ def main() -> int:
     ap.add_argument("--domains", type=int, default=64, help="c3: number of domains")
     ap.add_argument("--sides", type=int, nargs=2, default=[64, 512], help="c3: domain side range")
     ap.add_argument("--domain-steps", type=int, default=1000, help="c3: chain steps per domain")
+    ap.add_argument("--autotune", action="store_true", help="c3: measure the batch's tile configurations first")
     ap.add_argument("--kinds", nargs="+", default=["aztec", "notched", "tridimer", "recttri"],
                     help="c3: domain kinds of the mix (aztec, notched, tridimer, recttri)")
     ap.add_argument("--canvas", type=int, default=64, help="c5: largest vertex grid side (training canvas)")
@@ -388,8 +389,13 @@ def bench_c4(args) -> int:
     times, steps, restarts = [], [], []
     for k in range(args.warmup + args.steps):
         w0 = time.perf_counter()
+        def said(r, n, c, w0=w0, k=k):  # after every epoch and replay: the sample's progress
+            what = "running" if c is None else ("coalesced" if c else "not coalesced")
+            print(f"# sample {k}: replay {r}, {n} steps, {what}, {time.perf_counter() - w0:.1f} s", file=sys.stderr,
+                  flush=True)
+
         out, tot, r = ctx.domino_cftp(tmax, tmin, args.seed + k, rule=rule, p12=p12, mode=rt.RT_CFTP_DOUBLING,
-                                      first_steps=1024, max_restarts=30)
+                                      first_steps=1024, max_restarts=30, progress=said)
         print(f"# sample {k}: {time.perf_counter() - w0:.2f} s, {r} restarts", file=sys.stderr, flush=True)
         if k >= args.warmup:
             times.append(time.perf_counter() - w0)
@@ -529,6 +535,13 @@ def bench_c3(args) -> int:
     b = rt.Batch(ctx)
     for ch in batched:
         b.add(ch)
+    tuned = None
+    if args.autotune:
+        # every applicable configuration of each model timed on the batch itself (its chains
+        # are put back as they were), the fastest kept for the run below
+        w0 = time.perf_counter()
+        tuned = b.autotune(64)
+        print(f"# c3: batch autotune {tuned} in {time.perf_counter() - w0:.1f} s", file=sys.stderr, flush=True)
     ctx.sync()
     w0 = time.perf_counter()
     b.run(steps, True)
@@ -553,7 +566,8 @@ def bench_c3(args) -> int:
              "attempts_per_step_all_domains": att // steps, "per_kind_domains_attempts": kinds,
              "batch_seconds": batch_s, "one_run_per_domain_seconds": solo_s,
              "one_run_per_domain_flips_per_s": att / solo_s, "batch_launches": b.last_launches,
-             "batch_graphs": b.last_graphs, "bit_identical_batch_vs_individual": bool(same)}
+             "batch_graphs": b.last_graphs, "bit_identical_batch_vs_individual": bool(same),
+             "batch_configs_autotuned": tuned}
     line = _line("aggregate flip attempts/s (batched domains)", att / batch_s, "flip attempts/s", args,
                  batch_s * 1e3, "u8/i32/u16",
                  f"{args.domains} domains, sides U[{lo},{hi}], kinds {'/'.join(args.kinds)} (Aztec diamonds, "

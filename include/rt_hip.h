@@ -203,6 +203,16 @@ int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* 
                    int32_t N, int64_t seed, int mode, int64_t first_steps, int32_t max_restarts, int32_t* out,
                    int64_t* total_steps, int32_t* restarts);
 
+/* rt_domino_cftp with a progress callback (NULL: none), called after every replay's coupling
+ * test with the replay's index (0 = the first), its length in steps and whether it coalesced
+ * (1 or 0), and within a replay after each epoch but the last with the steps done so far and
+ * coalesced = -1 (the epoch is waited for first); for callers of long runs (order-1000
+ * diamonds take minutes).  No reference counterpart. */
+typedef void (*rt_cftp_progress)(void* user, int32_t restart, int64_t replay_steps, int32_t coalesced);
+int rt_domino_cftp_ex(rt_ctx* ctx, int rule, const double* rparams, const int32_t* tmax, const int32_t* tmin,
+                      int32_t N, int64_t seed, int mode, int64_t first_steps, int32_t max_restarts, int32_t* out,
+                      int64_t* total_steps, int32_t* restarts, rt_cftp_progress progress, void* user);
+
 /* ---- height functions ----------------------------------------------------------------- */
 
 /* MaxTiling (dir = +1) / MinTiling (dir = -1) on the device — DominoTiler.cpp:214-400,
@@ -335,6 +345,12 @@ int rt_batch_destroy(rt_batch* b);
 int rt_batch_add(rt_batch* b, rt_chain* ch);
 int rt_batch_run(rt_batch* b, int64_t steps, int use_graph);
 int rt_batch_sync(rt_batch* b);
+/* Measure every tile configuration that applies to each model of the batch (probe_steps steps
+ * of that model's chains alone, after a warm-up) and keep the fastest for later runs of the
+ * batch; chosen (may be NULL) receives the choice per model RT_DOMINO .. RT_TRIDIMER (-1: none
+ * in the batch).  Every chain is put back exactly as it was (state, generators and host
+ * schedule RNGs), so the trajectories do not depend on the probes.  No reference counterpart. */
+int rt_batch_autotune(rt_batch* b, int64_t probe_steps, int32_t* chosen);
 /* GPU milliseconds of the most recent rt_batch_run (HIP events on the context stream). */
 int rt_batch_elapsed_ms(rt_batch* b, double* ms);
 /* Kernel launches (those of graph replays included) and graphs instantiated by the last run

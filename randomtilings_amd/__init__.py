@@ -61,6 +61,8 @@ SIGNATURES = {
     "rt_height_closure": (C.c_int, [_P, C.c_int, _I32P, C.c_int32, C.c_int, _I32P, C.POINTER(C.c_uint8), _I32P]),
     "rt_domino_cftp": (C.c_int, [_P, C.c_int, _DP, _I32P, _I32P, C.c_int32, C.c_int64, C.c_int, C.c_int64, C.c_int32,
                                  _I32P, _I64P, _I32P]),
+    "rt_domino_cftp_ex": (C.c_int, [_P, C.c_int, _DP, _I32P, _I32P, C.c_int32, C.c_int64, C.c_int, C.c_int64,
+                                    C.c_int32, _I32P, _I64P, _I32P, C.c_void_p, C.c_void_p]),
     "rt_chain_create": (C.c_int, [_P, C.c_int, C.c_int, _DP, C.c_int32, _I32P, C.c_int32, C.POINTER(_P)]),
     "rt_chain_destroy": (C.c_int, [_P]),
     "rt_chain_set_path": (C.c_int, [_P, C.c_int, C.c_int32]),
@@ -104,12 +106,17 @@ SIGNATURES = {
     "rt_batch_add": (C.c_int, [_P, _P]),
     "rt_batch_run": (C.c_int, [_P, C.c_int64, C.c_int]),
     "rt_batch_sync": (C.c_int, [_P]),
+    "rt_batch_autotune": (C.c_int, [_P, C.c_int64, _I32P]),
     "rt_batch_elapsed_ms": (C.c_int, [_P, _DP]),
     "rt_batch_last_launches": (C.c_int64, [_P]),
     "rt_batch_last_graphs": (C.c_int32, [_P]),
 }
 
 _lib = None
+
+
+# rt_cftp_progress(user, restart, replay_steps, coalesced)
+CFTP_PROGRESS = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int64, C.c_int32)
 
 
 class RTError(RuntimeError):
@@ -306,8 +313,11 @@ class Context:
         return t
 
     def domino_cftp(self, tmax: np.ndarray, tmin: np.ndarray, seed: int, rule: int = RT_DOMINO_CFTP, p12=None,
-                    mode: int = RT_CFTP_REFERENCE, first_steps: int = 128, max_restarts: int = 64):
-        """Coupling from the past (rt_domino_cftp): (sample, steps of the final replay, restarts)."""
+                    mode: int = RT_CFTP_REFERENCE, first_steps: int = 128, max_restarts: int = 64, progress=None):
+        """Coupling from the past (rt_domino_cftp): (sample, steps of the final replay, restarts).
+        progress(restart, steps, coalesced), if given, is called after every replay (coalesced
+        True or False) and after every epoch but the last of a replay (coalesced None, steps so
+        far)."""
         a = np.ascontiguousarray(tmax, dtype=np.int32)
         b = np.ascontiguousarray(tmin, dtype=np.int32)
         N = int(round(np.sqrt(a.size)))
@@ -315,9 +325,10 @@ class Context:
         out = np.zeros_like(a)
         tot = C.c_int64()
         rs = C.c_int32()
-        _check(self._lib.rt_domino_cftp(self.h, rule, None if rp is None else rp.ctypes.data_as(_DP), _i32(a), _i32(b),
-                                        N, seed, mode, first_steps, max_restarts, _i32(out), C.byref(tot),
-                                        C.byref(rs)))
+        cb = None if progress is None else CFTP_PROGRESS(lambda u, r, n, c: progress(r, n, None if c < 0 else bool(c)))
+        _check(self._lib.rt_domino_cftp_ex(self.h, rule, None if rp is None else rp.ctypes.data_as(_DP), _i32(a),
+                                           _i32(b), N, seed, mode, first_steps, max_restarts, _i32(out), C.byref(tot),
+                                           C.byref(rs), C.cast(cb, C.c_void_p) if cb is not None else None, None))
         return out, tot.value, rs.value
 
     def extremal_tiling(self, model: int, domain: np.ndarray, maximal: bool = True):
@@ -502,6 +513,13 @@ class Batch:
 
     def sync(self) -> None:
         _check(self._lib.rt_batch_sync(self.h))
+
+    def autotune(self, probe_steps: int = 64):
+        """Time every applicable tile configuration per model on the batch's chains (put back
+        as they were) and keep the fastest: {model: configuration} for the models present."""
+        out = (C.c_int32 * 4)()
+        _check(self._lib.rt_batch_autotune(self.h, probe_steps, out))
+        return {m: out[m] for m in range(4) if out[m] >= 0}
 
     def elapsed_ms(self) -> float:
         v = C.c_double()

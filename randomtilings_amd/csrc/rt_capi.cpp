@@ -814,18 +814,24 @@ static bool c_has_params(const rt_ctx* c, const rt_chain* ch) { return c->params
 // Whether the packed sweep can run this chain's rule: every lozenge rule (their intervals are
 // disjoint); for dominoes and coupled domino pairs the rules that split k at one threshold
 // 0 < t < 2^23 per class and row parity (3 -> 12 below t, 12 -> 3 from t on: the CFTP and
-// weighted heat-bath rules).
+// weighted heat-bath rules), and the toggle rule of dominokernel.cl (both 3 -> 12 and 12 -> 3
+// below one threshold t, the same for every class and row parity).
 static bool packed_rule_ok(const rt_chain* ch) {
     if (ch->model == RT_LOZENGE) return true;
     if (ch->model != RT_DOMINO && ch->model != kDominoPair) return false;
     const FlipRule& f = ch->rp.fr;
+    bool split = true, toggle = true;
+    const uint32_t t0 = f.ab_hi[0][0];
     for (int c = 0; c < 2; c++)
         for (int p = 0; p < 2; p++) {
             const uint32_t t = f.ab_hi[c][p];
             if (f.ab_lo[c][p] != 0 || f.ba_lo[c][p] != t || f.ba_hi[c][p] != (1u << 23) || t == 0 || t >= (1u << 23))
-                return false;
+                split = false;
+            if (f.ab_lo[c][p] != 0 || f.ba_lo[c][p] != 0 || t != t0 || f.ba_hi[c][p] != t0 || t0 == 0 ||
+                t0 >= (1u << 23))
+                toggle = false;
         }
-    return true;
+    return split || toggle;
 }
 
 // Tile configuration for the LDS-resident paths.  *resident is true when the whole tile grid
@@ -1277,6 +1283,7 @@ struct rt_batch {
     hipGraphExec_t exec = nullptr;  // one round of every group, kept across runs
     std::vector<int64_t> graph_key;  // what the kept graph was captured for
     int64_t graph_launches = 0;      // kernel launches in one replay
+    int cfg_override[5] = {-1, -1, -1, -1, -1};  // per model: the configuration rt_batch_autotune chose
     bool timed = false;
 };
 
@@ -1385,6 +1392,7 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         for (rt_chain* ch : planar)
             if (ch->model == m && !packed_rule_ok(ch)) all_packed_ok = false;
         g.cfg = choose_persist_cfg(largest, &nb, &tcn, &res, !all_packed_ok, true);
+        if (b->cfg_override[m] >= 0) g.cfg = b->cfg_override[m];
         if (g.cfg < 0) return fail(RT_ERR_ARG, "rt_batch_run: no tile configuration for model %d", m);
         int tr, tc, h, nt;
         persist_config(m, g.cfg, &tr, &tc, &h, &nt);
@@ -1433,6 +1441,7 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         for (rt_chain* ch : tdc)
             if (ch->G > largest->G) largest = ch;
         if (largest->cfg >= 0) td_cfg = largest->cfg;
+        if (b->cfg_override[RT_TRIDIMER] >= 0) td_cfg = b->cfg_override[RT_TRIDIMER];
         int tr, tc, nt;
         td_config(td_cfg, &tr, &tc, &nt);
         for (size_t k = 0; k < tdc.size(); k++) {
@@ -1606,6 +1615,120 @@ int rt_batch_sync(rt_batch* b) {
     return RT_OK;
 }
 
+// A chain's state for the batch autotuner to put back after a probe: its current copies of the
+// state and generators, and the host schedule RNGs.
+struct ChainSnap {
+    void* cls = nullptr;
+    void* gens = nullptr;
+    size_t cbytes = 0, gbytes = 0;
+    int cur = 0, gcur = 0;
+    uint32_t epoch = 0;
+    std::mt19937 mt;
+    std::minstd_rand0 lcg;
+    std::uniform_int_distribution<int> d3, d2;
+};
+
+static size_t state_bytes(const rt_chain* ch) {
+    return ch->model == RT_TRIDIMER ? sizeof(uint4) * (size_t)ch->G : (size_t)ch->N * ch->P * ch->NC * ch->esize;
+}
+
+int rt_batch_autotune(rt_batch* b, int64_t probe_steps, int32_t* chosen) {
+    if (!b || probe_steps <= 0) return fail(RT_ERR_ARG, "rt_batch_autotune: bad arguments");
+    rt_ctx* c = b->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    for (rt_chain* ch : b->chains)
+        if (!ch->seeded) return fail(RT_ERR_ARG, "rt_batch_autotune: every chain must be seeded");
+    std::vector<ChainSnap> snap(b->chains.size());
+    int rc = RT_OK;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    for (size_t k = 0; k < b->chains.size() && e == hipSuccess; k++) {
+        rt_chain* ch = b->chains[k];
+        ChainSnap& sn = snap[k];
+        sn.cbytes = state_bytes(ch);
+        sn.gbytes = sizeof(uint4) * (size_t)ch->G;
+        e = hipMalloc(&sn.cls, sn.cbytes);
+        if (e == hipSuccess) e = hipMalloc(&sn.gens, sn.gbytes);
+        if (e == hipSuccess) e = hipMemcpyAsync(sn.cls, ch->cls[ch->cur], sn.cbytes, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sn.gens, ch->gens[ch->gcur], sn.gbytes, hipMemcpyDeviceToDevice, c->stream);
+        sn.cur = ch->cur;
+        sn.gcur = ch->gcur;
+        sn.epoch = ch->epoch;
+        sn.mt = ch->mt;
+        sn.lcg = ch->lcg;
+        sn.d3 = ch->d3;
+        sn.d2 = ch->d2;
+    }
+    auto restore = [&](const std::vector<size_t>& which) -> hipError_t {
+        hipError_t er = hipSuccess;
+        for (size_t k : which) {
+            rt_chain* ch = b->chains[k];
+            const ChainSnap& sn = snap[k];
+            ch->cur = sn.cur;
+            ch->gcur = sn.gcur;
+            ch->epoch = sn.epoch;
+            ch->mt = sn.mt;
+            ch->lcg = sn.lcg;
+            ch->d3 = sn.d3;
+            ch->d2 = sn.d2;
+            if (er == hipSuccess)
+                er = hipMemcpyAsync(ch->cls[ch->cur], sn.cls, sn.cbytes, hipMemcpyDeviceToDevice, c->stream);
+            if (er == hipSuccess)
+                er = hipMemcpyAsync(ch->gens[ch->gcur], sn.gens, sn.gbytes, hipMemcpyDeviceToDevice, c->stream);
+        }
+        return er;
+    };
+    if (e != hipSuccess) rc = fail(RT_ERR_HIP, "rt_batch_autotune: snapshot: %s", hipGetErrorString(e));
+    const int models[] = {RT_DOMINO, RT_LOZENGE, RT_RECTTRI, RT_TRIDIMER};
+    for (int m : models) {
+        if (rc != RT_OK) break;
+        std::vector<size_t> which;
+        for (size_t k = 0; k < b->chains.size(); k++)
+            if (b->chains[k]->model == m) which.push_back(k);
+        if (which.empty()) continue;
+        bool all_packed_ok = true;
+        for (size_t k : which) all_packed_ok = all_packed_ok && packed_rule_ok(b->chains[k]);
+        const int ncfg = m == RT_TRIDIMER ? td_config_count() : persist_config_count(m);
+        int best = -1;
+        double best_ms = 1e30;
+        for (int i = 0; i < ncfg && rc == RT_OK; i++) {
+            if (m != RT_TRIDIMER) {
+                int tr, tc, h, nt;
+                if (!persist_config(m, i, &tr, &tc, &h, &nt) || persist_waves_per_cu(m, i, c->uniform, true) <= 0) continue;
+                if (persist_config_packed(m, i) && !all_packed_ok) continue;
+            }
+            // the group alone, on configuration i, in a batch of its own
+            rt_batch* sub = nullptr;
+            rc = rt_batch_create(c, &sub);
+            if (rc != RT_OK) break;
+            for (size_t k : which) rt_batch_add(sub, b->chains[k]);
+            sub->cfg_override[m] = i;
+            double ms = 1e30;
+            rc = rt_batch_run(sub, probe_steps / 4 + 1, 1);  // warm-up (captures the graph)
+            if (rc == RT_OK) rc = rt_batch_run(sub, probe_steps, 1);
+            if (rc == RT_OK) rc = rt_batch_elapsed_ms(sub, &ms);
+            if (rc == RT_OK) rc = rt_batch_sync(sub);
+            rt_batch_destroy(sub);
+            const hipError_t er = restore(which);
+            if (rc == RT_OK && er != hipSuccess) rc = fail(RT_ERR_HIP, "rt_batch_autotune: restore: %s", hipGetErrorString(er));
+            if (rc == RT_OK && ms < best_ms) {
+                best_ms = ms;
+                best = i;
+            }
+        }
+        if (rc == RT_OK) b->cfg_override[m] = best;
+    }
+    (void)hipStreamSynchronize(c->stream);
+    for (ChainSnap& sn : snap) {
+        if (sn.cls) hipFree(sn.cls);
+        if (sn.gens) hipFree(sn.gens);
+    }
+    if (rc != RT_OK) return rc;
+    if (chosen)
+        for (int m = 0; m < 4; m++) chosen[m] = b->cfg_override[m];
+    return RT_OK;
+}
+
 int rt_batch_elapsed_ms(rt_batch* b, double* ms) {
     if (!b || !ms) return fail(RT_ERR_ARG, "rt_batch_elapsed_ms: null argument");
     *ms = 0.0;
@@ -1644,6 +1767,13 @@ int rt_domino_walk(rt_ctx* ctx, int rule, const double* rparams, int32_t* tiling
 int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* tmax, const int32_t* tmin,
                    int32_t N, int64_t seed, int mode, int64_t first_steps, int32_t max_restarts, int32_t* out,
                    int64_t* total_steps, int32_t* restarts) {
+    return rt_domino_cftp_ex(ctx, rule, rparams, tmax, tmin, N, seed, mode, first_steps, max_restarts, out,
+                             total_steps, restarts, nullptr, nullptr);
+}
+
+int rt_domino_cftp_ex(rt_ctx* ctx, int rule, const double* rparams, const int32_t* tmax, const int32_t* tmin,
+                      int32_t N, int64_t seed, int mode, int64_t first_steps, int32_t max_restarts, int32_t* out,
+                      int64_t* total_steps, int32_t* restarts, rt_cftp_progress progress, void* user) {
     if (!ctx || !tmax || !tmin || !out) return fail(RT_ERR_ARG, "rt_domino_cftp: null argument");
     if (rule != RT_DOMINO_CFTP && rule != RT_DOMINO_WEIGHTED)
         return fail(RT_ERR_ARG, "coupling from the past needs a monotone rule (RT_DOMINO_CFTP or RT_DOMINO_WEIGHTED)");
@@ -1705,6 +1835,10 @@ int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* 
             st = rt_chain_seed(ch, RT_SEED_EPOCH, epochs[k].second);
             if (st == RT_OK) st = rt_chain_run(ch, epochs[k].first);
             total += epochs[k].first;
+            if (progress && st == RT_OK && k + 1 < epochs.size()) {  // a replay's epochs can take minutes
+                st = rt_chain_sync(ch);
+                if (st == RT_OK) progress(user, r, total, -1);
+            }
         }
         uint32_t differ = 1;
         if (st == RT_OK && launch_pair_differs((const uint16_t*)ch->cls[ch->cur], (int64_t)(bytes / 2), count,
@@ -1717,6 +1851,7 @@ int rt_domino_cftp(rt_ctx* ctx, int rule, const double* rparams, const int32_t* 
             rc = st;
             break;
         }
+        if (progress) progress(user, r, total, differ == 0 ? 1 : 0);
         if (differ == 0) {
             std::vector<int32_t> both(NN);
             rc = rt_chain_read(ch, both.data());

@@ -241,6 +241,10 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
             dth1[g] = rules.fr.ab_hi[1][rmod[g]] << 9;
         }
     }
+    // dominokernel.cl's lazy rule (:11-17): below one threshold both 3 -> 12 and 12 -> 3, i.e.
+    // a toggle; its two intervals both start at 0 (a split rule's 12 -> 3 interval starts at its
+    // threshold t > 0).  The same for every thread of the workgroup.
+    const bool toggle = !PM::kLozenge && rules.fr.ba_lo[0][0] == 0u;
 
     // The colours of a block (H <= 8 steps) are fetched as one 8-byte word before the
     // previous block's exchange, so no step waits on a global load.  The schedule buffer is
@@ -268,7 +272,8 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
             const uint32_t t0 = tm_draw_raw(s0[g][b], s1[g][b], s2[g][b], s3[g][b], UNI ? a.m1 : pm1[g][b],
                                             UNI ? a.m2 : pm2[g][b], UNI ? a.tm : pmt[g][b]);
             if constexpr (!PM::kLozenge) {
-                x[b] = (t0 < dth) ? 3u : 12u;  // k < t: 3 -> 12; k >= t: 12 -> 3
+                // split rules: k < t: 3 -> 12; k >= t: 12 -> 3.  Toggle: 0x0f marks k < t.
+                x[b] = (t0 < dth) ? (toggle ? 0x0fu : 3u) : (toggle ? 0u : 12u);
             } else if constexpr (SPLIT) {
                 const uint32_t xb = (t0 >= bL) ? 38u : 0xffffffffu;
                 x[b] = (t0 < aN) ? 25u : xb;
@@ -286,12 +291,17 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
         }
     };
     // Toggle the bytes of quad g whose state equals its target (exact zero-byte test of
-    // S ^ X; 25 ^ 38 = 63, 3 ^ 12 = 15).
+    // S ^ X; 25 ^ 38 = 63, 3 ^ 12 = 15); under the lazy toggle rule, the bytes marked 0x0f
+    // whose state is 3 or 12.
+    auto zero_bytes = [](uint32_t d) __attribute__((always_inline)) {
+        const uint32_t t = (d & 0x7f7f7f7fu) + 0x7f7f7f7fu;
+        return RT_BITOP3(t, d, 0x80808080u, ~(A | B) & C);  // 0x80 where the byte of d is 0
+    };
     auto apply = [&](uint32_t* Ls, int g, uint32_t S, uint32_t X) __attribute__((always_inline)) {
         const int w = tid + g * NT;
-        const uint32_t d = S ^ X;
-        const uint32_t t = (d & 0x7f7f7f7fu) + 0x7f7f7f7fu;
-        const uint32_t z = RT_BITOP3(t, d, 0x80808080u, ~(A | B) & C);  // 0x80 where S_b == x_b
+        uint32_t z;
+        if (!PM::kLozenge && toggle) z = (zero_bytes(S ^ 0x03030303u) | zero_bytes(S ^ 0x0c0c0c0cu)) & (X << 4);
+        else z = zero_bytes(S ^ X);
         const uint32_t tog = ((z >> (PM::kLozenge ? 1 : 3)) - (z >> 7)) & gbytes[g];  // 63 / 15 there
         if (AREAW % NT == 0 || w < AREAW) Ls[w] = S ^ tog;
     };

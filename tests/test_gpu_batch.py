@@ -142,3 +142,37 @@ def test_batch_tridimer_configs(bctx, golden, cfg):
         b.close()
         for ch in solo + batched:
             ch.close()
+
+
+def test_batch_autotune_leaves_chains_untouched(bctx, golden):
+    """rt_batch_autotune times every applicable configuration of each model on the batch's own
+    chains and puts them back as they were: after it, the batch continues exactly where
+    individual runs of the same chains end, on the configurations it chose."""
+    specs = _specs(golden)
+    solo, batched = [], []
+    b = rt.Batch(bctx)
+    try:
+        for k, (m, rule, t0, rp) in enumerate(specs):
+            a, c = bctx.chain(m, rule, t0, rp), bctx.chain(m, rule, t0, rp)
+            a.seed(300 + k)
+            c.seed(300 + k)
+            solo.append(a)
+            batched.append(c)
+            b.add(c)
+        b.run(45)
+        chosen = b.autotune(40)
+        say("batch autotune", chosen)
+        assert set(chosen) == {rt.RT_DOMINO, rt.RT_LOZENGE, rt.RT_RECTTRI, rt.RT_TRIDIMER}
+        b.run(77)
+        b.sync()
+        for a in solo:
+            a.run(45)
+            a.run(77)
+        for k, (a, c) in enumerate(zip(solo, batched)):
+            assert np.array_equal(a.read(), c.read()), k
+        for m, cfg in chosen.items():
+            assert all(c.last_config == cfg for c, (mm, _, _, _) in zip(batched, specs) if mm == m)
+    finally:
+        b.close()
+        for ch in solo + batched:
+            ch.close()

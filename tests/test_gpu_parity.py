@@ -53,9 +53,7 @@ def paths(model):
 @pytest.mark.parametrize("path,cfg", paths(rt.RT_DOMINO))
 @pytest.mark.parametrize("rule", [rt.RT_DOMINO_LAZY, rt.RT_DOMINO_CFTP])
 def test_domino_walk_parity(gctx, golden, rule, path, cfg):
-    if rule == rt.RT_DOMINO_LAZY and cfg >= 0 and rt.tile_config_packed(rt.RT_DOMINO, cfg):
-        pytest.skip("the packed sweep runs rules that split k at one threshold; dominokernel.cl's "
-                    "overlapping intervals run the byte-per-position kernels")
+    # dominokernel.cl's lazy rule runs on the packed sweep as a toggle below one threshold
     for n, steps in [(1, 37), (3, 200), (16, 301), (64, 517)]:
         t0 = golden[f"domino_aztec{n}_min"]
         say("domino", rule, path, cfg, n, steps)
@@ -415,6 +413,25 @@ def test_domino_cftp_packed_configs(gctx, golden, cfg, rule, p12):
         o = O.domino_cftp(tmax, tmin, seed, default_table(BIG), rule=rule, p12=p12, mode=rt.RT_CFTP_DOUBLING,
                           max_restarts=9)
         assert np.array_equal(g[0], o[0]) and g[1:] == o[1:], (n, g[1:], o[1:])
+
+
+def test_domino_cftp_progress(gctx, golden):
+    """rt_domino_cftp_ex reports every replay (index, length, coalesced) and returns the same
+    sample as rt_domino_cftp: doubling from 8 steps, replays of 8, 16, 32, ... steps."""
+    tmax, tmin = golden["domino_aztec16_max"], golden["domino_aztec16_min"]
+    seen = []
+    inside = []
+    g = gctx.domino_cftp(tmax, tmin, 11, mode=rt.RT_CFTP_DOUBLING, first_steps=8, max_restarts=20,
+                         progress=lambda r, n, c: (inside if c is None else seen).append((r, n, c)))
+    h = gctx.domino_cftp(tmax, tmin, 11, mode=rt.RT_CFTP_DOUBLING, first_steps=8, max_restarts=20)
+    assert np.array_equal(g[0], h[0]) and g[1:] == h[1:]
+    assert [r for r, _, _ in seen] == list(range(g[2] + 1))
+    assert [n for _, n, _ in seen] == [8 << r for r in range(g[2] + 1)]
+    assert [c for _, _, c in seen] == [False] * g[2] + [True]
+    assert seen[-1][1] == g[1]
+    # within replay r (epochs 8 << (r - 1), ..., 8 newest first): the steps after each epoch
+    assert inside == [(r, sum(8 << e for e in range(r - 1, r - 1 - k, -1)), None)
+                      for r in range(1, g[2] + 1) for k in range(1, r + 1)]
 
 
 def test_domino_cftp_bound(gctx, golden):
