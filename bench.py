@@ -78,6 +78,7 @@ def main() -> int:
     ap.add_argument("--train-max-side", type=int, default=0, help="c5: largest training tiling side (0: 2 x canvas)")
     ap.add_argument("--held-big", type=int, default=0, help="c5: tiling side of one extra large held-out Aztec diamond")
     ap.add_argument("--big-horizon-n2", type=float, default=3.0, help="c5: horizon cap of the large one, in N^2")
+    ap.add_argument("--big-chains", type=int, default=4, help="c5: chains per start on held-out domains above 520")
     ap.add_argument("--held-sides", type=int, nargs=2, default=[96, 250], help="c5: held-out tiling side range")
     ap.add_argument("--eval-chains", type=int, default=16, help="c5: chains per start in the evaluation")
     args = ap.parse_args()
@@ -651,7 +652,7 @@ def bench_c5(args) -> int:
         tw = ws.warm_tiling(ctx, net, dom, args.canvas)  # U-Net forward, projection, tiling read-off
         prep_s = time.perf_counter() - w0
         tm = ws.midpoint_tiling(ctx, dom)
-        chains = args.eval_chains if dom.N <= 520 else max(4, args.eval_chains // 4)
+        chains = args.eval_chains if dom.N <= 520 else args.big_chains
         w0 = time.perf_counter()
         cmp = ws.compare_starts(ctx, dom, {"plain": dom.tmin, "midpoint": tm, "learned": tw}, horizon, every,
                                 77 + k, chains)
