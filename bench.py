@@ -656,8 +656,11 @@ def bench_c5(args) -> int:
         tm = ws.midpoint_tiling(ctx, dom)
         chains = args.eval_chains if dom.N <= 520 else args.big_chains
         w0 = time.perf_counter()
+        def said(p, step, total, k=k):
+            print(f"# held-out domain {k + 1}: pass {p}, {step}/{total} steps", file=sys.stderr, flush=True)
+
         cmp = ws.compare_starts(ctx, dom, {"plain": dom.tmin, "midpoint": tm, "learned": tw}, horizon, every,
-                                77 + k, chains)
+                                77 + k, chains, progress=said)
         eval_s = time.perf_counter() - w0
         # wall clock of one chain on this domain: seconds per chain step
         ch = dom.chain(ctx, dom.tmin)

@@ -89,6 +89,7 @@ struct rt_chain {
     std::uniform_int_distribution<int> d3{0, 2};
     std::uniform_int_distribution<int> d2{0, 1};  // triangle-dimer schedule (TriangleDimerTiler.cpp:211-213)
     int32_t* tsave = nullptr;  // triangle dimers: the caller's tiling; merge writes the move states into it
+    uint64_t* tbits = nullptr;  // triangle dimers: rotation bits of one draw block (td_bit_bytes)
     int path = RT_PATH_AUTO;
     int cfg = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -184,8 +185,20 @@ static void draw_schedule(rt_chain* ch, int64_t steps, uint8_t* out) {
 }
 
 // ---- triangle dimers (TriangleDimerTiler::Walk, TriangleDimerTiler.cpp:86-483) ----------
-// The chain state is M x M cells of eight 16-bit move states (M = N/2), derived on the
-// device from the uploaded tiling; the host draws each step's move exactly as Walk does.
+// The chain state is M x M 64-bit cells of the eight move states (M = N/2), derived on the
+// device from the uploaded tiling; the host draws each step's move exactly as Walk does.  The
+// tiled kernels read every step's Rotate outcomes from rotation bits drawn kDrawSteps steps at
+// a time.
+
+// Device buffers of a triangle-dimer chain: two copies of the cells and of the generators, and
+// the rotation bits.
+static hipError_t td_alloc(rt_chain* ch) {
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&ch->cls[b], sizeof(uint2) * (size_t)ch->G);
+    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&ch->gens[b], sizeof(uint4) * (size_t)ch->G);
+    if (e == hipSuccess) e = hipMalloc(&ch->tbits, td_bit_bytes(ch->W));
+    return e;
+}
 
 static int td_create(rt_chain* ch, const int32_t* tiling) {
     const int N = ch->N;
@@ -201,9 +214,7 @@ static int td_create(rt_chain* ch, const int32_t* tiling) {
     ch->esize = 2;
     ch->G = (int64_t)M * M;
     hipError_t e = hipSetDevice(ch->ctx->device);
-    const size_t cbytes = sizeof(uint4) * (size_t)ch->G;
-    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&ch->cls[b], cbytes);
-    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&ch->gens[b], sizeof(uint4) * (size_t)ch->G);
+    if (e == hipSuccess) e = td_alloc(ch);
     if (e == hipSuccess) e = hipMalloc(&ch->tsave, sizeof(int32_t) * (size_t)N * N);
     if (e == hipSuccess) e = hipMalloc(&ch->misc, 64);
     if (e == hipSuccess) e = hipEventCreate(&ch->ev0);
@@ -211,7 +222,7 @@ static int td_create(rt_chain* ch, const int32_t* tiling) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ch->ev_sched, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemcpy(ch->tsave, tiling, sizeof(int32_t) * (size_t)N * N, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(ch->misc, 0, 64);
-    if (e == hipSuccess) e = td_launch_derive(ch->tsave, N, (uint4*)ch->cls[0], ch->ctx->stream);
+    if (e == hipSuccess) e = td_launch_derive(ch->tsave, N, (uint2*)ch->cls[0], ch->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ch->ctx->stream);
     if (e != hipSuccess) return fail(RT_ERR_HIP, "triangle-dimer chain setup: %s", hipGetErrorString(e));
     ch->cur = 0;
@@ -263,8 +274,8 @@ static int td_run(rt_chain* ch, int64_t steps) {
         ch->sched_pending = true;
         const int64_t chunk = 1 << 16;  // keeps single launches short
         for (int64_t s0 = 0; s0 < steps; s0 += chunk) {
-            a.cells_in = (const uint4*)ch->cls[ch->cur];
-            a.cells_out = (uint4*)ch->cls[ch->cur ^ 1];
+            a.cells_in = (const uint2*)ch->cls[ch->cur];
+            a.cells_out = (uint2*)ch->cls[ch->cur ^ 1];
             a.gen_in = ch->gens[ch->gcur];
             a.gen_out = ch->gens[ch->gcur ^ 1];
             a.sched = ch->dsched + s0;
@@ -283,26 +294,33 @@ static int td_run(rt_chain* ch, int64_t steps) {
             auto it = g_tuned.find(std::make_tuple(ch->model, ch->N, c->uniform));
             if (it != g_tuned.end()) cfg = it->second;
         }
-        // two-step configurations advance pairs of steps per launch, an odd last step on the
-        // one-step kernel of the same tiles
+        // blocks of kDrawSteps steps: one draw launch, then the steps; two-step configurations
+        // advance pairs of steps per launch, an odd last step on the one-step kernel of the same
+        // tiles
         const int base = td_config_base(cfg);
-        int64_t s = 0;
-        for (; s < steps; s += 1) {
-            const bool pair = td_config_steps(cfg) == 2 && s + 1 < steps;
-            a.cells_in = (const uint4*)ch->cls[ch->cur];
-            a.cells_out = (uint4*)ch->cls[ch->cur ^ 1];
+        a.rbits = ch->tbits;
+        for (int64_t s0 = 0; s0 < steps; s0 += kDrawSteps) {
+            const int k = (int)std::min<int64_t>(kDrawSteps, steps - s0);
             a.gen_in = ch->gens[ch->gcur];
             a.gen_out = ch->gens[ch->gcur ^ 1];
-            if (pair) {
-                HIPCHK(td_launch_step2(base, a, c->uniform, (uint32_t)ch->hsched[s] | ((uint32_t)ch->hsched[s + 1] << 8),
-                                       c->stream));
-                s++;
-            } else {
-                HIPCHK(td_launch_step(base, a, c->uniform, ch->hsched[s], c->stream));
-            }
-            ch->cur ^= 1;
+            HIPCHK(td_launch_draw(a, c->uniform, k, c->stream));
             ch->gcur ^= 1;
             ch->last_launches++;
+            for (int p = 0; p < k; p++) {
+                const bool pair = td_config_steps(cfg) == 2 && p + 1 < k;
+                a.cells_in = (const uint2*)ch->cls[ch->cur];
+                a.cells_out = (uint2*)ch->cls[ch->cur ^ 1];
+                a.plane = p;
+                if (pair) {
+                    HIPCHK(td_launch_step2(base, a, (uint32_t)ch->hsched[s0 + p] | ((uint32_t)ch->hsched[s0 + p + 1] << 8),
+                                           c->stream));
+                    p++;
+                } else {
+                    HIPCHK(td_launch_step(base, a, ch->hsched[s0 + p], c->stream));
+                }
+                ch->cur ^= 1;
+                ch->last_launches++;
+            }
         }
         ch->last_cfg = cfg;
     }
@@ -313,7 +331,7 @@ static int td_run(rt_chain* ch, int64_t steps) {
 
 static int td_read(rt_chain* ch, int32_t* tiling) {
     HIPCHK(hipSetDevice(ch->ctx->device));
-    HIPCHK(td_launch_merge((const uint4*)ch->cls[ch->cur], ch->N, ch->tsave, ch->ctx->stream));
+    HIPCHK(td_launch_merge((const uint2*)ch->cls[ch->cur], ch->N, ch->tsave, ch->ctx->stream));
     HIPCHK(hipStreamSynchronize(ch->ctx->stream));
     HIPCHK(hipMemcpy(tiling, ch->tsave, sizeof(int32_t) * (size_t)ch->N * ch->N, hipMemcpyDeviceToHost));
     return RT_OK;
@@ -336,10 +354,8 @@ static int td_autotune(rt_chain* ch, int64_t probe_steps, int32_t* best) {
     t->NC = ch->NC;
     t->esize = ch->esize;
     t->G = ch->G;
-    const size_t cbytes = sizeof(uint4) * (size_t)ch->G;
-    hipError_t e = hipSuccess;
-    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&t->cls[b], cbytes);
-    for (int b = 0; b < 2 && e == hipSuccess; b++) e = hipMalloc(&t->gens[b], cbytes);
+    const size_t cbytes = sizeof(uint2) * (size_t)ch->G;
+    hipError_t e = td_alloc(t);
     if (e == hipSuccess) e = hipMalloc(&t->misc, 64);
     if (e == hipSuccess) e = hipEventCreate(&t->ev0);
     if (e == hipSuccess) e = hipEventCreate(&t->ev1);
@@ -639,6 +655,7 @@ int rt_chain_destroy(rt_chain* ch) {
     free_runs(ch);
     if (ch->dsched) hipFree(ch->dsched);
     if (ch->tsave) hipFree(ch->tsave);
+    if (ch->tbits) hipFree(ch->tbits);
     if (ch->misc) hipFree(ch->misc);
     if (ch->bar) hipFree(ch->bar);
     if (ch->ev0) hipEventDestroy(ch->ev0);
@@ -1206,7 +1223,7 @@ int rt_chain_read(rt_chain* ch, int32_t* tiling) {
 int64_t rt_chain_attempts_per_step(rt_chain* ch) { return ch ? ch->G : 0; }
 
 int64_t rt_chain_state_bytes(rt_chain* ch) {
-    if (ch && ch->model == RT_TRIDIMER) return ch->G * (int64_t)sizeof(uint4);
+    if (ch && ch->model == RT_TRIDIMER) return ch->G * (int64_t)sizeof(uint2);
     return ch ? (int64_t)ch->N * ch->W * ch->NC * ch->esize : 0;
 }
 
@@ -1335,6 +1352,7 @@ int rt_batch_add(rt_batch* b, rt_chain* ch) {
 // of the triangle-dimer steps per launch (1, 2), so every group's launches of one round cover
 // exactly these steps and one step-counter update serves them all.
 constexpr int64_t kRoundSteps = 32;
+static_assert(kRoundSteps % kDrawSteps == 0, "a round holds whole triangle-dimer draw blocks");
 
 int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
     if (!b || steps < 0) return fail(RT_ERR_ARG, "rt_batch_run: bad arguments");
@@ -1434,7 +1452,7 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
     // triangle-dimer entries: one tiled launch per step covers every chain
     // configuration: the largest chain's own if set, else default 0 (32 x 32 tiles, one step
     // per launch; tuned single-chain choices are not used, large tiles being slower in batches)
-    int td_cfg = 0, td_blocks = 0;
+    int td_cfg = 0, td_blocks = 0, td_dblocks = 0;
     std::vector<TdBatchEntry> htd;
     if (!tdc.empty()) {
         rt_chain* largest = tdc[0];
@@ -1447,10 +1465,11 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         for (size_t k = 0; k < tdc.size(); k++) {
             rt_chain* ch = tdc[k];
             TdBatchEntry e{};
-            e.cells[0] = (uint4*)ch->cls[0];
-            e.cells[1] = (uint4*)ch->cls[1];
+            e.cells[0] = (uint2*)ch->cls[0];
+            e.cells[1] = (uint2*)ch->cls[1];
             e.gens[0] = ch->gens[0];
             e.gens[1] = ch->gens[1];
+            e.rbits = ch->tbits;
             e.sched = b->dsched + (planar.size() + k) * seg;
             e.M = ch->W;
             e.buf0 = ch->cur;
@@ -1458,6 +1477,9 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
             e.tile0 = td_blocks;
             e.ntiles = ((ch->W + tr - 1) / tr) * ((ch->W + tc - 1) / tc);
             td_blocks += e.ntiles;
+            e.dblk0 = td_dblocks;
+            e.ndblk = td_draw_blocks(ch->W);
+            td_dblocks += e.ndblk;
             htd.push_back(e);
         }
         if (b->dtd_cap < htd.size()) {
@@ -1507,8 +1529,9 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
     ta.tm = c->utm;
     const int td_base = tdc.empty() ? 0 : td_config_base(td_cfg), td_spl = tdc.empty() ? 1 : td_config_steps(td_cfg);
     // launches of one group covering `n` steps from offset `off` (planar: blocks of h and a
-    // partial last block; triangle dimers: pairs of steps when the configuration runs two, then
-    // single steps)
+    // partial last block; triangle dimers: per kDrawSteps steps (off and n multiples of it but
+    // for the run's last block) one draw launch, then pairs of steps when the configuration
+    // runs two, then single steps)
     auto run_span = [&](int64_t off, int64_t n, int64_t* nl) -> hipError_t {
         for (const Group& g : groups) {
             PersistArgs a = planar_args(g);
@@ -1520,13 +1543,18 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
                 ++*nl;
             }
         }
-        if (!htd.empty()) {
-            const int64_t pairs = td_spl == 2 ? n / 2 : 0;
-            for (int64_t j = 0; j < n - pairs; j++) {  // pairs first, then single steps
+        for (int64_t d0 = 0; !htd.empty() && d0 < n; d0 += kDrawSteps) {
+            const int64_t k = std::min<int64_t>(kDrawSteps, n - d0);
+            hipError_t e = td_launch_draw_batch(ta, c->uniform, b->dtd, (int)htd.size(), td_dblocks, b->ctr, off + d0,
+                                                (int)k, c->stream);
+            if (e != hipSuccess) return e;
+            ++*nl;
+            const int64_t pairs = td_spl == 2 ? k / 2 : 0;
+            for (int64_t j = 0; j < k - pairs; j++) {  // pairs first, then single steps
                 const bool two = j < pairs;
                 const int64_t o = two ? 2 * j : 2 * pairs + (j - pairs);
-                const hipError_t e = td_launch_step_batch(td_base, ta, c->uniform, b->dtd, (int)htd.size(), td_blocks,
-                                                          b->ctr, off + o, two, td_spl, c->stream);
+                e = td_launch_step_batch(td_base, ta, b->dtd, (int)htd.size(), td_blocks, b->ctr, off + d0 + o, two,
+                                         td_spl, c->stream);
                 if (e != hipSuccess) return e;
                 ++*nl;
             }
@@ -1540,7 +1568,7 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
     if (replays > 0) {
         std::vector<int64_t> key{(int64_t)(intptr_t)b->dent, (int64_t)(intptr_t)b->dtd, (int64_t)c->uniform,
                                  (int64_t)(intptr_t)c->params, (int64_t)c->um1, (int64_t)c->um2, (int64_t)c->utm,
-                                 (int64_t)td_cfg, (int64_t)td_blocks, (int64_t)htd.size()};
+                                 (int64_t)td_cfg, (int64_t)td_blocks, (int64_t)td_dblocks, (int64_t)htd.size()};
         for (const Group& g : groups)
             for (int64_t v : {(int64_t)g.model, (int64_t)g.cfg, (int64_t)g.first, (int64_t)g.count, (int64_t)g.blocks})
                 key.push_back(v);
@@ -1592,13 +1620,18 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         }
     }
     if (!htd.empty()) {
-        const int64_t pairs = td_spl == 2 ? steps / 2 : 0, launches = steps - pairs;
+        // cells flip once per step launch, generators once per draw launch (every kDrawSteps
+        // steps, the run's last block possibly shorter); pairs never straddle a draw block
+        int64_t launches = 0;
+        for (int64_t d0 = 0; d0 < steps; d0 += kDrawSteps) {
+            const int64_t k = std::min<int64_t>(kDrawSteps, steps - d0);
+            launches += k - (td_spl == 2 ? k / 2 : 0);
+        }
+        const int64_t draws = (steps + kDrawSteps - 1) / kDrawSteps;
         for (rt_chain* ch : tdc) {
-            if (launches & 1) {
-                ch->cur ^= 1;
-                ch->gcur ^= 1;
-            }
-            ch->last_launches = launches;
+            if (launches & 1) ch->cur ^= 1;
+            if (draws & 1) ch->gcur ^= 1;
+            ch->last_launches = launches + draws;
             ch->last_cfg = td_cfg;
             ch->timed = false;
         }
@@ -1629,7 +1662,7 @@ struct ChainSnap {
 };
 
 static size_t state_bytes(const rt_chain* ch) {
-    return ch->model == RT_TRIDIMER ? sizeof(uint4) * (size_t)ch->G : (size_t)ch->N * ch->P * ch->NC * ch->esize;
+    return ch->model == RT_TRIDIMER ? sizeof(uint2) * (size_t)ch->G : (size_t)ch->N * ch->P * ch->NC * ch->esize;
 }
 
 int rt_batch_autotune(rt_batch* b, int64_t probe_steps, int32_t* chosen) {

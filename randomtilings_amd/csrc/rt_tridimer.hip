@@ -3,20 +3,24 @@
 //
 // State: per site (a, b) of the M x M move lattice (M = N/2) eight small integers — the
 // lozenge-move states eH, eL, eR, the triangle-move states tU, tD and the butterfly-move
-// states bH, bL, bR — packed as eight 16-bit fields in one 16-byte cell, so a site moves
-// between HBM and LDS as one vector access.  Values stay below 2^10 whenever the input
-// tiling's states are 4-bit (checked on the host), so 16 bits reproduce the reference's
-// int arithmetic exactly.
+// states bH, bL, bR — packed into one 64-bit cell, so a site moves between HBM and LDS as one
+// 8-byte access.  With 4-bit lozenge states in the input tiling (checked on the host) every
+// state stays inside its field: lozenge states are sums of four distinct bits (<= 15),
+// triangle states three base-4 digits of at most 6 (<= 126, 7 bits), butterfly states four
+// digits of at most 6 plus 256 c (<= 766, 10 bits); the fields reproduce the reference's int
+// arithmetic exactly.
 //
 // A reference step is one Rotate kernel plus three to nine update kernels, each reading the
-// previous one's output at distance <= 1.  Here a step is one kernel: every workgroup loads
-// its tile plus a halo into LDS, draws every loaded site's TinyMT number, applies the
-// rotation, and replays the update kernels as LDS phases separated by workgroup barriers.
-// Phase p only computes sites at least p away from a halo edge that cuts the domain, which is
-// exactly the region whose inputs are still valid.  The halo is as wide as the step's chain of
-// phases: 4 for lozenge and triangle moves, kHalo = 6 for butterfly moves (the move type of
-// a step is known when its block starts).  Generators are double-buffered: halo copies read the
-// previous states while owners write the new ones.
+// previous one's output at distance <= 1.  The Rotate kernel's random number only decides
+// whether a site rotates (rd < 0.5), and a site's generator advances once per step whatever
+// the tiling, so the draws are made apart from the tiling: a draw launch advances every
+// generator kDrawSteps times and keeps each outcome as one bit per site and step.  A step is
+// then one kernel: every workgroup loads its tile plus a halo of cells and rotation bits into
+// LDS, applies the rotation, and replays the update kernels as LDS phases separated by
+// workgroup barriers.  Phase p only computes sites at least p away from a halo edge that cuts
+// the domain, which is exactly the region whose inputs are still valid.  The halo is as wide as
+// the step's chain of phases: 4 for lozenge and triangle moves, kHalo = 6 for butterfly moves
+// (the move type of a step is known when its block starts).
 //
 // Domains up to 64 x 64 sites run every step in one launch of one workgroup (the whole
 // domain in LDS, generator states in VGPRs), with only workgroup barriers between phases.
@@ -449,9 +453,17 @@ __device__ void replay_updates(const View& V, const Region& R, const Move& mv, c
     }
 }
 
-__device__ __forceinline__ void unpack_cell(const uint4 c, int v[8]) {
-    v[0] = c.x & 0xffff, v[1] = c.x >> 16, v[2] = c.y & 0xffff, v[3] = c.y >> 16;
-    v[4] = c.z & 0xffff, v[5] = c.z >> 16, v[6] = c.w & 0xffff, v[7] = c.w >> 16;
+// 64-bit cell: x = eH | eL << 4 | eR << 8 | tU << 12 | tD << 19, y = bH | bL << 10 | bR << 20.
+__device__ __forceinline__ void unpack_cell(const uint2 c, int v[8]) {
+    v[EH] = c.x & 15u, v[EL] = (c.x >> 4) & 15u, v[ER] = (c.x >> 8) & 15u;
+    v[TU] = (c.x >> 12) & 127u, v[TD] = (c.x >> 19) & 127u;
+    v[BH] = c.y & 1023u, v[BL] = (c.y >> 10) & 1023u, v[BR] = (c.y >> 20) & 1023u;
+}
+__device__ __forceinline__ uint2 pack_cell(const View& V, int i, int j) {
+    uint2 c;
+    c.x = V.g(EH, i, j) | V.g(EL, i, j) << 4 | V.g(ER, i, j) << 8 | V.g(TU, i, j) << 12 | V.g(TD, i, j) << 19;
+    c.y = V.g(BH, i, j) | V.g(BL, i, j) << 10 | V.g(BR, i, j) << 20;
+    return c;
 }
 
 template <bool UNI>
@@ -464,8 +476,76 @@ __device__ __forceinline__ void load_params(const TdArgs& a, size_t id, uint32_t
     }
 }
 
+// ---- rotation bits ----------------------------------------------------------------------
+// One wave per 64 sites of a row: each lane advances its site's generator k times, and the
+// wave's ballot of each draw's outcome (tempered output < 2^31, the Rotate kernels' rd < 0.5)
+// is the row's word of that step's plane, stored by lane s for step s.
+template <bool UNI>
+__device__ __forceinline__ void td_draw_row(const TdArgs& a, const uint4* __restrict__ gin, uint4* __restrict__ gout,
+                                            int M, int bx, int gi, int k, uint64_t* __restrict__ bits) {
+    const int lane = (int)threadIdx.x & 63;
+    const int gj = bx * 64 + lane;
+    const bool live = gj < M;
+    const size_t id = (size_t)gi * M + gj;
+    uint4 g = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t m1 = 0, m2 = 0, tm = 0;
+    if (live) {
+        g = gin[id];
+        load_params<UNI>(a, id, m1, m2, tm);
+    }
+    uint64_t mine = 0;
+    for (int s = 0; s < k; s++) {
+        bool low = false;
+        if (live) low = tm_draw_raw(g.x, g.y, g.z, g.w, m1, m2, tm) < 0x80000000u;
+        const uint64_t w = __ballot(low);
+        if (lane == s) mine = w;
+    }
+    if (live) gout[id] = g;
+    if (lane < k) bits[((size_t)lane * M + gi) * td_bit_words(M) + bx] = mine;
+}
+
+// Blocks of four waves: blockIdx.x = the word of the row, blockIdx.y = four rows.
+template <bool UNI>
+__global__ __launch_bounds__(256) void k_td_draw(TdArgs a, int k) {
+    const int gi = (int)blockIdx.y * 4 + ((int)threadIdx.x >> 6);
+    if (gi >= a.M) return;  // whole waves
+    td_draw_row<UNI>(a, a.gen_in, a.gen_out, a.M, (int)blockIdx.x, gi, k, a.rbits);
+}
+
+// The chain of a batched block: the entry whose range [first(e), ...) holds block b.
+template <class F>
+__device__ __forceinline__ int td_entry(const TdBatchEntry* __restrict__ batch, int nbatch, int b, F first) {
+    int lo = 0, hi = nbatch - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first(batch[mid]) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Batched draws of the k steps from s = *step_ctr + off (s a multiple of kDrawSteps from the
+// run's start): generators flip copies once per draw launch.
+template <bool UNI>
+__global__ __launch_bounds__(256) void k_td_draw_batch(TdArgs a, const TdBatchEntry* __restrict__ batch, int nbatch,
+                                                       const int64_t* __restrict__ step_ctr, int64_t off, int k) {
+    const int64_t s = *step_ctr + off;
+    const int b = (int)blockIdx.x;
+    const TdBatchEntry& e = batch[td_entry(batch, nbatch, b, [](const TdBatchEntry& x) { return x.dblk0; })];
+    const int rw = td_bit_words(e.M), r = b - e.dblk0;
+    const int gi = (r / rw) * 4 + ((int)threadIdx.x >> 6);
+    if (gi >= e.M) return;  // whole waves
+    const int par = (int)((s / kDrawSteps) & 1);
+    td_draw_row<UNI>(a, e.gens[e.g0 ^ par], e.gens[e.g0 ^ par ^ 1], e.M, r % rw, gi, k, e.rbits);
+}
+
+// Rotation bit of site (gi, gj) in plane p.
+__device__ __forceinline__ const uint64_t* bit_word(const TdArgs& a, int p, int gi, int gj) {
+    return a.rbits + ((size_t)p * a.M + gi) * td_bit_words(a.M) + (gj >> 6);
+}
+
 // ---- one step per launch, tiles + halo --------------------------------------------------
-template <int TR, int TC, int NT, bool UNI>
+template <int TR, int TC, int NT>
 __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t move, const int tile) {
     constexpr int RR = TR + 2 * kHalo, RC = TC + 2 * kHalo;
     __shared__ uint16_t lds[8 * RR * RC];
@@ -492,20 +572,20 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     const int rp = rotate_plane(mv);
     const int n = R.rows * R.cols;
     const int oi1 = min(M, oi + TR), oj1 = min(M, oj + TC);
-    // load the region, draw every site's number, rotate.  Every load of the thread's sites is
+    // load the region's cells and rotation bits, rotate.  Every load of the thread's sites is
     // issued before the first is used, so the loads overlap instead of waiting one by one.
     constexpr int SPT = (RR * RC + NT - 1) / NT;  // sites per thread
     const FastDiv fc((uint32_t)R.cols);
-    uint4 cin[SPT], gin[SPT];
+    uint2 cin[SPT];
+    uint64_t bw[SPT];
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
         const int idx = threadIdx.x + q * NT;
         if (idx < n) {
             const int qi = (int)fc.div((uint32_t)idx);
             const int gi = R.gi0 + qi, gj = R.gj0 + idx - qi * R.cols;
-            const size_t id = (size_t)gi * M + gj;
-            cin[q] = a.cells_in[id];
-            gin[q] = a.gen_in[id];
+            cin[q] = a.cells_in[(size_t)gi * M + gj];
+            bw[q] = *bit_word(a, a.plane, gi, gj);
         }
     }
 #pragma unroll
@@ -514,15 +594,9 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
         if (idx < n) {
             const int i = (int)fc.div((uint32_t)idx), j = idx - i * R.cols;
             const int gi = R.gi0 + i, gj = R.gj0 + j;
-            const size_t id = (size_t)gi * M + gj;
             int v[8];
             unpack_cell(cin[q], v);
-            uint4 g = gin[q];
-            uint32_t m1, m2, tm;
-            load_params<UNI>(a, id, m1, m2, tm);
-            const uint32_t u = tm_draw_raw(g.x, g.y, g.z, g.w, m1, m2, tm);
-            if (gi >= oi && gi < oi1 && gj >= oj && gj < oj1) a.gen_out[id] = g;
-            v[rp] = rotate_value(mv, v[rp], gi, gj, u < 0x80000000u);
+            v[rp] = rotate_value(mv, v[rp], gi, gj, (bw[q] >> (gj & 63)) & 1u);
 #pragma unroll
             for (int p = 0; p < 8; p++) V.s(p, i, j, v[p]);
         }
@@ -537,61 +611,49 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
         const int qi = (int)fo.div((uint32_t)idx);
         const int gi = oi + qi, gj = oj + idx - qi * ow;
         const int i = gi - R.gi0, j = gj - R.gj0;
-        uint4 c;
-        c.x = V.g(0, i, j) | (V.g(1, i, j) << 16);
-        c.y = V.g(2, i, j) | (V.g(3, i, j) << 16);
-        c.z = V.g(4, i, j) | (V.g(5, i, j) << 16);
-        c.w = V.g(6, i, j) | (V.g(7, i, j) << 16);
-        a.cells_out[(size_t)gi * M + gj] = c;
+        a.cells_out[(size_t)gi * M + gj] = pack_cell(V, i, j);
     }
 }
 
-template <int TR, int TC, int NT, bool UNI>
+template <int TR, int TC, int NT>
 __global__ __launch_bounds__(NT) void k_td_step(TdArgs a, uint32_t move) {
-    td_step_body<TR, TC, NT, UNI>(a, move, xcd_tile(blockIdx.x, gridDim.x));
+    td_step_body<TR, TC, NT>(a, move, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Batched: every block runs one tile of one chain; the step index comes from a device
-// counter (one captured launch replays as is), and with it the chain's moves and copies: the
-// copies flip once per launch, and the launches so far are s / spl (spl: steps per launch of
-// the run's earlier launches).
+// counter (one captured launch replays as is), and with it the chain's moves, rotation-bit
+// plane and copies: the copies flip once per launch, and the launches so far are s / spl
+// (spl: steps per launch of the run's earlier launches).
 __device__ __forceinline__ const TdBatchEntry& td_resolve(TdArgs& a, const TdBatchEntry* __restrict__ batch,
                                                          int nbatch, int64_t s, int spl, int& tile) {
     const int b = blockIdx.x;
-    int lo = 0, hi = nbatch - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (batch[mid].tile0 <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const TdBatchEntry& e = batch[lo];
+    const TdBatchEntry& e = batch[td_entry(batch, nbatch, b, [](const TdBatchEntry& x) { return x.tile0; })];
     const int par = (int)((s / spl) & 1);
     a.M = e.M;
     a.cells_in = e.cells[e.buf0 ^ par];
     a.cells_out = e.cells[e.buf0 ^ par ^ 1];
-    a.gen_in = e.gens[e.g0 ^ par];
-    a.gen_out = e.gens[e.g0 ^ par ^ 1];
+    a.rbits = e.rbits;
+    a.plane = (int)(s % kDrawSteps);
     tile = b - e.tile0;
     return e;
 }
 
-template <int TR, int TC, int NT, bool UNI>
+template <int TR, int TC, int NT>
 __global__ __launch_bounds__(NT) void k_td_step_batch(TdArgs a0, const TdBatchEntry* __restrict__ batch, int nbatch,
                                                       const int64_t* __restrict__ step_ctr, int64_t off, int spl) {
     const int64_t s = *step_ctr + off;
     TdArgs a = a0;
     int tile;
     const TdBatchEntry& e = td_resolve(a, batch, nbatch, s, spl, tile);
-    td_step_body<TR, TC, NT, UNI>(a, e.sched[s], tile);
+    td_step_body<TR, TC, NT>(a, e.sched[s], tile);
 }
 
 // ---- S steps per launch, tiles + a halo of up to S * kHalo --------------------------------
 // Every step's phases use up its move's reach (4, or kHalo for butterfly moves) of margin at
 // each cut edge, so with a halo of the summed reaches the owned tile is exact after S steps;
-// the sites in between hold values no owned site reads.  The generators stay in registers between the steps, so the tile's cells
-// and generators cross HBM once per S steps (at S = 2 and 64 x 64 tiles 0.95 region loads per
-// owned site and step instead of 1.41).
-template <int TR, int TC, int NT, bool UNI, int S>
+// the sites in between hold values no owned site reads.  The tile's cells cross HBM once per S
+// steps (at S = 2 and 64 x 64 tiles 0.95 region loads per owned site and step instead of 1.41).
+template <int TR, int TC, int NT, int S>
 __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t moves, const int tile) {
     constexpr int HMAX = S * kHalo;
     constexpr int RR = TR + 2 * HMAX, RC = TC + 2 * HMAX;
@@ -620,17 +682,14 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
     const int oi1 = min(M, oi + TR), oj1 = min(M, oj + TC);
     constexpr int SPT = (RR * RC + NT - 1) / NT;  // sites per thread
     const FastDiv fc((uint32_t)R.cols);
-    uint4 g[SPT];
     {
-        uint4 cin[SPT];
+        uint2 cin[SPT];
 #pragma unroll
         for (int q = 0; q < SPT; q++) {
             const int idx = threadIdx.x + q * NT;
             if (idx < n) {
                 const int qi = (int)fc.div((uint32_t)idx);
-                const size_t id = (size_t)(R.gi0 + qi) * M + (R.gj0 + idx - qi * R.cols);
-                cin[q] = a.cells_in[id];
-                g[q] = a.gen_in[id];
+                cin[q] = a.cells_in[(size_t)(R.gi0 + qi) * M + (R.gj0 + idx - qi * R.cols)];
             }
         }
 #pragma unroll
@@ -649,6 +708,15 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
     for (int s = 0; s < S; s++) {
         const Move mv = decode((moves >> (8 * s)) & 0xffu);
         const int rp = rotate_plane(mv);
+        uint64_t bw[SPT];
+#pragma unroll
+        for (int q = 0; q < SPT; q++) {
+            const int idx = threadIdx.x + q * NT;
+            if (idx < n) {
+                const int i = (int)fc.div((uint32_t)idx);
+                bw[q] = *bit_word(a, a.plane + s, R.gi0 + i, R.gj0 + idx - i * R.cols);
+            }
+        }
         if (s > 0) __syncthreads();  // the previous step's last phase wrote neighbours' states
 #pragma unroll
         for (int q = 0; q < SPT; q++) {
@@ -656,26 +724,14 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
             if (idx < n) {
                 const int i = (int)fc.div((uint32_t)idx), j = idx - i * R.cols;
                 const int gi = R.gi0 + i, gj = R.gj0 + j;
-                uint32_t m1, m2, tm;
-                load_params<UNI>(a, (size_t)gi * M + gj, m1, m2, tm);
-                const uint32_t u = tm_draw_raw(g[q].x, g[q].y, g[q].z, g[q].w, m1, m2, tm);
                 const int v = V.g(rp, i, j);
-                const int w = rotate_value(mv, v, gi, gj, u < 0x80000000u);
+                const int w = rotate_value(mv, v, gi, gj, (bw[q] >> (gj & 63)) & 1u);
                 if (w != v) V.s(rp, i, j, w);
             }
         }
         __syncthreads();
         replay_updates<NT>(V, R, mv, used);
         used += move_reach(mv);
-    }
-#pragma unroll
-    for (int q = 0; q < SPT; q++) {
-        const int idx = threadIdx.x + q * NT;
-        if (idx < n) {
-            const int qi = (int)fc.div((uint32_t)idx);
-            const int gi = R.gi0 + qi, gj = R.gj0 + idx - qi * R.cols;
-            if (gi >= oi && gi < oi1 && gj >= oj && gj < oj1) a.gen_out[(size_t)gi * M + gj] = g[q];
-        }
     }
     __syncthreads();
     const int ow = oj1 - oj, on = (oi1 - oi) * ow;
@@ -684,29 +740,24 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
         const int qi = (int)fo.div((uint32_t)idx);
         const int gi = oi + qi, gj = oj + idx - qi * ow;
         const int i = gi - R.gi0, j = gj - R.gj0;
-        uint4 c;
-        c.x = V.g(0, i, j) | (V.g(1, i, j) << 16);
-        c.y = V.g(2, i, j) | (V.g(3, i, j) << 16);
-        c.z = V.g(4, i, j) | (V.g(5, i, j) << 16);
-        c.w = V.g(6, i, j) | (V.g(7, i, j) << 16);
-        a.cells_out[(size_t)gi * M + gj] = c;
+        a.cells_out[(size_t)gi * M + gj] = pack_cell(V, i, j);
     }
 }
 
 // Two steps per launch: moves holds the two steps' move codes, the first in the low byte.
-template <int TR, int TC, int NT, bool UNI>
+template <int TR, int TC, int NT>
 __global__ __launch_bounds__(NT) void k_td_step2(TdArgs a, uint32_t moves) {
-    td_multi_body<TR, TC, NT, UNI, 2>(a, moves, xcd_tile(blockIdx.x, gridDim.x));
+    td_multi_body<TR, TC, NT, 2>(a, moves, xcd_tile(blockIdx.x, gridDim.x));
 }
 
-template <int TR, int TC, int NT, bool UNI>
+template <int TR, int TC, int NT>
 __global__ __launch_bounds__(NT) void k_td_step2_batch(TdArgs a0, const TdBatchEntry* __restrict__ batch, int nbatch,
                                                        const int64_t* __restrict__ step_ctr, int64_t off) {
     const int64_t s = *step_ctr + off;
     TdArgs a = a0;
     int tile;
     const TdBatchEntry& e = td_resolve(a, batch, nbatch, s, 2, tile);
-    td_multi_body<TR, TC, NT, UNI, 2>(a, (uint32_t)e.sched[s] | ((uint32_t)e.sched[s + 1] << 8), tile);
+    td_multi_body<TR, TC, NT, 2>(a, (uint32_t)e.sched[s] | ((uint32_t)e.sched[s + 1] << 8), tile);
 }
 
 // ---- whole domain in one workgroup, every step in one launch ----------------------------
@@ -766,12 +817,7 @@ __global__ __launch_bounds__(kSmallNT) void k_td_small(TdArgs a) {
         if (id < n) {
             a.gen_out[id] = g[k];
             const int i = id / M, j = id % M;
-            uint4 c;
-            c.x = V.g(0, i, j) | (V.g(1, i, j) << 16);
-            c.y = V.g(2, i, j) | (V.g(3, i, j) << 16);
-            c.z = V.g(4, i, j) | (V.g(5, i, j) << 16);
-            c.w = V.g(6, i, j) | (V.g(7, i, j) << 16);
-            a.cells_out[id] = c;
+            a.cells_out[id] = pack_cell(V, i, j);
         }
     }
 }
@@ -826,7 +872,7 @@ __device__ int bfly_state(const int32_t* t, int N, int i, int j) {
 // Walk's host split (TriangleDimerTiler.cpp:104-129): site (a, b) takes eH from t[2a][2b+1],
 // eL from t[2a+1][2b], eR from t[2a+1][2b+1], tD/tU from triangle states (a, 2b)/(a, 2b+1),
 // bH/bL/bR from butterfly states at the eH/eL/eR positions.
-__global__ void k_td_derive(const int32_t* t, int N, uint4* cells) {
+__global__ void k_td_derive(const int32_t* t, int N, uint2* cells) {
     const int M = N / 2;
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= M * M) return;
@@ -835,19 +881,19 @@ __global__ void k_td_derive(const int32_t* t, int N, uint4* cells) {
     const int eh = t[r0 * N + c1], el = t[r1 * N + c0], er = t[r1 * N + c1];
     const int tu = tri_state(t, N, a, c1), tdn = tri_state(t, N, a, c0);
     const int bh = bfly_state(t, N, r0, c1), bl = bfly_state(t, N, r1, c0), br = bfly_state(t, N, r1, c1);
-    cells[id] = make_uint4(eh | (el << 16), er | (tu << 16), tdn | (bh << 16), bl | (br << 16));
+    cells[id] = make_uint2((uint32_t)(eh | el << 4 | er << 8 | tu << 12 | tdn << 19), (uint32_t)(bh | bl << 10 | br << 20));
 }
 
 // Walk's merge (TriangleDimerTiler.cpp:437-451): only the lozenge-move states go back.
-__global__ void k_td_merge(const uint4* cells, int N, int32_t* t) {
+__global__ void k_td_merge(const uint2* cells, int N, int32_t* t) {
     const int M = N / 2;
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= M * M) return;
     const int a = id / M, b = id % M;
-    const uint4 c = cells[id];
-    t[(2 * a) * N + 2 * b + 1] = c.x & 0xffff;
-    t[(2 * a + 1) * N + 2 * b] = c.x >> 16;
-    t[(2 * a + 1) * N + 2 * b + 1] = c.y & 0xffff;
+    const uint32_t c = cells[id].x;
+    t[(2 * a) * N + 2 * b + 1] = c & 15u;
+    t[(2 * a + 1) * N + 2 * b] = (c >> 4) & 15u;
+    t[(2 * a + 1) * N + 2 * b + 1] = (c >> 8) & 15u;
 }
 
 // Tile configurations of the one-step kernel: {rows, cols, threads}.
@@ -857,32 +903,32 @@ struct TdCfg {
 constexpr TdCfg kCfgs[] = {{32, 32, 256}, {64, 64, 1024}, {32, 64, 512}};
 constexpr int kNumCfgs = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-template <int I, bool UNI>
+template <int I>
 hipError_t launch_cfg(const TdArgs& a, uint32_t move, hipStream_t st) {
     constexpr TdCfg c = kCfgs[I];
     const int tiles = ((a.M + c.tr - 1) / c.tr) * ((a.M + c.tc - 1) / c.tc);
-    hipLaunchKernelGGL((k_td_step<c.tr, c.tc, c.nt, UNI>), dim3(tiles), dim3(c.nt), 0, st, a, move);
+    hipLaunchKernelGGL((k_td_step<c.tr, c.tc, c.nt>), dim3(tiles), dim3(c.nt), 0, st, a, move);
     return hipGetLastError();
 }
 
-template <int I, bool UNI>
+template <int I>
 hipError_t launch_cfg2(const TdArgs& a, uint32_t moves, hipStream_t st) {
     constexpr TdCfg c = kCfgs[I];
     const int tiles = ((a.M + c.tr - 1) / c.tr) * ((a.M + c.tc - 1) / c.tc);
-    hipLaunchKernelGGL((k_td_step2<c.tr, c.tc, c.nt, UNI>), dim3(tiles), dim3(c.nt), 0, st, a, moves);
+    hipLaunchKernelGGL((k_td_step2<c.tr, c.tc, c.nt>), dim3(tiles), dim3(c.nt), 0, st, a, moves);
     return hipGetLastError();
 }
 
-template <int I, bool UNI>
+template <int I>
 hipError_t launch_batch_cfg(const TdArgs& a, const TdBatchEntry* batch, int nbatch, int blocks,
                             const int64_t* step_ctr, int64_t off, bool two, int spl, hipStream_t st) {
     constexpr TdCfg c = kCfgs[I];
     if (two)
-        hipLaunchKernelGGL((k_td_step2_batch<c.tr, c.tc, c.nt, UNI>), dim3(blocks), dim3(c.nt), 0, st, a, batch,
-                           nbatch, step_ctr, off);
+        hipLaunchKernelGGL((k_td_step2_batch<c.tr, c.tc, c.nt>), dim3(blocks), dim3(c.nt), 0, st, a, batch, nbatch,
+                           step_ctr, off);
     else
-        hipLaunchKernelGGL((k_td_step_batch<c.tr, c.tc, c.nt, UNI>), dim3(blocks), dim3(c.nt), 0, st, a, batch,
-                           nbatch, step_ctr, off, spl);
+        hipLaunchKernelGGL((k_td_step_batch<c.tr, c.tc, c.nt>), dim3(blocks), dim3(c.nt), 0, st, a, batch, nbatch,
+                           step_ctr, off, spl);
     return hipGetLastError();
 }
 
@@ -903,15 +949,35 @@ bool td_config(int idx, int* tr, int* tc, int* nt) {
     return true;
 }
 
-hipError_t td_launch_derive(const int32_t* t, int N, uint4* cells, hipStream_t st) {
+hipError_t td_launch_derive(const int32_t* t, int N, uint2* cells, hipStream_t st) {
     const int n = (N / 2) * (N / 2);
     hipLaunchKernelGGL(td::k_td_derive, dim3((n + 255) / 256), dim3(256), 0, st, t, N, cells);
     return hipGetLastError();
 }
 
-hipError_t td_launch_merge(const uint4* cells, int N, int32_t* t, hipStream_t st) {
+hipError_t td_launch_merge(const uint2* cells, int N, int32_t* t, hipStream_t st) {
     const int n = (N / 2) * (N / 2);
     hipLaunchKernelGGL(td::k_td_merge, dim3((n + 255) / 256), dim3(256), 0, st, cells, N, t);
+    return hipGetLastError();
+}
+
+int td_draw_blocks(int M) { return td_bit_words(M) * ((M + 3) / 4); }
+
+hipError_t td_launch_draw(const TdArgs& a, bool uni, int k, hipStream_t st) {
+    if (k < 1 || k > kDrawSteps || !a.rbits) return hipErrorInvalidValue;
+    const dim3 grid(td_bit_words(a.M), (a.M + 3) / 4);
+    if (uni) hipLaunchKernelGGL(td::k_td_draw<true>, grid, dim3(256), 0, st, a, k);
+    else hipLaunchKernelGGL(td::k_td_draw<false>, grid, dim3(256), 0, st, a, k);
+    return hipGetLastError();
+}
+
+hipError_t td_launch_draw_batch(const TdArgs& a, bool uni, const TdBatchEntry* batch, int nbatch, int blocks,
+                                const int64_t* step_ctr, int64_t off, int k, hipStream_t st) {
+    if (k < 1 || k > kDrawSteps) return hipErrorInvalidValue;
+    if (uni)
+        hipLaunchKernelGGL(td::k_td_draw_batch<true>, dim3(blocks), dim3(256), 0, st, a, batch, nbatch, step_ctr, off, k);
+    else
+        hipLaunchKernelGGL(td::k_td_draw_batch<false>, dim3(blocks), dim3(256), 0, st, a, batch, nbatch, step_ctr, off, k);
     return hipGetLastError();
 }
 
@@ -922,38 +988,36 @@ hipError_t td_launch_small(const TdArgs& a, bool uni, hipStream_t st) {
     return hipGetLastError();
 }
 
-
 // cfg: a one-step configuration (the tiles).  two: the launch runs two steps (sched[s] and
 // sched[s + 1]); otherwise one.  s = *step_ctr + off.  spl: steps per launch of the run's
 // earlier launches (1 or 2), which fixes the copy parity of step s.
-hipError_t td_launch_step_batch(int cfg, const TdArgs& a, bool uni, const TdBatchEntry* batch, int nbatch, int blocks,
+hipError_t td_launch_step_batch(int cfg, const TdArgs& a, const TdBatchEntry* batch, int nbatch, int blocks,
                                 const int64_t* step_ctr, int64_t off, bool two, int spl, hipStream_t st) {
     if (spl != 1 && spl != 2) return hipErrorInvalidValue;
     switch (cfg) {
-        case 0: return uni ? td::launch_batch_cfg<0, true>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st)
-                           : td::launch_batch_cfg<0, false>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st);
-        case 1: return uni ? td::launch_batch_cfg<1, true>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st)
-                           : td::launch_batch_cfg<1, false>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st);
-        case 2: return uni ? td::launch_batch_cfg<2, true>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st)
-                           : td::launch_batch_cfg<2, false>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st);
+        case 0: return td::launch_batch_cfg<0>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st);
+        case 1: return td::launch_batch_cfg<1>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st);
+        case 2: return td::launch_batch_cfg<2>(a, batch, nbatch, blocks, step_ctr, off, two, spl, st);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t td_launch_step2(int cfg, const TdArgs& a, bool uni, uint32_t moves, hipStream_t st) {
+hipError_t td_launch_step2(int cfg, const TdArgs& a, uint32_t moves, hipStream_t st) {
+    if (!a.rbits || a.plane < 0 || a.plane + 1 >= kDrawSteps) return hipErrorInvalidValue;
     switch (cfg) {
-        case 0: return uni ? td::launch_cfg2<0, true>(a, moves, st) : td::launch_cfg2<0, false>(a, moves, st);
-        case 1: return uni ? td::launch_cfg2<1, true>(a, moves, st) : td::launch_cfg2<1, false>(a, moves, st);
-        case 2: return uni ? td::launch_cfg2<2, true>(a, moves, st) : td::launch_cfg2<2, false>(a, moves, st);
+        case 0: return td::launch_cfg2<0>(a, moves, st);
+        case 1: return td::launch_cfg2<1>(a, moves, st);
+        case 2: return td::launch_cfg2<2>(a, moves, st);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t td_launch_step(int cfg, const TdArgs& a, bool uni, uint32_t move, hipStream_t st) {
+hipError_t td_launch_step(int cfg, const TdArgs& a, uint32_t move, hipStream_t st) {
+    if (!a.rbits || a.plane < 0 || a.plane >= kDrawSteps) return hipErrorInvalidValue;
     switch (cfg) {
-        case 0: return uni ? td::launch_cfg<0, true>(a, move, st) : td::launch_cfg<0, false>(a, move, st);
-        case 1: return uni ? td::launch_cfg<1, true>(a, move, st) : td::launch_cfg<1, false>(a, move, st);
-        case 2: return uni ? td::launch_cfg<2, true>(a, move, st) : td::launch_cfg<2, false>(a, move, st);
+        case 0: return td::launch_cfg<0>(a, move, st);
+        case 1: return td::launch_cfg<1>(a, move, st);
+        case 2: return td::launch_cfg<2>(a, move, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -297,7 +297,7 @@ def midpoint_tiling(ctx, dom: Domain) -> np.ndarray:
 
 
 def trajectories(ctx: rt.Context, dom: Domain, starts, steps: int, every: int, seed: int, chains: int = 16,
-                 ref=None, mean_from: int = -1, settle=None):
+                 ref=None, mean_from: int = -1, settle=None, progress=None):
     """Observables of `chains` independent chains from each start tiling, all in one batch and
     read on the device every `every` chain steps (index 0: the starts).  Returns (per start:
     (mean-height fractions, mean |h - ref| fractions or None) averaged over its chains; the mean
@@ -309,7 +309,10 @@ def trajectories(ctx: rt.Context, dom: Domain, starts, steps: int, every: int, s
     running means) first agree within tol, at step T, then as many steps again, at most `steps`
     in all; the mean field is then taken from T on.  Under a monotone rule with shared random
     numbers, chains from the minimal and the maximal tiling bracket the chains from any other
-    start, so once they agree every start has forgotten where it began."""
+    start, so once they agree every start has forgotten where it began.
+
+    progress(step, total): called after a reading when 20 s have passed since the last call."""
+    import time
     chs, owner = [], []
     for i, t0 in enumerate(starts):
         for k in range(chains):
@@ -346,10 +349,14 @@ def trajectories(ctx: rt.Context, dom: Domain, starts, steps: int, every: int, s
 
     read(0)
     step = 0
+    said = time.monotonic()
     while step < total:
         b.run(every, True)
         step += every
         read(step)
+        if progress is not None and time.monotonic() - said >= 20.0:
+            progress(step, total)
+            said = time.monotonic()
         if settle is not None and not settled:
             i, j, tol = settle
             k = min(5, len(o1[i]))
@@ -374,19 +381,23 @@ def trajectory(ctx: rt.Context, dom: Domain, t0: np.ndarray, steps: int, every: 
 
 
 def compare_starts(ctx: rt.Context, dom: Domain, starts: dict, steps: int, every: int, seed: int,
-                   chains: int = 16, tol: float = 0.01) -> dict:
+                   chains: int = 16, tol: float = 0.01, progress=None) -> dict:
     """Chain steps each start needs to reach equilibrium on both observables.  The chains from
     the minimal and the maximal tiling run beside the starts as brackets; a first pass runs
     until the brackets' mean heights agree within tol / 4 (at most `steps`), then as long
     again, and takes the long-run mean height field h_bar over that second half; a second pass
     repeats the same trajectories (same seeds and length, bit for bit) measuring |h - h_bar|.
     Each observable's equilibrium value is its average over the second half of every
-    trajectory, brackets included."""
+    trajectory, brackets included.  progress(pass, step, total) reports long passes."""
     names = list(starts)
     ts = [starts[n] for n in names] + [dom.tmin, dom.tmax]
     lo_i, hi_i = len(names), len(names) + 1
-    res1, hbar, run = trajectories(ctx, dom, ts, steps, every, seed, chains, settle=(lo_i, hi_i, tol / 4))
-    res2, _, _ = trajectories(ctx, dom, ts, run, every, seed, chains, ref=hbar)
+    def pr(k):
+        return None if progress is None else (lambda step, total: progress(k, step, total))
+
+    res1, hbar, run = trajectories(ctx, dom, ts, steps, every, seed, chains, settle=(lo_i, hi_i, tol / 4),
+                                   progress=pr(1))
+    res2, _, _ = trajectories(ctx, dom, ts, run, every, seed, chains, ref=hbar, progress=pr(2))
     o1 = [r[0] for r in res1]
     o2 = [r[1] for r in res2]
     half = len(o1[0]) // 2

@@ -365,17 +365,39 @@ def test_tridimer_scrambled_params(sctx, golden):
 
 
 def test_tridimer_chain_runs_compose(gctx, golden):
+    """Runs of 1, 9, 150 and 40 steps end where one 200-step walk ends: the tiled path's draw
+    blocks (32 steps of rotation bits) restart with every run, on one- and two-step tiles."""
+    from randomtilings_amd import statics as S
     t0 = golden["tridimer_cone10_0"]
-    for path in [rt.RT_PATH_AUTO, rt.RT_PATH_STEPWISE]:
-        say("tridimer compose", path)
-        ch = gctx.chain(rt.RT_TRIDIMER, 0, t0)
-        ch.set_path(path, -1)
+    t_big = S.call("tridimer.IceCreamCone", args=[150, 0])
+    for t, path, cfg in [(t0, rt.RT_PATH_AUTO, -1), (t0, rt.RT_PATH_STEPWISE, -1), (t_big, rt.RT_PATH_STEPWISE, 0),
+                         (t_big, rt.RT_PATH_STEPWISE, 3)]:
+        say("tridimer compose", path, cfg)
+        ch = gctx.chain(rt.RT_TRIDIMER, 0, t)
+        ch.set_path(path, cfg)
         ch.seed(31)
         for s in [1, 9, 150, 40]:
             ch.run(s)
         g = ch.read()
         ch.close()
-        assert np.array_equal(g, O.tridimer_walk(t0, 200, 31, default_table(BIG)))
+        assert np.array_equal(g, O.tridimer_walk(t, 200, 31, default_table(BIG))), (path, cfg)
+
+
+@pytest.mark.parametrize("path,cfg", [(rt.RT_PATH_AUTO, -1), (rt.RT_PATH_STEPWISE, 0), (rt.RT_PATH_STEPWISE, 4)])
+def test_tridimer_arbitrary_states_parity(gctx, path, cfg):
+    """Uniformly random 4-bit lozenge states (no tiling at all): the derived triangle and
+    butterfly states reach their largest values (up to 126 and 766), which the 64-bit cells
+    must hold without loss; the walk still equals the oracle's int arithmetic."""
+    rng = np.random.default_rng(2024)
+    for N in (60, 180):
+        t0 = rng.integers(0, 16, N * N).astype(np.int32)
+        say("tridimer arbitrary states", path, cfg, N)
+        gctx.set_path(path, cfg)
+        try:
+            g = gctx.tridimer_walk(t0, 97, 123)
+        finally:
+            gctx.set_path(rt.RT_PATH_AUTO, -1)
+        assert np.array_equal(g, O.tridimer_walk(t0, 97, 123, default_table(BIG))), N
 
 
 # ---- coupling from the past -------------------------------------------------------------------
