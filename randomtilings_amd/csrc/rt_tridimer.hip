@@ -25,300 +25,37 @@
 // Domains up to 64 x 64 sites run every step in one launch of one workgroup (the whole
 // domain in LDS, generator states in VGPRs), with only workgroup barriers between phases.
 #include "rt_internal.hpp"
+#include "rt_tdsite.hpp"
 #include "rt_tridimer.hpp"
 
 namespace rt {
 namespace td {
 
-enum Plane { EH = 0, EL = 1, ER = 2, TU = 3, TD = 4, BH = 5, BL = 6, BR = 7 };
-
-struct Move {
-    int type, r1, r2, p1, p2;
-};
-__device__ __forceinline__ Move decode(uint32_t m) {
-    return {(int)(m & 3), (int)((m >> 2) & 3), (int)((m >> 4) & 1), (int)((m >> 5) & 3), (int)((m >> 7) & 1)};
-}
-
-__device__ __forceinline__ int mod3(int x) { return ((x % 3) + 3) % 3; }
-
-// Region of the move lattice held in LDS: eight uint16 planes of `rows` x pitch.
-struct View {
-    uint16_t* base;
-    int pitch, plane;
-    __device__ __forceinline__ int g(int p, int i, int j) const { return base[p * plane + i * pitch + j]; }
-    __device__ __forceinline__ void s(int p, int i, int j, int v) const { base[p * plane + i * pitch + j] = (uint16_t)v; }
-};
-
-// ---- Rotate kernels (triangledimerkernel.cl:4-22, :192-212, :330-352) ------------------
-// The plane a move rotates, and the rotated value of state v at global site (gi, gj) given
-// the draw's low half (rd < 0.5 <=> tempered output < 2^31).
-__device__ __forceinline__ int rotate_plane(const Move& m) {
-    return m.type == 0 ? EH + m.r1 : m.type == 1 ? (m.r2 ? TD : TU) : BH + m.r1;
-}
-__device__ __forceinline__ int rotate_value(const Move& m, int v, int gi, int gj, bool low) {
-    if (!low) return v;
-    if (m.type == 0) {
-        const int t = m.r1;
-        if (((1 - (t & 1)) * gi + ((1 + t) / 2) * gj) % 2 != m.r2) return v;
-        return v == 5 ? 10 : v == 10 ? 5 : v;
-    }
-    if (m.type == 1) {
-        if (mod3(gj - gi) != m.r1) return v;
-        return v == 45 ? 54 : v == 54 ? 45 : v;
-    }
-    const int t = m.r1, b1 = 1 - (t & 1), b2 = (1 + t) / 2, b3 = t & 1;
-    if ((b1 * gi + b2 * gj) % 3 != m.p1) return v;
-    const bool par = ((b1 * gj + b3 * gi) & 1) == m.p2;
-    if (v == 170) return par ? 85 : 170;
-    if (v == 85) return par ? 170 : 85;
-    return v;
-}
-
-// Distance a step's update chain reaches: its phases (one Rotate plus the re-derivations of
-// replay_updates) each read neighbours at distance <= 1; lozenge and triangle moves take four
-// phases, butterfly moves six (kHalo).
-__device__ __forceinline__ int move_reach(const Move& m) { return m.type == 2 ? kHalo : 4; }
-
-// ---- update kernels, per site (local i, j; global gi, gj) -------------------------------
-
-// UpdateLozengesFlipped, :24-34
-__device__ __forceinline__ void loz_flipped(const View& V, int i, int j, int gi, int gj, int t, int c) {
-    if (((1 - (t & 1)) * gi + ((1 + t) / 2) * gj) % 2 == c) return;
-    const int a = EH + t, k = (2 - t) / 2, s = t & 1;
-    V.s(a, i, j,
-        (V.g(a, i - 1, j + s) & 4) / 4 + (V.g(a, i - k, j + 1) & 8) / 4 + (V.g(a, i + 1, j - s) & 1) * 4 +
-            (V.g(a, i + k, j - 1) & 2) * 4);
-}
-
-// UpdateLozenges0/1/2, :36-81
-__device__ __forceinline__ void loz_update(const View& V, int i, int j, int t) {
-    const int A = EH + t, B = EH + (t + 1) % 3, C = EH + (t + 2) % 3;
-    int b = V.g(B, i, j), c = V.g(C, i, j);
-    if (t == 0) {
-        b = (b & ~10) | ((V.g(A, i + 1, j) & 1) * 2 + (V.g(A, i + 1, j - 1) & 1) * 8);
-        c = (c & ~10) | ((V.g(A, i + 1, j) & 2) + (V.g(A, i, j) & 8));
-    } else if (t == 1) {
-        b = (b & ~5) | ((V.g(A, i, j) & 1) + (V.g(A, i + 1, j) & 1) * 4);
-        c = (c & ~5) | ((V.g(A, i - 1, j) & 2) / 2 + (V.g(A, i, j) & 2) * 2);
-    } else {
-        b = (b & ~10) | ((V.g(A, i - 1, j) & 2) + (V.g(A, i, j - 1) & 2) * 4);
-        c = (c & ~5) | ((V.g(A, i, j) & 1) + (V.g(A, i, j - 1) & 4));
-    }
-    V.s(B, i, j, b);
-    V.s(C, i, j, c);
-}
-
-__device__ __forceinline__ int sel(bool c, int v) { return c ? v : 0; }
-
-// UpdateTriangleUFromLozenges, :100-121
-__device__ __forceinline__ void tri_u_from_loz(const View& V, int i, int j) {
-    const int l0 = V.g(EL, i, j), l1 = V.g(EL, i, j + 1), h0 = V.g(EH, i, j), h1 = V.g(EH, i + 1, j);
-    const int L = sel(l0 & 1, 1) + sel(h0 & 8, 2) + sel(h1 & 1, 3);
-    const int R = sel(l1 & 1, 1) + sel(h1 & 2, 2) + sel(l1 & 2, 3);
-    const int U = sel(l1 & 4, 1) + sel(h1 & 8, 2) + sel(h1 & 4, 3);
-    V.s(TU, i, j, L + 4 * R + 16 * U);
-}
-
-__device__ __forceinline__ int pack(int t1, int t4, int t16, int t64, int c) {
-    return t1 + 4 * t4 + 16 * t16 + 64 * t64 + 256 * c;
-}
-
-// UpdateButterflys{H,L,R}FromLozenge, :123-190
-__device__ __forceinline__ void bfly_from_loz(const View& V, int i, int j, int w) {
-    int c, t1, t4, t16, t64;
-    if (w == 0) {
-        const int lh = V.g(EH, i, j), lr1 = V.g(ER, i - 1, j), ll1 = V.g(EL, i - 1, j + 1), lr2 = V.g(ER, i, j),
-                  ll2 = V.g(EL, i, j);
-        c = (ll1 & 4) / 4;
-        t1 = (lr1 & 8) / 8 + 2 * (lr1 & 1) + 3 * (lh & 1);
-        t4 = (ll1 & 1) + (ll1 & 2) + ((lh & 2) / 2) * 3;
-        t16 = (lr2 & 2) / 2 + (lr2 & 4) / 2 + ((lh & 4) / 4) * 3;
-        t64 = (ll2 & 4) / 4 + (ll2 & 8) / 4 + ((lh & 8) / 8) * 3;
-    } else if (w == 1) {
-        const int ll = V.g(EL, i, j), lh1 = V.g(EH, i, j), lr1 = V.g(ER, i, j), lh2 = V.g(EH, i + 1, j - 1),
-                  lr2 = V.g(ER, i, j - 1);
-        c = (lr1 & 8) / 8;
-        t1 = (lh1 & 1) + (lh1 & 2) + 3 * (ll & 1);
-        t4 = (lr1 & 2) / 2 + (lr1 & 4) / 2 + ((ll & 2) / 2) * 3;
-        t16 = (lh2 & 4) / 4 + (lh2 & 8) / 4 + ((ll & 4) / 4) * 3;
-        t64 = (lr2 & 8) / 8 + 2 * (lr2 & 1) + ((ll & 8) / 8) * 3;
-    } else {
-        const int lr = V.g(ER, i, j), lh1 = V.g(EH, i, j), ll1 = V.g(EL, i, j + 1), lh2 = V.g(EH, i + 1, j),
-                  ll2 = V.g(EL, i, j);
-        c = (ll1 & 8) / 8;
-        t1 = (lh1 & 1) + (lh1 & 2) + 3 * (lr & 1);
-        t4 = (ll1 & 1) + (ll1 & 2) + ((lr & 2) / 2) * 3;
-        t16 = (lh2 & 4) / 4 + (lh2 & 8) / 4 + ((lr & 4) / 4) * 3;
-        t64 = (ll2 & 4) / 4 + (ll2 & 8) / 4 + ((lr & 8) / 8) * 3;
-    }
-    V.s(BH + w, i, j, pack(t1, t4, t16, t64, c));
-}
-
-// UpdateTrianglesFlipped0 (:214-234) / UpdateTrianglesFlipped1 (:236-256)
-__device__ __forceinline__ void tri_flipped(const View& V, int i, int j, int gi, int gj, int c, bool down) {
-    const int cl = mod3(gj - gi);
-    if (cl == c) return;
-    const int p = (cl == (c + 1) % 3) ? 1 : 0;
-    // masks and the power-of-two divisors (4 - 3p) * 4, 15p + 1, 4 - 3p as shifts
-    const int t1 = (4 - 3 * p) * 12, s1 = p ? 2 : 4;
-    const int t4 = (15 * p + 1) * 3, s4 = p ? 4 : 0;
-    const int t16 = (4 - 3 * p) * 3, s16 = p ? 0 : 2;
-    const int a = down ? TD : TU;
-    int x1, x4, x16;
-    if (!down) {
-        x1 = V.g(a, i - (1 - p), j - p);
-        x4 = V.g(a, i - p, j + 1);
-        x16 = V.g(a, i + 1, j - 1 + p);
-    } else {
-        x1 = V.g(a, i + 1 - p, j - 1);
-        x4 = V.g(a, i + p, j + 1 - p);
-        x16 = V.g(a, i - 1, j + p);
-    }
-    V.s(a, i, j, ((x1 & t1) >> s1) + (((x4 & t4) >> s4) << 2) + (((x16 & t16) >> s16) << 4));
-}
-
-// UpdateTriangles, :258-281: t = 0 down from up, t = 1 up from down
-__device__ __forceinline__ int tri_digit(int v) {
-    return sel((v & 48) == 16, 1) + sel((v & 3) == 3, 2) + sel((v & 12) == 8, 3);
-}
-__device__ __forceinline__ void tri_update(const View& V, int i, int j, int t) {
-    const int src = t ? TD : TU, dst = t ? TU : TD, p = t ? 1 : -1;
-    V.s(dst, i, j,
-        tri_digit(V.g(src, i, j - 1 + t)) + 4 * tri_digit(V.g(src, i, j + t)) + 16 * tri_digit(V.g(src, i + p, j)));
-}
-
-// UpdateLozenge{H,L,R}FromTriangles, :283-328
-__device__ __forceinline__ void loz_from_tri(const View& V, int i, int j) {
-    const int d = V.g(TD, i, j), uh = V.g(TU, i - 1, j), ul = V.g(TU, i, j - 1), ur = V.g(TU, i, j);
-    V.s(EH, i, j, sel((d & 48) == 32, 1) + sel((d & 48) == 48, 2) + sel((uh & 48) == 48, 4) + sel((uh & 48) == 32, 8));
-    V.s(EL, i, j, sel((ul & 12) == 4, 1) + sel((ul & 12) == 12, 2) + sel((d & 3) == 1, 4) + sel((d & 3) == 2, 8));
-    V.s(ER, i, j, sel((ur & 3) == 1, 1) + sel((d & 12) == 12, 2) + sel((d & 12) == 4, 4) + sel((ur & 3) == 2, 8));
-}
-
-__device__ __forceinline__ int bslice(int t, int gi, int gj) { return t == 0 ? gi % 3 : t == 1 ? gj % 3 : (gi + gj) % 3; }
-
-// UpdateButterflysFlipped{H,L,R}1, :355-375, :454-472, :552-572
-__device__ __forceinline__ void bfly_flipped1(const View& V, int i, int j, int gi, int gj, int t, int p1, int p2) {
-    const int a = BH + t;
-    const int par = (t == 1) ? (gi & 1) : (gj & 1);
-    if (bslice(t, gi, gj) != p1 || par == p2) return;
-    const int c = (V.g(a, i, j) & 256) / 256;
-    int t1, t4, t16, t64;
-    if (t != 2) {
-        const int B1 = (t == 0) ? V.g(a, i, j - 1) : V.g(a, i - 1, j);
-        const int B2 = (t == 0) ? V.g(a, i, j + 1) : V.g(a, i + 1, j);
-        t1 = ((B1 & 12) / 4 == 0) ? 0 : ((B1 & 12) / 4) % 3 + 1;
-        t4 = ((B2 & 3) == 0) ? 0 : ((B2 & 3) + 1) % 3 + 1;
-        t16 = ((B2 & 192) / 64 == 0) ? 0 : ((B2 & 192) / 64) % 3 + 1;
-        t64 = ((B1 & 48) / 16 == 0) ? 0 : ((B1 & 48) / 16 + 1) % 3 + 1;
-    } else {
-        const int B1 = V.g(a, i + 1, j - 1), B2 = V.g(a, i - 1, j + 1);
-        t1 = ((B2 & 192) / 64 == 0) ? 0 : ((B2 & 192) / 64 + 1) % 3 + 1;
-        t4 = ((B2 & 48) / 16 == 0) ? 0 : ((B2 & 48) / 16) % 3 + 1;
-        t16 = ((B1 & 12) / 4 == 0) ? 0 : ((B1 & 12) / 4 + 1) % 3 + 1;
-        t64 = ((B1 & 3) == 0) ? 0 : (B1 & 3) % 3 + 1;
-    }
-    V.s(a, i, j, pack(t1, t4, t16, t64, c));
-}
-
-// UpdateButterflysFlipped{H,L}21 (:377-398, :474-495), R21 (:574-595)
-__device__ __forceinline__ void bfly_flipped21(const View& V, int i, int j, int gi, int gj, int t, int p1) {
-    if (bslice(t, gi, gj) != ((t == 0) ? (p1 + 1) % 3 : (p1 + 2) % 3)) return;
-    const int a = BH + t, Bf = V.g(a, i, j);
-    if (t == 2) {
-        const int B2 = V.g(a, i, j + 1), B3 = V.g(a, i + 1, j);
-        const int c = sel((B3 & 3) == 1, 1);
-        const int t4 = ((Bf & 12) / 4 == 2) ? (Bf & 12) / 4 : (B2 & 3) / 3 + ((B2 & 192) / 192) * 3;
-        const int t16 = ((Bf & 48) / 16 == 1) ? (Bf & 48) / 16 : ((B3 & 3) / 3) * 3 + ((B3 & 192) / 192) * 2;
-        V.s(a, i, j, pack(Bf & 3, t4, t16, (Bf & 192) / 64, c));
-    } else {
-        const int B1 = (t == 0) ? V.g(a, i - 1, j) : V.g(a, i - 1, j + 1);
-        const int B2 = (t == 0) ? V.g(a, i - 1, j + 1) : V.g(a, i, j + 1);
-        const int c = sel((B1 & 48) / 16 == 2, 1);
-        const int t1 = ((Bf & 3) == 2) ? (Bf & 3) : (B1 & 192) / 192 + ((B1 & 48) / 48) * 3;
-        const int t4 = ((Bf & 12) / 4 == 1) ? (Bf & 12) / 4 : ((B2 & 192) / 192) * 3 + ((B2 & 48) / 48) * 2;
-        V.s(a, i, j, pack(t1, t4, (Bf & 48) / 16, (Bf & 192) / 64, c));
-    }
-}
-
-// UpdateButterflysFlipped{H,L,R}22, :400-428, :497-527, :597-627
-__device__ __forceinline__ void bfly_flipped22(const View& V, int i, int j, int gi, int gj, int t, int p1) {
-    if (bslice(t, gi, gj) != ((t == 0) ? (p1 + 2) % 3 : (p1 + 1) % 3)) return;
-    const int a = BH + t, Bf = V.g(a, i, j);
-    if (t == 2) {
-        const int s4 = sel((Bf & 12) / 4 != 2, 1), s16 = sel((Bf & 48) / 16 != 1, 1);
-        const int B1 = V.g(a, i - 1, j), B2 = V.g(a, i, j + 1), B3 = V.g(a, i + 1, j), B4 = V.g(a, i, j - 1);
-        const int c = sel((B1 & 48) / 16 == 1, 1);
-        const int t1 = ((Bf & 3) == 1) ? (Bf & 3) : ((B1 & 48) / 48) * 3 + ((B1 & 12) / 12) * 2;
-        const int t4 = ((B2 & 256) / 256 == 1) ? 2 : s4 * (Bf & 12) / 4;
-        const int t16 = ((B3 & 256) / 256 == 1) ? 1 : s16 * (Bf & 48) / 16;
-        const int t64 = ((Bf & 192) / 64 == 2) ? (Bf & 192) / 64 : (B4 & 48) / 48 + ((B4 & 12) / 12) * 3;
-        V.s(a, i, j, pack(t1, t4, t16, t64, c));
-    } else {
-        const int s1 = sel((Bf & 3) != 2, 1), s4 = sel((Bf & 12) / 4 != 1, 1);
-        const int B1 = (t == 0) ? V.g(a, i - 1, j) : V.g(a, i - 1, j + 1);
-        const int B2 = (t == 0) ? V.g(a, i - 1, j + 1) : V.g(a, i, j + 1);
-        const int B3 = (t == 0) ? V.g(a, i + 1, j) : V.g(a, i + 1, j - 1);
-        const int B4 = (t == 0) ? V.g(a, i + 1, j - 1) : V.g(a, i, j - 1);
-        const int c = sel((B3 & 3) == 2, 1);
-        const int t1 = ((B1 & 256) == 256) ? 2 : s1 * (Bf & 3);
-        const int t4 = ((B2 & 256) == 256) ? 1 : s4 * ((Bf & 12) / 4);
-        const int t16 = ((Bf & 48) / 16 == 2) ? (Bf & 48) / 16 : ((B3 & 3) / 3) * 3 + (B3 & 12) / 12;
-        const int t64 = ((Bf & 192) / 64 == 1) ? (Bf & 192) / 64 : ((B4 & 3) / 3) * 2 + ((B4 & 12) / 12) * 3;
-        V.s(a, i, j, pack(t1, t4, t16, t64, c));
-    }
-}
-
-// UpdateButterflysFlipped{H,L,R}23, :430-452, :529-550, :629-651
-__device__ __forceinline__ void bfly_flipped23(const View& V, int i, int j, int gi, int gj, int t, int p1) {
-    if (bslice(t, gi, gj) != ((t == 0) ? (p1 + 1) % 3 : (p1 + 2) % 3)) return;
-    const int a = BH + t, Bf = V.g(a, i, j);
-    const int c = (Bf & 256) / 256;
-    if (t == 2) {
-        const int s1 = sel((Bf & 3) != 1, 1), s64 = sel((Bf & 192) / 64 != 2, 1);
-        const int B1 = V.g(a, i - 1, j), B4 = V.g(a, i, j - 1);
-        const int t1 = ((B1 & 256) / 256 == 1) ? 1 : s1 * (Bf & 3);
-        const int t64 = ((B4 & 256) / 256 == 1) ? 2 : s64 * (Bf & 192) / 64;
-        V.s(a, i, j, pack(t1, (Bf & 12) / 4, (Bf & 48) / 16, t64, c));
-    } else {
-        const int s16 = sel((Bf & 48) / 16 != 2, 1), s64 = sel((Bf & 192) / 64 != 1, 1);
-        const int B3 = (t == 0) ? V.g(a, i + 1, j) : V.g(a, i + 1, j - 1);
-        const int B4 = (t == 0) ? V.g(a, i + 1, j - 1) : V.g(a, i, j - 1);
-        const int t16 = ((B3 & 256) == 256) ? 2 : s16 * ((Bf & 48) / 16);
-        const int t64 = ((B4 & 256) == 256) ? 1 : s64 * ((Bf & 192) / 64);
-        V.s(a, i, j, pack(Bf & 3, (Bf & 12) / 4, t16, t64, c));
-    }
-}
-
-// UpdateLozengeFromButterflys{H,L,R}, :654-738
-__device__ __forceinline__ void loz_from_bfly(const View& V, int i, int j, int t) {
-    const int F = V.g(BH + t, i, j);
-    const int full = sel((F & 3) == 3, 1) + sel((F & 12) == 12, 2) + sel((F & 48) == 48, 4) + sel((F & 192) == 192, 8);
-    const int c = (F & 256) == 256;
-    if (t == 0) {
-        V.s(EH, i, j, full);
-        V.s(EL, i, j, sel(c, 1) + sel((F & 48) == 48, 2) + sel((F & 192) == 64, 4) + sel((F & 192) == 128, 8));
-        V.s(ER, i, j, sel(c, 1) + sel((F & 48) == 16, 2) + sel((F & 48) == 32, 4) + sel((F & 192) == 192, 8));
-    } else if (t == 1) {
-        V.s(EL, i, j, full);
-        V.s(ER, i, j, sel((F & 3) == 3, 1) + sel((F & 12) == 4, 2) + sel((F & 12) == 8, 4) + sel(c, 8));
-        V.s(EH, i, j, sel((F & 3) == 1, 1) + sel((F & 3) == 2, 2) + sel((F & 12) == 12, 4) + sel(c, 8));
-    } else {
-        V.s(ER, i, j, full);
-        V.s(EH, i, j, sel((F & 3) == 1, 1) + sel((F & 3) == 2, 2) + sel(c, 4) + sel((F & 192) == 192, 8));
-        V.s(EL, i, j, sel((F & 3) == 3, 1) + sel(c, 2) + sel((F & 192) == 64, 4) + sel((F & 192) == 128, 8));
-    }
-}
-
 // ---- phase driver ----------------------------------------------------------------------
 
 // x / d by a multiply-high with m = floor(2^32 / d) + 1: exact for x < 2^17 and d <= 1024
-// (checked exhaustively), which covers every site index of a region (at most 76 x 76 sites).
-// A hardware division would cost dozens of instructions per site and phase.
+// (checked exhaustively), which covers every site index of a region (at most 88 x 88 sites).
+// A hardware division would cost dozens of instructions per site and phase.  The divisors are
+// region and band widths, the same for the whole workgroup and at most kRecipMax, so m comes
+// from a table in constant memory (one scalar load) instead of a 64-bit division per thread.
+constexpr int kRecipMax = 128;
+struct Recips {
+    uint32_t m[kRecipMax + 1];
+};
+constexpr Recips make_recips() {
+    Recips r{};
+    for (int d = 2; d <= kRecipMax; d++) r.m[d] = (uint32_t)(0x100000000ull / (unsigned)d + 1);
+    return r;
+}
+__constant__ Recips kRecips = make_recips();
+
 struct FastDiv {
     uint32_t d, m;
-    __device__ __forceinline__ explicit FastDiv(uint32_t d_)
-        : d(d_), m(d_ > 1 ? (uint32_t)(0x100000000ull / d_ + 1) : 0u) {}
+    // d_: the same in every lane
+    __device__ __forceinline__ explicit FastDiv(uint32_t d_) : d(d_) {
+        const uint32_t du = __builtin_amdgcn_readfirstlane(d_);
+        m = du <= (uint32_t)kRecipMax ? kRecips.m[du] : (uint32_t)(0x100000000ull / du + 1);
+    }
     __device__ __forceinline__ uint32_t div(uint32_t x) const { return d > 1 ? __umulhi(x, m) : x; }
 };
 
@@ -339,16 +76,16 @@ __device__ __forceinline__ void for_sites(const Region& R, int m, F f) {
     const int j0 = max(R.cutL ? m : 0, 1 - R.gj0), j1 = min(R.cols - (R.cutR ? m : 0), R.M - 1 - R.gj0);
     const int w = j1 - j0;
     if (w <= 0 || i1 <= i0) return;
+    const FastDiv fw((uint32_t)w);
     if (w <= NT) {
-        const int band = NT / w;
-        const int r = (int)threadIdx.x / w;
+        const int band = (int)fw.div(NT);
+        const int r = (int)fw.div(threadIdx.x);
         if (r >= band) return;
         const int j = j0 + (int)threadIdx.x - r * w, gj = R.gj0 + j;
         for (int i = i0 + r; i < i1; i += band) f(i, j, R.gi0 + i, gj);
         return;
     }
     const int n = (i1 - i0) * w;
-    const FastDiv fw((uint32_t)w);
     for (int idx = threadIdx.x; idx < n; idx += NT) {
         const int qi = (int)fw.div((uint32_t)idx);
         const int i = i0 + qi, j = j0 + idx - qi * w;
@@ -356,100 +93,132 @@ __device__ __forceinline__ void for_sites(const Region& R, int m, F f) {
     }
 }
 
-// Like for_sites, over the sites with (a * gi + b * gj) mod q == want only (a, b in {0, 1}, q in
-// {2, 3}): the colour classes the Rotate-adjacent phases touch.  Rows are walked with the
-// residue's first column and a stride of q, so threads are not spent on sites the phase skips.
+// Like for_sites, over the sites with (a * gi + b * gj) mod q == want only (a, b in {0, 1} and
+// not both 0, q in {2, 3}, 0 <= want < q): the colour classes the Rotate-adjacent phases touch.
+// Rows are walked with the residue's first column and a stride of q, so threads are not spent
+// on sites the phase skips; the first column of the residue moves by a fixed amount from one
+// row of the thread to its next, so no remainder is taken inside the loop.
 template <int NT, class F>
 __device__ __forceinline__ void for_sites_mod(const Region& R, int m, int a, int b, int q, int want, F f) {
     const int i0 = max(R.cutT ? m : 0, 1 - R.gi0), i1 = min(R.rows - (R.cutB ? m : 0), R.M - 1 - R.gi0);
     const int j0 = max(R.cutL ? m : 0, 1 - R.gj0), j1 = min(R.cols - (R.cutR ? m : 0), R.M - 1 - R.gj0);
     const int w = j1 - j0;
     if (w <= 0 || i1 <= i0) return;
-    if (b == 0) {  // a condition on rows only: every column, rows of one residue
-        const int band = NT / w;
-        const int r = (int)threadIdx.x / w;
+    if (b == 0) {  // a == 1: a condition on rows only, every column of the rows gi = want mod q
+        const FastDiv fw((uint32_t)w);
+        const int band = (int)fw.div(NT);
+        const int r = (int)fw.div(threadIdx.x);
         if (r >= band) return;
         const int j = j0 + (int)threadIdx.x - r * w, gj = R.gj0 + j;
-        int i = i0;
-        while (i < i1 && ((a * (R.gi0 + i)) % q) != want) i++;
-        for (i += r * q; i < i1; i += band * q) f(i, j, R.gi0 + i, gj);
+        const int first = i0 + modq(want - (R.gi0 + i0), q);
+        for (int i = first + r * q; i < i1; i += band * q) f(i, j, R.gi0 + i, gj);
         return;
     }
-    const int wc = (w + q - 1) / q;  // columns of one residue in a row, at most
-    const int band = NT / wc;
-    const int r = (int)threadIdx.x / wc;
+    const int wc = q == 2 ? (w + 1) >> 1 : (int)((uint32_t)(w + 2) / 3u);  // columns of one residue in a row, at most
+    const FastDiv fwc((uint32_t)wc);
+    const int band = (int)fwc.div(NT);
+    const int r = (int)fwc.div(threadIdx.x);
     if (r >= band) return;
     const int k = (int)threadIdx.x - r * wc;
+    // off: first column j >= j0 with (a gi + gj) mod q == want, less j0; the next row of this
+    // thread (gi + band) moves it by -a band mod q
+    int off = modq(want - a * (R.gi0 + i0 + r) - (R.gj0 + j0), q);
+    const int doff = modq(-a * band, q);
     for (int i = i0 + r; i < i1; i += band) {
-        const int gi = R.gi0 + i;
-        // first column j >= j0 with (a gi + gj) mod q == want
-        const int j = j0 + (((want - a * gi - (R.gj0 + j0)) % q) + q) % q + q * k;
-        if (j < j1) f(i, j, gi, R.gj0 + j);
+        const int j = j0 + off + q * k;
+        if (j < j1) f(i, j, R.gi0 + i, R.gj0 + j);
+        off += doff;
+        off -= off >= q ? q : 0;
     }
+}
+
+// Like for_sites, over pairs of sites (words of the pair view): each thread takes one pair
+// column of a band of rows.  A pair may reach one site past [j0, j1): a site of the margin
+// that no later phase reads (its inputs were already out of date), or a site outside the
+// domain's update range 0 < gj < M - 1, which the write mask wm excludes (its value is kept).
+template <int NT, class F>
+__device__ __forceinline__ void for_pairs(const Region& R, int m, F f) {
+    const int i0 = max(R.cutT ? m : 0, 1 - R.gi0), i1 = min(R.rows - (R.cutB ? m : 0), R.M - 1 - R.gi0);
+    const int j0 = max(R.cutL ? m : 0, 1 - R.gj0), j1 = min(R.cols - (R.cutR ? m : 0), R.M - 1 - R.gj0);
+    const int q0 = j0 >> 1, w = ((j1 + 1) >> 1) - q0;  // pairs covering [j0, j1)
+    if (w <= 0 || i1 <= i0) return;
+    const FastDiv fw((uint32_t)w);
+    const int band = (int)fw.div(NT);
+    const int r = (int)fw.div(threadIdx.x);
+    if (r >= band) return;
+    const int jp = q0 + (int)threadIdx.x - r * w, gj = R.gj0 + 2 * jp;
+    const uint32_t wm = (gj >= 1 && gj <= R.M - 2 ? 0x0000ffffu : 0u) | (gj + 1 <= R.M - 2 ? 0xffff0000u : 0u);
+    for (int i = i0 + r; i < i1; i += band) f(i, jp, R.gi0 + i, gj, wm);
 }
 
 // The update kernels of one step after its Rotate (TriangleDimerTiler.cpp:217-338).  m0: the
 // margin earlier steps of the same launch have already used up (their phases' inputs ran out
-// there), added to every phase's own.
+// there), added to every phase's own.  The phases over every site run on pairs of sites; the
+// ones over one colour class (a half or a third of the sites) run per site.
 template <int NT>
 __device__ void replay_updates(const View& V, const Region& R, const Move& mv, const int m0 = 0) {
+    const PView P = V.pairs();
     if (mv.type == 0) {
         const int t = mv.r1, c = mv.r2;
         // UpdateLozengesFlipped rewrites the class ((1 - (t & 1)) gi + ((1 + t) / 2) gj) mod 2 != c
         for_sites_mod<NT>(R, m0 + 1, 1 - (t & 1), (1 + t) / 2, 2, 1 - c,
                           [&](int i, int j, int gi, int gj) { loz_flipped(V, i, j, gi, gj, t, c); });
         __syncthreads();
-        for_sites<NT>(R, m0 + 2, [&](int i, int j, int, int) { loz_update(V, i, j, t); });
+        for_pairs<NT>(R, m0 + 2, [&](int i, int jp, int, int, uint32_t wm) { loz_update_p(P, i, jp, t, wm); });
         __syncthreads();
-        for_sites<NT>(R, m0 + 3, [&](int i, int j, int, int) {
-            tri_u_from_loz(V, i, j);
-            bfly_from_loz(V, i, j, 0);
-            bfly_from_loz(V, i, j, 1);
-            bfly_from_loz(V, i, j, 2);
+        for_pairs<NT>(R, m0 + 3, [&](int i, int jp, int, int, uint32_t wm) {
+            put(P, TU, i, jp, tri_u_p(P, i, jp), wm);
+            put(P, BH, i, jp, bfly_from_loz_p(P, i, jp, 0), wm);
+            put(P, BL, i, jp, bfly_from_loz_p(P, i, jp, 1), wm);
+            put(P, BR, i, jp, bfly_from_loz_p(P, i, jp, 2), wm);
         });
         __syncthreads();
-        for_sites<NT>(R, m0 + 4, [&](int i, int j, int, int) { tri_update(V, i, j, 0); });
+        for_pairs<NT>(R, m0 + 4, [&](int i, int jp, int, int, uint32_t wm) { put(P, TD, i, jp, tri_update_p(P, i, jp, 0), wm); });
     } else if (mv.type == 1) {
         const int c = mv.r1;
         const bool down = mv.r2 == 1;
         for_sites<NT>(R, m0 + 1, [&](int i, int j, int gi, int gj) { tri_flipped(V, i, j, gi, gj, c, down); });
         __syncthreads();
-        for_sites<NT>(R, m0 + 2, [&](int i, int j, int, int) { tri_update(V, i, j, down ? 1 : 0); });
+        for_pairs<NT>(R, m0 + 2, [&](int i, int jp, int, int, uint32_t wm) {
+            put(P, down ? TU : TD, i, jp, tri_update_p(P, i, jp, down ? 1 : 0), wm);
+        });
         __syncthreads();
-        for_sites<NT>(R, m0 + 3, [&](int i, int j, int, int) { loz_from_tri(V, i, j); });
+        for_pairs<NT>(R, m0 + 3, [&](int i, int jp, int, int, uint32_t wm) { loz_from_tri_p(P, i, jp, wm); });
         __syncthreads();
-        for_sites<NT>(R, m0 + 4, [&](int i, int j, int, int) {
-            bfly_from_loz(V, i, j, 0);
-            bfly_from_loz(V, i, j, 1);
-            bfly_from_loz(V, i, j, 2);
+        for_pairs<NT>(R, m0 + 4, [&](int i, int jp, int, int, uint32_t wm) {
+            put(P, BH, i, jp, bfly_from_loz_p(P, i, jp, 0), wm);
+            put(P, BL, i, jp, bfly_from_loz_p(P, i, jp, 1), wm);
+            put(P, BR, i, jp, bfly_from_loz_p(P, i, jp, 2), wm);
         });
     } else {
         const int t = mv.r1, p1 = mv.p1, p2 = mv.p2;
         // the slice classes bslice(t) = (b1 gi + b2 gj) mod 3 these phases rewrite
         const int sa = t == 1 ? 0 : 1, sb = t == 0 ? 0 : 1;
         for_sites_mod<NT>(R, m0 + 1, sa, sb, 3, p1,
-                          [&](int i, int j, int gi, int gj) { bfly_flipped1(V, i, j, gi, gj, t, p1, p2); });
+                          [&](int i, int j, int gi, int gj) { bfly_flipped1(V, i, j, gi, gj, t, p2); });
         __syncthreads();
         for_sites_mod<NT>(R, m0 + 2, sa, sb, 3, t == 0 ? (p1 + 1) % 3 : (p1 + 2) % 3,
-                          [&](int i, int j, int gi, int gj) { bfly_flipped21(V, i, j, gi, gj, t, p1); });
+                          [&](int i, int j, int, int) { bfly_flipped21(V, i, j, t); });
         __syncthreads();
         for_sites_mod<NT>(R, m0 + 3, sa, sb, 3, t == 0 ? (p1 + 2) % 3 : (p1 + 1) % 3,
-                          [&](int i, int j, int gi, int gj) { bfly_flipped22(V, i, j, gi, gj, t, p1); });
+                          [&](int i, int j, int, int) { bfly_flipped22(V, i, j, t); });
         __syncthreads();
-        // a site's lozenge states depend only on its own (just final) butterfly state
-        for_sites<NT>(R, m0 + 4, [&](int i, int j, int gi, int gj) {
-            bfly_flipped23(V, i, j, gi, gj, t, p1);
-            loz_from_bfly(V, i, j, t);
+        // UpdateButterflysFlipped23 on its slice, then every site's lozenge states from its own
+        // (just final) butterfly state
+        const int ts = t == 0 ? (p1 + 1) % 3 : (p1 + 2) % 3;
+        for_pairs<NT>(R, m0 + 4, [&](int i, int jp, int gi, int gj, uint32_t wm) {
+            const uint32_t sm = (bslice(t, gi, gj) == ts ? 0x0000ffffu : 0u) | (bslice(t, gi, gj + 1) == ts ? 0xffff0000u : 0u);
+            bfly23_loz_from_bfly_p(P, i, jp, t, sm, wm);
         });
         __syncthreads();
-        for_sites<NT>(R, m0 + 5, [&](int i, int j, int, int) {
-            if (t != 0) bfly_from_loz(V, i, j, 0);
-            if (t != 1) bfly_from_loz(V, i, j, 1);
-            if (t != 2) bfly_from_loz(V, i, j, 2);
-            tri_u_from_loz(V, i, j);
+        for_pairs<NT>(R, m0 + 5, [&](int i, int jp, int, int, uint32_t wm) {
+            if (t != 0) put(P, BH, i, jp, bfly_from_loz_p(P, i, jp, 0), wm);
+            if (t != 1) put(P, BL, i, jp, bfly_from_loz_p(P, i, jp, 1), wm);
+            if (t != 2) put(P, BR, i, jp, bfly_from_loz_p(P, i, jp, 2), wm);
+            put(P, TU, i, jp, tri_u_p(P, i, jp), wm);
         });
         __syncthreads();
-        for_sites<NT>(R, m0 + 6, [&](int i, int j, int, int) { tri_update(V, i, j, 0); });
+        for_pairs<NT>(R, m0 + 6, [&](int i, int jp, int, int, uint32_t wm) { put(P, TD, i, jp, tri_update_p(P, i, jp, 0), wm); });
     }
 }
 
@@ -539,16 +308,19 @@ __global__ __launch_bounds__(256) void k_td_draw_batch(TdArgs a, const TdBatchEn
     td_draw_row<UNI>(a, e.gens[e.g0 ^ par], e.gens[e.g0 ^ par ^ 1], e.M, r % rw, gi, k, e.rbits);
 }
 
-// Rotation bit of site (gi, gj) in plane p.
-__device__ __forceinline__ const uint64_t* bit_word(const TdArgs& a, int p, int gi, int gj) {
-    return a.rbits + ((size_t)p * a.M + gi) * td_bit_words(a.M) + (gj >> 6);
+// The 32-bit half of the rotation-bit word holding site (gi, gj) in plane p (bit gj & 31).
+__device__ __forceinline__ const uint32_t* bit_word(const TdArgs& a, int p, int gi, int gj) {
+    return reinterpret_cast<const uint32_t*>(a.rbits + ((size_t)p * a.M + gi) * td_bit_words(a.M) + (gj >> 6)) +
+           ((gj >> 5) & 1);
 }
 
 // ---- one step per launch, tiles + halo --------------------------------------------------
 template <int TR, int TC, int NT>
 __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t move, const int tile) {
     constexpr int RR = TR + 2 * kHalo, RC = TC + 2 * kHalo;
-    __shared__ uint16_t lds[8 * RR * RC];
+    static_assert(RC % 2 == 0, "the pair view needs an even pitch");
+    // 8 planes of 16-bit sites, read as pairs too (4-byte aligned), a guard of two words each side
+    __shared__ uint32_t lds[8 * RR * RC / 2 + 4];
     const int M = a.M;
     const int tiles_c = (M + TC - 1) / TC;
     const int ti = tile / tiles_c, tj = tile % tiles_c;
@@ -568,7 +340,7 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     R.cutL = R.gj0 > 0;
     R.cutB = gi1 < M;
     R.cutR = gj1 < M;
-    const View V{lds, RC, RR * RC};
+    const View V{reinterpret_cast<u16a*>(lds + 2), RC, RR * RC};
     const int rp = rotate_plane(mv);
     const int n = R.rows * R.cols;
     const int oi1 = min(M, oi + TR), oj1 = min(M, oj + TC);
@@ -577,7 +349,7 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     constexpr int SPT = (RR * RC + NT - 1) / NT;  // sites per thread
     const FastDiv fc((uint32_t)R.cols);
     uint2 cin[SPT];
-    uint64_t bw[SPT];
+    uint32_t bw[SPT];
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
         const int idx = threadIdx.x + q * NT;
@@ -596,7 +368,7 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
             const int gi = R.gi0 + i, gj = R.gj0 + j;
             int v[8];
             unpack_cell(cin[q], v);
-            v[rp] = rotate_value(mv, v[rp], gi, gj, (bw[q] >> (gj & 63)) & 1u);
+            v[rp] = rotate_value(mv, v[rp], gi, gj, (bw[q] >> (gj & 31)) & 1u);
 #pragma unroll
             for (int p = 0; p < 8; p++) V.s(p, i, j, v[p]);
         }
@@ -657,7 +429,9 @@ template <int TR, int TC, int NT, int S>
 __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t moves, const int tile) {
     constexpr int HMAX = S * kHalo;
     constexpr int RR = TR + 2 * HMAX, RC = TC + 2 * HMAX;
-    __shared__ uint16_t lds[8 * RR * RC];
+    static_assert(RC % 2 == 0, "the pair view needs an even pitch");
+    // 8 planes of 16-bit sites, read as pairs too (4-byte aligned), a guard of two words each side
+    __shared__ uint32_t lds[8 * RR * RC / 2 + 4];
     const int M = a.M;
     const int tiles_c = (M + TC - 1) / TC;
     const int ti = tile / tiles_c, tj = tile % tiles_c;
@@ -677,7 +451,7 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
     R.cutL = R.gj0 > 0;
     R.cutB = gi1 < M;
     R.cutR = gj1 < M;
-    const View V{lds, RC, RR * RC};
+    const View V{reinterpret_cast<u16a*>(lds + 2), RC, RR * RC};
     const int n = R.rows * R.cols;
     const int oi1 = min(M, oi + TR), oj1 = min(M, oj + TC);
     constexpr int SPT = (RR * RC + NT - 1) / NT;  // sites per thread
@@ -708,7 +482,7 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
     for (int s = 0; s < S; s++) {
         const Move mv = decode((moves >> (8 * s)) & 0xffu);
         const int rp = rotate_plane(mv);
-        uint64_t bw[SPT];
+        uint32_t bw[SPT];
 #pragma unroll
         for (int q = 0; q < SPT; q++) {
             const int idx = threadIdx.x + q * NT;
@@ -725,7 +499,7 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
                 const int i = (int)fc.div((uint32_t)idx), j = idx - i * R.cols;
                 const int gi = R.gi0 + i, gj = R.gj0 + j;
                 const int v = V.g(rp, i, j);
-                const int w = rotate_value(mv, v, gi, gj, (bw[q] >> (gj & 63)) & 1u);
+                const int w = rotate_value(mv, v, gi, gj, (bw[q] >> (gj & 31)) & 1u);
                 if (w != v) V.s(rp, i, j, w);
             }
         }
@@ -766,9 +540,10 @@ constexpr int kSmallPer = (kSmallMax * kSmallMax + kSmallNT - 1) / kSmallNT;  //
 
 template <bool UNI>
 __global__ __launch_bounds__(kSmallNT) void k_td_small(TdArgs a) {
-    __shared__ uint16_t lds[8 * kSmallMax * kSmallMax];
-    const int M = a.M, n = M * M;
-    const View V{lds, M, M * M};
+    // 8 planes of M rows of an even pitch (the pair view), a guard of two words each side
+    __shared__ uint32_t lds[8 * kSmallMax * kSmallMax / 2 + 4];
+    const int M = a.M, n = M * M, pitch = (M + 1) & ~1;
+    const View V{reinterpret_cast<u16a*>(lds + 2), pitch, M * pitch};
     Region R{0, 0, M, M, M, false, false, false, false};
     uint4 g[kSmallPer];
     uint32_t pm1[kSmallPer], pm2[kSmallPer], ptm[kSmallPer];
