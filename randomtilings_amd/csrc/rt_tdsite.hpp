@@ -481,5 +481,30 @@ RT_TDHD void bfly23_loz_from_bfly_p(const PView& P, int i, int jp, int t, uint32
     }
 }
 
+// UpdateTrianglesFlipped0/1 (:214-256) on a pair: the sites of class cl = (gj - gi) mod 3 != c
+// take three two-bit fields of neighbours whose positions depend on p = [cl == c + 1]; both
+// placements are computed for the whole word and each field keeps its own (class-c fields keep
+// their value).  gi, gj: the low site.
+RT_TDHD void tri_flipped_p(const PView& P, int i, int jp, int gi, int gj, int c, bool down, uint32_t wm) {
+    const int a = down ? TD : TU;
+    const int cl0 = mod3(gj - gi), cl1 = cl0 == 2 ? 0 : cl0 + 1, c1 = c == 2 ? 0 : c + 1;
+    const uint32_t act = (cl0 != c ? 0x0000ffffu : 0u) | (cl1 != c ? 0xffff0000u : 0u);
+    if (!act) return;
+    const uint32_t one = (cl0 == c1 ? 0x0000ffffu : 0u) | (cl1 == c1 ? 0xffff0000u : 0u);  // fields with p = 1
+    uint32_t y1, y4, y16, z1, z4, z16;  // p = 0 and p = 1 neighbours
+    if (!down) {
+        y1 = P.w(a, i - 1, jp), y4 = P.n(a, i, jp, 1), y16 = P.n(a, i + 1, jp, -1);
+        z1 = P.n(a, i, jp, -1), z4 = P.n(a, i - 1, jp, 1), z16 = P.w(a, i + 1, jp);
+    } else {
+        y1 = P.n(a, i + 1, jp, -1), y4 = P.n(a, i, jp, 1), y16 = P.w(a, i - 1, jp);
+        z1 = P.n(a, i, jp, -1), z4 = P.w(a, i + 1, jp), z16 = P.n(a, i - 1, jp, 1);
+    }
+    const uint32_t v0 = ((y1 >> 4) & P16(3)) | ((y4 & P16(3)) << 2) | (((y16 >> 2) & P16(3)) << 4);
+    const uint32_t v1 = ((z1 >> 2) & P16(3)) | (((z4 >> 4) & P16(3)) << 2) | ((z16 & P16(3)) << 4);
+    const uint32_t old = P.w(a, i, jp);
+    const uint32_t nv = (((v1 & one) | (v0 & ~one)) & act) | (old & ~act);
+    put(P, a, i, jp, nv, wm);
+}
+
 }  // namespace td
 }  // namespace rt

@@ -153,8 +153,8 @@ __device__ __forceinline__ void for_pairs(const Region& R, int m, F f) {
 
 // The update kernels of one step after its Rotate (TriangleDimerTiler.cpp:217-338).  m0: the
 // margin earlier steps of the same launch have already used up (their phases' inputs ran out
-// there), added to every phase's own.  The phases over every site run on pairs of sites; the
-// ones over one colour class (a half or a third of the sites) run per site.
+// there), added to every phase's own.  The phases over every site (or two thirds of them) run on
+// pairs of sites; the ones over a half or a third of the sites run per site on those sites only.
 template <int NT>
 __device__ void replay_updates(const View& V, const Region& R, const Move& mv, const int m0 = 0) {
     const PView P = V.pairs();
@@ -177,7 +177,7 @@ __device__ void replay_updates(const View& V, const Region& R, const Move& mv, c
     } else if (mv.type == 1) {
         const int c = mv.r1;
         const bool down = mv.r2 == 1;
-        for_sites<NT>(R, m0 + 1, [&](int i, int j, int gi, int gj) { tri_flipped(V, i, j, gi, gj, c, down); });
+        for_pairs<NT>(R, m0 + 1, [&](int i, int jp, int gi, int gj, uint32_t wm) { tri_flipped_p(P, i, jp, gi, gj, c, down, wm); });
         __syncthreads();
         for_pairs<NT>(R, m0 + 2, [&](int i, int jp, int, int, uint32_t wm) {
             put(P, down ? TU : TD, i, jp, tri_update_p(P, i, jp, down ? 1 : 0), wm);
@@ -233,6 +233,32 @@ __device__ __forceinline__ uint2 pack_cell(const View& V, int i, int j) {
     c.x = V.g(EH, i, j) | V.g(EL, i, j) << 4 | V.g(ER, i, j) << 8 | V.g(TU, i, j) << 12 | V.g(TD, i, j) << 19;
     c.y = V.g(BH, i, j) | V.g(BL, i, j) << 10 | V.g(BR, i, j) << 20;
     return c;
+}
+// The cells of a pair of sites from its eight plane words (field f = 0: the low site).
+__device__ __forceinline__ uint2 pack_cell_pair(const uint32_t w[8], int f) {
+    const int sh = 16 * f;
+    auto g = [&](int p) { return (w[p] >> sh) & 0xffffu; };
+    return make_uint2(g(EH) | g(EL) << 4 | g(ER) << 8 | g(TU) << 12 | g(TD) << 19, g(BH) | g(BL) << 10 | g(BR) << 20);
+}
+
+// Write the owned tile [oi, oi1) x [oj, oj1) from the planes, a pair of sites per thread and
+// iteration (the region starts at an even column, so every owned pair is one word).
+template <int NT>
+__device__ __forceinline__ void store_owned(const TdArgs& a, const PView& P, const Region& R, int oi, int oj, int oi1,
+                                            int oj1) {
+    const int ow = oj1 - oj, opc = (ow + 1) >> 1, on = (oi1 - oi) * opc;
+    const int jo = (oj - R.gj0) >> 1;
+    const FastDiv fo((uint32_t)opc);
+    for (int idx = threadIdx.x; idx < on; idx += NT) {
+        const int qi = (int)fo.div((uint32_t)idx), k = idx - qi * opc;
+        const int gi = oi + qi, gj = oj + 2 * k;
+        uint32_t w[8];
+#pragma unroll
+        for (int p = 0; p < 8; p++) w[p] = P.w(p, gi - R.gi0, jo + k);
+        uint2* dst = a.cells_out + (size_t)gi * a.M + gj;
+        dst[0] = pack_cell_pair(w, 0);
+        if (2 * k + 1 < ow) dst[1] = pack_cell_pair(w, 1);
+    }
 }
 
 template <bool UNI>
@@ -342,49 +368,49 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     R.cutR = gj1 < M;
     const View V{reinterpret_cast<u16a*>(lds + 2), RC, RR * RC};
     const int rp = rotate_plane(mv);
-    const int n = R.rows * R.cols;
     const int oi1 = min(M, oi + TR), oj1 = min(M, oj + TC);
-    // load the region's cells and rotation bits, rotate.  Every load of the thread's sites is
-    // issued before the first is used, so the loads overlap instead of waiting one by one.
-    constexpr int SPT = (RR * RC + NT - 1) / NT;  // sites per thread
-    const FastDiv fc((uint32_t)R.cols);
-    uint2 cin[SPT];
-    uint32_t bw[SPT];
+    // load the region's cells and rotation bits a pair of sites at a time (the region starts at
+    // an even column: tile origins and halo widths are even), rotate, store the pair's plane
+    // words.  Every load of the thread's pairs is issued before the first is used, so the loads
+    // overlap instead of waiting one by one.
+    const PView P = V.pairs();
+    constexpr int PPT = (RR * (RC / 2) + NT - 1) / NT;  // pairs per thread
+    const int pc = (R.cols + 1) >> 1, np = R.rows * pc;
+    const FastDiv fp((uint32_t)pc);
+    uint2 c0[PPT], c1[PPT];
+    uint32_t bw[PPT];
 #pragma unroll
-    for (int q = 0; q < SPT; q++) {
+    for (int q = 0; q < PPT; q++) {
         const int idx = threadIdx.x + q * NT;
-        if (idx < n) {
-            const int qi = (int)fc.div((uint32_t)idx);
-            const int gi = R.gi0 + qi, gj = R.gj0 + idx - qi * R.cols;
-            cin[q] = a.cells_in[(size_t)gi * M + gj];
-            bw[q] = *bit_word(a, a.plane, gi, gj);
+        if (idx < np) {
+            const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
+            const int gi = R.gi0 + qi, gj = R.gj0 + 2 * jp;
+            const uint2* src = a.cells_in + (size_t)gi * M + gj;
+            c0[q] = src[0];
+            c1[q] = 2 * jp + 1 < R.cols ? src[1] : make_uint2(0u, 0u);
+            bw[q] = *bit_word(a, a.plane, gi, gj);  // gj even: gj + 1 is in the same half
         }
     }
 #pragma unroll
-    for (int q = 0; q < SPT; q++) {
+    for (int q = 0; q < PPT; q++) {
         const int idx = threadIdx.x + q * NT;
-        if (idx < n) {
-            const int i = (int)fc.div((uint32_t)idx), j = idx - i * R.cols;
-            const int gi = R.gi0 + i, gj = R.gj0 + j;
-            int v[8];
-            unpack_cell(cin[q], v);
-            v[rp] = rotate_value(mv, v[rp], gi, gj, (bw[q] >> (gj & 31)) & 1u);
+        if (idx < np) {
+            const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
+            const int gi = R.gi0 + qi, gj = R.gj0 + 2 * jp;
+            int v0[8], v1[8];
+            unpack_cell(c0[q], v0);
+            unpack_cell(c1[q], v1);
+            const uint32_t b = bw[q] >> (gj & 31);
+            v0[rp] = rotate_value(mv, v0[rp], gi, gj, b & 1u);
+            v1[rp] = rotate_value(mv, v1[rp], gi, gj + 1, (b >> 1) & 1u);
 #pragma unroll
-            for (int p = 0; p < 8; p++) V.s(p, i, j, v[p]);
+            for (int p = 0; p < 8; p++) P.set(p, qi, jp, (uint32_t)v0[p] | ((uint32_t)v1[p] << 16));
         }
     }
     __syncthreads();
     replay_updates<NT>(V, R, mv);
     __syncthreads();
-    // write the owned tile
-    const int ow = oj1 - oj, on = (oi1 - oi) * ow;
-    const FastDiv fo((uint32_t)ow);
-    for (int idx = threadIdx.x; idx < on; idx += NT) {
-        const int qi = (int)fo.div((uint32_t)idx);
-        const int gi = oi + qi, gj = oj + idx - qi * ow;
-        const int i = gi - R.gi0, j = gj - R.gj0;
-        a.cells_out[(size_t)gi * M + gj] = pack_cell(V, i, j);
-    }
+    store_owned<NT>(a, P, R, oi, oj, oi1, oj1);
 }
 
 template <int TR, int TC, int NT>
@@ -452,29 +478,34 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
     R.cutB = gi1 < M;
     R.cutR = gj1 < M;
     const View V{reinterpret_cast<u16a*>(lds + 2), RC, RR * RC};
-    const int n = R.rows * R.cols;
     const int oi1 = min(M, oi + TR), oj1 = min(M, oj + TC);
-    constexpr int SPT = (RR * RC + NT - 1) / NT;  // sites per thread
-    const FastDiv fc((uint32_t)R.cols);
+    // pairs of sites as in td_step_body
+    const PView P = V.pairs();
+    constexpr int PPT = (RR * (RC / 2) + NT - 1) / NT;  // pairs per thread
+    const int pc = (R.cols + 1) >> 1, np = R.rows * pc;
+    const FastDiv fp((uint32_t)pc);
     {
-        uint2 cin[SPT];
+        uint2 c0[PPT], c1[PPT];
 #pragma unroll
-        for (int q = 0; q < SPT; q++) {
+        for (int q = 0; q < PPT; q++) {
             const int idx = threadIdx.x + q * NT;
-            if (idx < n) {
-                const int qi = (int)fc.div((uint32_t)idx);
-                cin[q] = a.cells_in[(size_t)(R.gi0 + qi) * M + (R.gj0 + idx - qi * R.cols)];
+            if (idx < np) {
+                const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
+                const uint2* src = a.cells_in + (size_t)(R.gi0 + qi) * M + (R.gj0 + 2 * jp);
+                c0[q] = src[0];
+                c1[q] = 2 * jp + 1 < R.cols ? src[1] : make_uint2(0u, 0u);
             }
         }
 #pragma unroll
-        for (int q = 0; q < SPT; q++) {
+        for (int q = 0; q < PPT; q++) {
             const int idx = threadIdx.x + q * NT;
-            if (idx < n) {
-                const int i = (int)fc.div((uint32_t)idx), j = idx - i * R.cols;
-                int v[8];
-                unpack_cell(cin[q], v);
+            if (idx < np) {
+                const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
+                int v0[8], v1[8];
+                unpack_cell(c0[q], v0);
+                unpack_cell(c1[q], v1);
 #pragma unroll
-                for (int p = 0; p < 8; p++) V.s(p, i, j, v[p]);
+                for (int p = 0; p < 8; p++) P.set(p, qi, jp, (uint32_t)v0[p] | ((uint32_t)v1[p] << 16));
             }
         }
     }
@@ -482,25 +513,27 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
     for (int s = 0; s < S; s++) {
         const Move mv = decode((moves >> (8 * s)) & 0xffu);
         const int rp = rotate_plane(mv);
-        uint32_t bw[SPT];
+        uint32_t bw[PPT];
 #pragma unroll
-        for (int q = 0; q < SPT; q++) {
+        for (int q = 0; q < PPT; q++) {
             const int idx = threadIdx.x + q * NT;
-            if (idx < n) {
-                const int i = (int)fc.div((uint32_t)idx);
-                bw[q] = *bit_word(a, a.plane + s, R.gi0 + i, R.gj0 + idx - i * R.cols);
+            if (idx < np) {
+                const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
+                bw[q] = *bit_word(a, a.plane + s, R.gi0 + qi, R.gj0 + 2 * jp);
             }
         }
         if (s > 0) __syncthreads();  // the previous step's last phase wrote neighbours' states
 #pragma unroll
-        for (int q = 0; q < SPT; q++) {
+        for (int q = 0; q < PPT; q++) {
             const int idx = threadIdx.x + q * NT;
-            if (idx < n) {
-                const int i = (int)fc.div((uint32_t)idx), j = idx - i * R.cols;
-                const int gi = R.gi0 + i, gj = R.gj0 + j;
-                const int v = V.g(rp, i, j);
-                const int w = rotate_value(mv, v, gi, gj, (bw[q] >> (gj & 31)) & 1u);
-                if (w != v) V.s(rp, i, j, w);
+            if (idx < np) {
+                const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
+                const int gi = R.gi0 + qi, gj = R.gj0 + 2 * jp;
+                const uint32_t w = P.w(rp, qi, jp), b = bw[q] >> (gj & 31);
+                const uint32_t lo = (uint32_t)rotate_value(mv, (int)(w & 0xffffu), gi, gj, b & 1u);
+                const uint32_t hi = (uint32_t)rotate_value(mv, (int)(w >> 16), gi, gj + 1, (b >> 1) & 1u);
+                const uint32_t nw = lo | (hi << 16);
+                if (nw != w) P.set(rp, qi, jp, nw);
             }
         }
         __syncthreads();
@@ -508,14 +541,7 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
         used += move_reach(mv);
     }
     __syncthreads();
-    const int ow = oj1 - oj, on = (oi1 - oi) * ow;
-    const FastDiv fo((uint32_t)ow);
-    for (int idx = threadIdx.x; idx < on; idx += NT) {
-        const int qi = (int)fo.div((uint32_t)idx);
-        const int gi = oi + qi, gj = oj + idx - qi * ow;
-        const int i = gi - R.gi0, j = gj - R.gj0;
-        a.cells_out[(size_t)gi * M + gj] = pack_cell(V, i, j);
-    }
+    store_owned<NT>(a, P, R, oi, oj, oi1, oj1);
 }
 
 // Two steps per launch: moves holds the two steps' move codes, the first in the low byte.
