@@ -223,6 +223,7 @@ __device__ void replay_updates(const View& V, const Region& R, const Move& mv, c
 }
 
 // 64-bit cell: x = eH | eL << 4 | eR << 8 | tU << 12 | tD << 19, y = bH | bL << 10 | bR << 20.
+__constant__ int kCellShift[8] = {0, 4, 8, 12, 19, 0, 10, 20};  // per plane, in x (EH..TD) or y (BH..BR)
 __device__ __forceinline__ void unpack_cell(const uint2 c, int v[8]) {
     v[EH] = c.x & 15u, v[EL] = (c.x >> 4) & 15u, v[ER] = (c.x >> 8) & 15u;
     v[TU] = (c.x >> 12) & 127u, v[TD] = (c.x >> 19) & 127u;
@@ -391,18 +392,27 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
             bw[q] = *bit_word(a, a.plane, gi, gj);  // gj even: gj + 1 is in the same half
         }
     }
+    // the rotated plane's field in the cell (the same in every lane): word, shift and width
+    const bool rhi = rp >= BH;
+    const int rsh = kCellShift[rp];
+    const uint32_t rmask = rhi ? 1023u : rp >= TU ? 127u : 15u;
+    auto rotate_cell = [&](uint2& c, int gi, int gj, uint32_t low) __attribute__((always_inline)) {
+        uint32_t& w = rhi ? c.y : c.x;
+        const int v = (int)((w >> rsh) & rmask);
+        w ^= (uint32_t)(v ^ rotate_value(mv, v, gi, gj, low)) << rsh;
+    };
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
         const int idx = threadIdx.x + q * NT;
         if (idx < np) {
             const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
             const int gi = R.gi0 + qi, gj = R.gj0 + 2 * jp;
+            const uint32_t b = bw[q] >> (gj & 31);
+            rotate_cell(c0[q], gi, gj, b & 1u);
+            rotate_cell(c1[q], gi, gj + 1, (b >> 1) & 1u);
             int v0[8], v1[8];
             unpack_cell(c0[q], v0);
             unpack_cell(c1[q], v1);
-            const uint32_t b = bw[q] >> (gj & 31);
-            v0[rp] = rotate_value(mv, v0[rp], gi, gj, b & 1u);
-            v1[rp] = rotate_value(mv, v1[rp], gi, gj + 1, (b >> 1) & 1u);
 #pragma unroll
             for (int p = 0; p < 8; p++) P.set(p, qi, jp, (uint32_t)v0[p] | ((uint32_t)v1[p] << 16));
         }
