@@ -608,7 +608,7 @@ def bench_c5(args) -> int:
     random domains (Aztec diamonds and notched squares fitting a --canvas vertex grid), each
     with edge weights a, b ~ U[0.3, 1] (weighted heat-bath, p12 = 1/2, 1/2, a^2/(a^2+b^2),
     b^2/(a^2+b^2)); the target of each is the mean height fraction of --samples-per-domain
-    chains run --eq-steps steps from the minimal tiling.  The U-Net (4 levels, 32 base channels,
+    chains run --eq-steps steps, alternately from the minimal and the maximal tiling.  The U-Net (4 levels, 32 base channels,
     bf16 matrix cores) is trained on the GPU.  Evaluation on --held held-out weighted domains
     with tiling sides in --held-sides (predicted on features sampled down to the canvas):
     chain steps to equilibrium from the minimal tiling (plain), from the midpoint
