@@ -63,6 +63,8 @@ def main() -> int:
     ap.add_argument("--sides", type=int, nargs=2, default=[64, 512], help="c3: domain side range")
     ap.add_argument("--domain-steps", type=int, default=1000, help="c3: chain steps per domain")
     ap.add_argument("--autotune", action="store_true", help="c3: measure the batch's tile configurations first")
+    ap.add_argument("--no-solo", action="store_true",
+                    help="c3: skip the one-run-per-domain pass (and the comparison with it): only the batched run")
     ap.add_argument("--kinds", nargs="+", default=["aztec", "notched", "tridimer", "recttri"],
                     help="c3: domain kinds of the mix (aztec, notched, tridimer, recttri)")
     ap.add_argument("--canvas", type=int, default=64, help="c5: largest vertex grid side (training canvas)")
@@ -522,18 +524,20 @@ def bench_c3(args) -> int:
             ch.seed(args.seed + k)
         return chs
 
-    solo = make()
-    ctx.sync()
-    w0 = time.perf_counter()
-    for k, ch in enumerate(solo):
-        ch.run(steps)
-        if k % 64 == 63:  # progress (the stream runs the chains one after another either way)
-            ctx.sync()
-            print(f"# c3: {k + 1} domains run one by one, {time.perf_counter() - w0:.1f} s", file=sys.stderr,
-                  flush=True)
-    ctx.sync()
-    solo_s = time.perf_counter() - w0
-    print(f"# c3: one run per domain {solo_s:.1f} s", file=sys.stderr, flush=True)
+    solo, solo_s = [], None
+    if not args.no_solo:
+        solo = make()
+        ctx.sync()
+        w0 = time.perf_counter()
+        for k, ch in enumerate(solo):
+            ch.run(steps)
+            if k % 64 == 63:  # progress (the stream runs the chains one after another either way)
+                ctx.sync()
+                print(f"# c3: {k + 1} domains run one by one, {time.perf_counter() - w0:.1f} s", file=sys.stderr,
+                      flush=True)
+        ctx.sync()
+        solo_s = time.perf_counter() - w0
+        print(f"# c3: one run per domain {solo_s:.1f} s", file=sys.stderr, flush=True)
     batched = make()
     b = rt.Batch(ctx)
     for ch in batched:
@@ -552,7 +556,7 @@ def bench_c3(args) -> int:
     batch_s = time.perf_counter() - w0
     print(f"# c3: batched {batch_s:.1f} s", file=sys.stderr, flush=True)
     att = sum(ch.attempts_per_step for ch in batched) * steps
-    same = True
+    same = None if args.no_solo else True
     for k, (a, c) in enumerate(zip(solo, batched)):
         try:
             same = same and bool(np.array_equal(a.read(), c.read()))
@@ -568,8 +572,8 @@ def bench_c3(args) -> int:
     extra = {"domains": args.domains, "sides": [lo, hi], "steps_per_domain": steps,
              "attempts_per_step_all_domains": att // steps, "per_kind_domains_attempts": kinds,
              "batch_seconds": batch_s, "one_run_per_domain_seconds": solo_s,
-             "one_run_per_domain_flips_per_s": att / solo_s, "batch_launches": b.last_launches,
-             "batch_graphs": b.last_graphs, "bit_identical_batch_vs_individual": bool(same),
+             "one_run_per_domain_flips_per_s": att / solo_s if solo_s else None, "batch_launches": b.last_launches,
+             "batch_graphs": b.last_graphs, "bit_identical_batch_vs_individual": same,
              "batch_configs_autotuned": tuned}
     line = _line("aggregate flip attempts/s (batched domains)", att / batch_s, "flip attempts/s", args,
                  batch_s * 1e3, "u8/i32/u16",
