@@ -73,6 +73,19 @@ struct PersistArgs {
     int64_t step_off;  // steps this launch lies past the counter (launches of one captured round)
 };
 
+// A value every lane of the workgroup holds alike (derived from blockIdx and memory every lane
+// reads at the same address), moved to a scalar register.  The compiler cannot prove such
+// values uniform when they are loaded through a pointer it cannot prove unwritten, and then
+// computes with them per lane and turns branches on them into exec-mask sequences.
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+template <class T>
+__device__ __forceinline__ T* uniform(T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
 // Resolve a batched block: the chain owning blockIdx.x, with the copies and schedule offset
 // of the current block of steps.  Returns the chain's rule tables where they lie, in the entry
 // in global memory: copying them into the caller's PersistArgs (which the kernel has to copy to
@@ -89,18 +102,18 @@ __device__ __forceinline__ const RuleParams& resolve_batch(PersistArgs& a, int& 
     const BatchEntry& e = a.batch[lo];
     const int64_t s0 = *a.step_ctr + a.step_off;
     const int par = (int)((s0 / a.block_h) & 1);
-    a.cls[0] = e.cls[0];
-    a.cls[1] = e.cls[1];
-    a.buf = e.buf0 ^ par;
-    a.gen_in = e.gens[e.g0 ^ par];
-    a.gen_out = e.gens[e.g0 ^ par ^ 1];
-    a.sched = e.sched + s0;
-    a.N = e.N;
-    a.W = e.W;
-    a.P = e.P;
-    a.tiles_c = e.tiles_c;
-    tile = b - e.tile0;
-    ntiles = e.ntiles;
+    a.cls[0] = uniform(e.cls[0]);
+    a.cls[1] = uniform(e.cls[1]);
+    a.buf = uniform(e.buf0 ^ par);
+    a.gen_in = uniform(e.gens[e.g0 ^ par]);
+    a.gen_out = uniform(e.gens[e.g0 ^ par ^ 1]);
+    a.sched = uniform(e.sched) + s0;
+    a.N = uniform(e.N);
+    a.W = uniform(e.W);
+    a.P = uniform(e.P);
+    a.tiles_c = uniform(e.tiles_c);
+    tile = uniform(b - e.tile0);
+    ntiles = uniform(e.ntiles);
     return e.rp;
 }
 

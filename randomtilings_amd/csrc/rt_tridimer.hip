@@ -437,12 +437,12 @@ __device__ __forceinline__ const TdBatchEntry& td_resolve(TdArgs& a, const TdBat
     const int b = blockIdx.x;
     const TdBatchEntry& e = batch[td_entry(batch, nbatch, b, [](const TdBatchEntry& x) { return x.tile0; })];
     const int par = (int)((s / spl) & 1);
-    a.M = e.M;
-    a.cells_in = e.cells[e.buf0 ^ par];
-    a.cells_out = e.cells[e.buf0 ^ par ^ 1];
-    a.rbits = e.rbits;
+    a.M = uniform(e.M);
+    a.cells_in = uniform(e.cells[e.buf0 ^ par]);
+    a.cells_out = uniform(e.cells[e.buf0 ^ par ^ 1]);
+    a.rbits = uniform(e.rbits);
     a.plane = (int)(s % kDrawSteps);
-    tile = b - e.tile0;
+    tile = uniform(b - e.tile0);
     return e;
 }
 
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(NT) void k_td_step_batch(TdArgs a0, const TdBatchEn
     TdArgs a = a0;
     int tile;
     const TdBatchEntry& e = td_resolve(a, batch, nbatch, s, spl, tile);
-    td_step_body<TR, TC, NT>(a, e.sched[s], tile);
+    td_step_body<TR, TC, NT>(a, (uint32_t)uniform((int)e.sched[s]), tile);
 }
 
 // ---- S steps per launch, tiles + a halo of up to S * kHalo --------------------------------
@@ -567,7 +567,7 @@ __global__ __launch_bounds__(NT) void k_td_step2_batch(TdArgs a0, const TdBatchE
     TdArgs a = a0;
     int tile;
     const TdBatchEntry& e = td_resolve(a, batch, nbatch, s, 2, tile);
-    td_multi_body<TR, TC, NT, 2>(a, (uint32_t)e.sched[s] | ((uint32_t)e.sched[s + 1] << 8), tile);
+    td_multi_body<TR, TC, NT, 2>(a, (uint32_t)uniform((int)e.sched[s] | ((int)e.sched[s + 1] << 8)), tile);
 }
 
 // ---- whole domain in one workgroup, every step in one launch ----------------------------
