@@ -86,6 +86,30 @@ RT_TDHD int rotate_value(const Move& m, int v, int gi, int gj, bool low) {
     if (v == 85) return par ? 170 : 85;
     return v;
 }
+// The same Rotate as one xor, without branches: every move swaps two states a <-> b, which is
+// v ^ (a ^ b) (5 <-> 10: 15; 45 <-> 54: 27; 85 <-> 170: 255), applied where the draw is low, the
+// site is in the move's class and v is one of the two.  rotate_value(m, v, ...) ==
+// v ^ rotate_xor(m, v, ...) for every move and state (tests/test_td_pairs.py).
+RT_TDHD int rotate_xor(const Move& m, int v, int gi, int gj, bool low) {
+    const int t = m.r1;
+    bool cls;
+    int a, b;
+    if (m.type == 0) {
+        // ((1 - (t & 1)) gi + ((1 + t) / 2) gj) mod 2 == r2
+        cls = ((((t & 1) ? 0 : gi) + (t >= 1 ? gj : 0)) & 1) == m.r2;
+        a = 5, b = 10;
+    } else if (m.type == 1) {
+        cls = mod3(gj - gi) == m.r1;
+        a = 45, b = 54;
+    } else {
+        // b1 = 1 - (t & 1), b2 = (1 + t) / 2, b3 = t & 1: slice (b1 gi + b2 gj) mod 3 == p1 and
+        // parity (b1 gj + b3 gi) mod 2 == p2
+        const int b1gi = (t & 1) ? 0 : gi, b1gj = (t & 1) ? 0 : gj, b2gj = t >= 1 ? gj : 0, b3gi = (t & 1) ? gi : 0;
+        cls = (umod3(b1gi + b2gj) == m.p1) & (((b1gj + b3gi) & 1) == m.p2);
+        a = 85, b = 170;
+    }
+    return (low & cls & ((v == a) | (v == b))) ? (a ^ b) : 0;
+}
 
 // Distance a step's update chain reaches: its phases (one Rotate plus the re-derivations of
 // replay_updates) each read neighbours at distance <= 1; lozenge and triangle moves take four

@@ -399,7 +399,7 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
     auto rotate_cell = [&](uint2& c, int gi, int gj, uint32_t low) __attribute__((always_inline)) {
         uint32_t& w = rhi ? c.y : c.x;
         const int v = (int)((w >> rsh) & rmask);
-        w ^= (uint32_t)(v ^ rotate_value(mv, v, gi, gj, low)) << rsh;
+        w ^= (uint32_t)rotate_xor(mv, v, gi, gj, low) << rsh;
     };
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
@@ -540,10 +540,9 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
                 const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
                 const int gi = R.gi0 + qi, gj = R.gj0 + 2 * jp;
                 const uint32_t w = P.w(rp, qi, jp), b = bw[q] >> (gj & 31);
-                const uint32_t lo = (uint32_t)rotate_value(mv, (int)(w & 0xffffu), gi, gj, b & 1u);
-                const uint32_t hi = (uint32_t)rotate_value(mv, (int)(w >> 16), gi, gj + 1, (b >> 1) & 1u);
-                const uint32_t nw = lo | (hi << 16);
-                if (nw != w) P.set(rp, qi, jp, nw);
+                const uint32_t x = (uint32_t)rotate_xor(mv, (int)(w & 0xffffu), gi, gj, b & 1u) |
+                                   ((uint32_t)rotate_xor(mv, (int)(w >> 16), gi, gj + 1, (b >> 1) & 1u) << 16);
+                if (x) P.set(rp, qi, jp, w ^ x);
             }
         }
         __syncthreads();
@@ -614,8 +613,8 @@ __global__ __launch_bounds__(kSmallNT) void k_td_small(TdArgs a) {
                 const uint32_t u = tm_draw_raw(g[k].x, g[k].y, g[k].z, g[k].w, pm1[k], pm2[k], ptm[k]);
                 const int gi = si[k], gj = sj[k];
                 const int v = V.g(rp, gi, gj);
-                const int w = rotate_value(mv, v, gi, gj, u < 0x80000000u);
-                if (w != v) V.s(rp, gi, gj, w);
+                const int x = rotate_xor(mv, v, gi, gj, u < 0x80000000u);
+                if (x) V.s(rp, gi, gj, v ^ x);
             }
         }
         __syncthreads();

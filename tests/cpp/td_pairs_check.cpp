@@ -7,6 +7,7 @@
 #include <random>
 #include <vector>
 using namespace rt::td;
+static uint32_t td_move_code(int type, int r1, int r2, int p1, int p2) { return rt::td_move(type, r1, r2, p1, p2); }
 #ifndef NTRIALS
 #define NTRIALS 1500
 #endif
@@ -26,7 +27,27 @@ static void cmp(Buf& a, Buf& b, const char* what) {
                     if (fails++ < 20) printf("%s: plane %d (%d,%d) site %d pair %d\n", what, p, i, j, A.g(p, i, j), B.g(p, i, j));
                 }
 }
+// every move code, state, draw half and site residue: one xor equals the branchy Rotate
+static void check_rotate() {
+    for (int type = 0; type < 3; type++)
+        for (int r1 = 0; r1 < 3; r1++)
+            for (int r2 = 0; r2 < 2; r2++)
+                for (int p1 = 0; p1 < 3; p1++)
+                    for (int p2 = 0; p2 < 2; p2++) {
+                        const Move m = decode(td_move_code(type, r1, r2, p1, p2));
+                        for (int gi = 0; gi < 12; gi++)
+                            for (int gj = 0; gj < 12; gj++)
+                                for (int low = 0; low < 2; low++)
+                                    for (int v = 0; v < 1024; v++)
+                                        if ((v ^ rotate_xor(m, v, gi, gj, low)) != rotate_value(m, v, gi, gj, low) &&
+                                            fails++ < 20)
+                                            printf("rotate: type %d r1 %d r2 %d p1 %d p2 %d (%d,%d) low %d v %d\n", type, r1,
+                                                   r2, p1, p2, gi, gj, low, v);
+                    }
+}
+
 int main() {
+    check_rotate();
     std::mt19937 rng(1);
     for (int trial = 0; trial < NTRIALS; trial++) {
         Buf base;
