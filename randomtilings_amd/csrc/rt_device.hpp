@@ -12,6 +12,34 @@
 
 namespace rt {
 
+// Chain memory is always device (global) memory, but a pointer the compiler meets as a
+// generic pointer (a field of a table in memory, as in batched launches) makes it emit flat
+// loads and stores, which also count against the LDS wait counter: every wait on an LDS access
+// then waits for the flat loads too.  gload/gstore access through a global-address-space
+// pointer (the HIP vector types through the compiler's own vector types, whose copies may bind
+// to any address space).
+#define RT_GLOBAL __attribute__((address_space(1)))
+typedef unsigned int rt_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int rt_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint4 gload(const uint4* p) {
+    const rt_u32x4 v = *(const RT_GLOBAL rt_u32x4*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 gload(const uint2* p) {
+    const rt_u32x2 v = *(const RT_GLOBAL rt_u32x2*)p;
+    return make_uint2(v.x, v.y);
+}
+template <class T>
+__device__ __forceinline__ T gload(const T* p) {
+    return *(const RT_GLOBAL T*)p;
+}
+__device__ __forceinline__ void gstore(uint4* p, uint4 v) { *(RT_GLOBAL rt_u32x4*)p = rt_u32x4{v.x, v.y, v.z, v.w}; }
+__device__ __forceinline__ void gstore(uint2* p, uint2 v) { *(RT_GLOBAL rt_u32x2*)p = rt_u32x2{v.x, v.y}; }
+template <class T>
+__device__ __forceinline__ void gstore(T* p, T v) {
+    *(RT_GLOBAL T*)p = v;
+}
+
 // Rule thresholds on k = u >> 9 (the 23-bit mantissa of the reference's rd), computed on
 // the host from the reference's float/double predicates.  A transition X -> Y fires iff
 // lo <= k < hi.  Indexed [colour class][row parity].

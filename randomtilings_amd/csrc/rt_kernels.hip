@@ -207,13 +207,13 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
             if (M::has_gen(gi, gj, N, W)) {
                 gmask |= 1u << g;
                 const int id = M::gen_id(gi, gj, N, W);
-                const uint4 s = a.gen_in[id];
+                const uint4 s = gload(a.gen_in + id);
                 s0[g] = s.x;
                 s1[g] = s.y;
                 s2[g] = s.z;
                 s3[g] = s.w;
                 if (!UNI) {
-                    const uint4 q = a.params[id];
+                    const uint4 q = gload(a.params + id);
                     pm1[g] = q.x;
                     pm2[g] = q.y;
                     pmt[g] = q.z;
@@ -254,7 +254,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
     while (s < a.nsteps) {
         const int64_t blk = (a.nsteps - s < H) ? (a.nsteps - s) : H;
         for (int64_t q = 0; q < blk; q++, s++) {
-            const int c = a.sched[s];
+            const int c = gload(a.sched + s);
             T* lc = lds + c * AREA;
             const auto& sr = M::step_rule(rp, c);
 #pragma unroll
@@ -326,7 +326,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
             const int p = tid + g * NT;
             const int li = p / EC, lj = p - (p / EC) * EC;
             const int id = M::gen_id(r0 + li, c0 + lj, N, W);
-            a.gen_out[id] = make_uint4(s0[g], s1[g], s2[g], s3[g]);
+            gstore(a.gen_out + id, make_uint4(s0[g], s1[g], s2[g], s3[g]));
         }
     }
 }

@@ -23,10 +23,10 @@
 namespace rt {
 
 __device__ __forceinline__ void xstore(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store((RT_GLOBAL uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t xload(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load((const RT_GLOBAL uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // byte-aligned funnel shift: bytes [sh, sh+4) of the little-endian pair (lo, hi)
@@ -116,13 +116,13 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
                 if (M::has_gen(gi, gj, N, W)) {
                     gbytes[g] |= posmask(b);
                     const int id = M::gen_id(gi, gj, N, W);
-                    const uint4 s = a.gen_in[id];
+                    const uint4 s = gload(a.gen_in + id);
                     s0[g][b] = s.x;
                     s1[g][b] = s.y;
                     s2[g][b] = s.z;
                     s3[g][b] = s.w;
                     if (!UNI) {
-                        const uint4 q = a.params[id];
+                        const uint4 q = gload(a.params + id);
                         pm1[g][b] = q.x;
                         pm2[g][b] = q.y;
                         pmt[g][b] = q.z;
@@ -251,7 +251,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
     // padded so the last (partial) block may read past the final step.
     static_assert(H <= 8, "a block's colours are fetched as one 8-byte word");
     auto fetch_block = [&](int64_t s_at) -> uint64_t {
-        const uint2 v = *reinterpret_cast<const uint2*>(a.sched + s_at);
+        const uint2 v = gload(reinterpret_cast<const uint2*>(a.sched + s_at));
         return ((uint64_t)v.y << 32) | v.x;
     };
 
@@ -538,7 +538,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
         for (int b = 0; b < QPW; b++) {
             if (!(gbytes[g] & posmask(b))) continue;
             const int id = M::gen_id(r0 + li, c0 + QPW * lw + b, N, W);
-            a.gen_out[id] = make_uint4(s0[g][b], s1[g][b], s2[g][b], s3[g][b]);
+            gstore(a.gen_out + id, make_uint4(s0[g][b], s1[g][b], s2[g][b], s3[g][b]));
         }
     }
 }

@@ -257,8 +257,8 @@ __device__ __forceinline__ void store_owned(const TdArgs& a, const PView& P, con
 #pragma unroll
         for (int p = 0; p < 8; p++) w[p] = P.w(p, gi - R.gi0, jo + k);
         uint2* dst = a.cells_out + (size_t)gi * a.M + gj;
-        dst[0] = pack_cell_pair(w, 0);
-        if (2 * k + 1 < ow) dst[1] = pack_cell_pair(w, 1);
+        gstore(dst, pack_cell_pair(w, 0));
+        if (2 * k + 1 < ow) gstore(dst + 1, pack_cell_pair(w, 1));
     }
 }
 
@@ -286,7 +286,7 @@ __device__ __forceinline__ void td_draw_row(const TdArgs& a, const uint4* __rest
     uint4 g = make_uint4(0u, 0u, 0u, 0u);
     uint32_t m1 = 0, m2 = 0, tm = 0;
     if (live) {
-        g = gin[id];
+        g = gload(gin + id);
         load_params<UNI>(a, id, m1, m2, tm);
     }
     uint64_t mine = 0;
@@ -296,8 +296,8 @@ __device__ __forceinline__ void td_draw_row(const TdArgs& a, const uint4* __rest
         const uint64_t w = __ballot(low);
         if (lane == s) mine = w;
     }
-    if (live) gout[id] = g;
-    if (lane < k) bits[((size_t)lane * M + gi) * td_bit_words(M) + bx] = mine;
+    if (live) gstore(gout + id, g);
+    if (lane < k) gstore(bits + ((size_t)lane * M + gi) * td_bit_words(M) + bx, mine);
 }
 
 // Blocks of four waves: blockIdx.x = the word of the row, blockIdx.y = four rows.
@@ -387,9 +387,10 @@ __device__ __forceinline__ void td_step_body(const TdArgs& a, const uint32_t mov
             const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
             const int gi = R.gi0 + qi, gj = R.gj0 + 2 * jp;
             const uint2* src = a.cells_in + (size_t)gi * M + gj;
-            c0[q] = src[0];
-            c1[q] = 2 * jp + 1 < R.cols ? src[1] : make_uint2(0u, 0u);
-            bw[q] = *bit_word(a, a.plane, gi, gj);  // gj even: gj + 1 is in the same half
+            c0[q] = gload(src);
+            c1[q] = make_uint2(0u, 0u);
+            if (2 * jp + 1 < R.cols) c1[q] = gload(src + 1);
+            bw[q] = gload(bit_word(a, a.plane, gi, gj));  // gj even: gj + 1 is in the same half
         }
     }
     // the rotated plane's field in the cell (the same in every lane): word, shift and width
@@ -502,8 +503,9 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
             if (idx < np) {
                 const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
                 const uint2* src = a.cells_in + (size_t)(R.gi0 + qi) * M + (R.gj0 + 2 * jp);
-                c0[q] = src[0];
-                c1[q] = 2 * jp + 1 < R.cols ? src[1] : make_uint2(0u, 0u);
+                c0[q] = gload(src);
+                c1[q] = make_uint2(0u, 0u);
+                if (2 * jp + 1 < R.cols) c1[q] = gload(src + 1);
             }
         }
 #pragma unroll
@@ -529,7 +531,7 @@ __device__ __forceinline__ void td_multi_body(const TdArgs& a, const uint32_t mo
             const int idx = threadIdx.x + q * NT;
             if (idx < np) {
                 const int qi = (int)fp.div((uint32_t)idx), jp = idx - qi * pc;
-                bw[q] = *bit_word(a, a.plane + s, R.gi0 + qi, R.gj0 + 2 * jp);
+                bw[q] = gload(bit_word(a, a.plane + s, R.gi0 + qi, R.gj0 + 2 * jp));
             }
         }
         if (s > 0) __syncthreads();  // the previous step's last phase wrote neighbours' states
