@@ -401,9 +401,11 @@ int32_t rt_persist_capacity(int model, int32_t idx, int device, int uniform_para
  * rows, columns, halo width and threads per workgroup of configuration idx. */
 int32_t rt_tile_config_count(int model);
 int rt_tile_config(int model, int32_t idx, int32_t* rows, int32_t* cols, int32_t* halo, int32_t* threads);
-/* 1 when tile configuration idx of the model runs the packed sweep (four positions per
- * 32-bit word: lozenge, and domino chains whose rule splits k at one threshold per class and
- * row parity); 0 otherwise.  No counterpart in the reference. */
+/* 1 when tile configuration idx of the model runs the specialised sweep: for lozenge and
+ * domino chains the packed sweep (four positions per 32-bit word; dominoes when the rule
+ * splits k at one threshold per class and row parity or is the lazy toggle), for
+ * rectangle-triangle chains the row-wave sweep (extended tile rows of 64 positions, one
+ * wavefront per row); 0 otherwise.  No counterpart in the reference. */
 int32_t rt_tile_config_packed(int model, int32_t idx);
 
 /* Number of persistent tile configurations compiled for this chain's model. */

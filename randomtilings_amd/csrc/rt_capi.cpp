@@ -828,13 +828,14 @@ int rt_chain_seed(rt_chain* ch, int mode, int64_t seed) {
 // The parameter table covers the chain (what seeding and autotuning need).
 static bool c_has_params(const rt_ctx* c, const rt_chain* ch) { return c->params && c->nparams >= ch->G; }
 
-// Whether the packed sweep can run this chain's rule: every lozenge rule (their intervals are
-// disjoint); for dominoes and coupled domino pairs the rules that split k at one threshold
+// Whether the packed sweep can run this chain's rule: every lozenge and rectangle-triangle
+// rule (lozenge intervals are disjoint; the rectangle-triangle row-wave body reads the same
+// rule tables as the generic one); for dominoes and coupled domino pairs the rules that split k at one threshold
 // 0 < t < 2^23 per class and row parity (3 -> 12 below t, 12 -> 3 from t on: the CFTP and
 // weighted heat-bath rules), and the toggle rule of dominokernel.cl (both 3 -> 12 and 12 -> 3
 // below one threshold t, the same for every class and row parity).
 static bool packed_rule_ok(const rt_chain* ch) {
-    if (ch->model == RT_LOZENGE) return true;
+    if (ch->model == RT_LOZENGE || ch->model == RT_RECTTRI) return true;
     if (ch->model != RT_DOMINO && ch->model != kDominoPair) return false;
     const FlipRule& f = ch->rp.fr;
     bool split = true, toggle = true;

@@ -192,6 +192,35 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
     __syncthreads();
     const RuleParams& rp = srp;
 
+    // Rectangle-triangle tiles whose extended row is one wavefront (EC = 64): position
+    // tid + g * NT lies in row wave + g * (NT / 64), so a wave's row, its residue mod 3 and
+    // with them the update's neighbour offsets are the same on every lane.  The flip reads
+    // one entry of a table merged from the rule (slot h of the state hash, band b of k):
+    // {state, accept below, target}, where the acceptance bound already lies at or below the
+    // gate and an empty slot accepts nothing.  Bounds and bands are compared with the raw
+    // tempered output t0 (k = t0 >> 9) as t << 9: every one lies at or below the gate, which
+    // is below 2^23, so k < t <=> t0 < t << 9 exactly.
+    constexpr bool kRowWave = std::is_same<M, RectTriModel>::value && EC == 64 && NT % 64 == 0 && ER % (NT / 64) == 0;
+    constexpr int NW = NT / 64;
+    __shared__ uint4 rtab[kRowWave ? 128 : 1];
+    uint32_t band0 = 0, band1 = 0, band2 = 0;
+    if constexpr (kRowWave) {
+        const RectTriRule& r = rp.rt;
+        if (tid < 128) {
+            const int h = tid >> 2, b = tid & 3;
+            const uint32_t key = r.hkey[h], idx = r.hval[h];
+            uint4 e = make_uint4(key, 0u, 0u, 0u);
+            if (key != 0xffffffffu) {
+                e.y = (idx < 5 ? r.cyc_acc[idx][b] : r.tog_acc[idx - 5]) << 9;
+                e.z = (uint32_t)(idx < 5 ? r.cyc_nt[idx][b] : r.tog_nt[idx - 5]);
+            }
+            rtab[tid] = e;
+        }
+        band0 = r.band[0] << 9;
+        band1 = r.band[1] << 9;
+        band2 = r.band[2] << 9;
+    }
+
     // generator states of every tile + halo position this thread handles
     uint32_t s0[GPT], s1[GPT], s2[GPT], s3[GPT];
     uint32_t pm1[UNI ? 1 : GPT], pm2[UNI ? 1 : GPT], pmt[UNI ? 1 : GPT];
@@ -248,34 +277,102 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
         if (rr >= 0 && rr < tiles_r && cc >= 0 && cc < a.tiles_c) nbr_tile = rr * a.tiles_c + cc;
     }
 
+    // Row-wave layout: per g, the row's residue mod 3 and whether the row is updated (both the
+    // same on every lane); per lane, whether its column is updated.
+    int rw_res[kRowWave ? GPT : 1];
+    uint32_t rw_upd = 0;
+    bool rw_col = false;
+    if constexpr (kRowWave) {
+        const int wave = uniform(tid >> 6);
+#pragma unroll
+        for (int g = 0; g < GPT; g++) {
+            const int li = wave + g * NW, gi = r0 + li;
+            rw_res[g] = ((gi % 3) + 3) % 3;
+            if (li >= 1 && li <= ER - 2 && gi >= 1 && gi <= N - 2) rw_upd |= 1u << g;
+        }
+        const int lane = tid & 63, gj = c0 + lane;
+        rw_col = lane >= 1 && lane <= EC - 2 && gj >= 1 && gj <= W - 2;
+    }
+
     int buf = a.buf;
     uint32_t nblk = 0;
     int64_t s = 0;
     while (s < a.nsteps) {
         const int64_t blk = (a.nsteps - s < H) ? (a.nsteps - s) : H;
         for (int64_t q = 0; q < blk; q++, s++) {
-            const int c = gload(a.sched + s);
+            const int c = uniform((int)gload(a.sched + s));
             T* lc = lds + c * AREA;
             const auto& sr = M::step_rule(rp, c);
+            if constexpr (kRowWave) {
+                // Every lane draws: a position without a generator holds state 0, which no
+                // table entry matches, and its draws are discarded.
 #pragma unroll
-            for (int g = 0; g < GPT; g++) {
-                if (gmask & (1u << g)) {
+                for (int g = 0; g < GPT; g++) {
                     const int p = tid + g * NT;
-                    const uint32_t k = tm_draw(s0[g], s1[g], s2[g], s3[g], UNI ? a.m1 : pm1[g], UNI ? a.m2 : pm2[g],
-                                               UNI ? a.tm : pmt[g]);
-                    const int gi = r0 + p / EC;
-                    const T st = lc[p];
-                    const T ns = M::flip(sr, gi, st, k);
-                    if (ns != st) lc[p] = ns;
+                    const uint32_t t0 = tm_draw_raw(s0[g], s1[g], s2[g], s3[g], UNI ? a.m1 : pm1[g],
+                                                    UNI ? a.m2 : pm2[g], UNI ? a.tm : pmt[g]);
+                    const uint32_t st = (uint32_t)lc[p];
+                    const uint32_t h = ((uint32_t)__umul24(st, kRtHash) >> 27) & 31u;
+                    const uint32_t b = (uint32_t)(t0 >= band0) + (uint32_t)(t0 >= band1) + (uint32_t)(t0 >= band2);
+                    const uint4 e = rtab[h * 4 + b];  // one read; written back unchanged if no move
+                    lc[p] = (T)(((e.x == st) & (t0 < e.y)) ? e.z : st);
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < GPT; g++) {
+                    if (gmask & (1u << g)) {
+                        const int p = tid + g * NT;
+                        const uint32_t k = tm_draw(s0[g], s1[g], s2[g], s3[g], UNI ? a.m1 : pm1[g],
+                                                   UNI ? a.m2 : pm2[g], UNI ? a.tm : pmt[g]);
+                        const int gi = r0 + p / EC;
+                        const T st = lc[p];
+                        const T ns = M::flip(sr, gi, st, k);
+                        if (ns != st) lc[p] = ns;
+                    }
                 }
             }
             __syncthreads();
+            if constexpr (kRowWave) {
+                // recttrianglekernel.cl:120-146 with e = (row - c) mod 3 choosing the shifted
+                // neighbours (the same on the whole wave); each class word is two field
+                // selects and one byte permutation:
+                //   c+1: bytes (u.1, u.2, x.0, 0), u = y & 0xf00f00 | z & ~0xf00f00
+                //   c+2: bytes (z.2, v.0, v.1, 0), v = x & 0xf00f | y & ~0xf00f
+                const T* sc = lds + c * AREA;
+                T* d1p = lds + (c == 2 ? 0 : c + 1) * AREA;
+                T* d2p = lds + (c == 0 ? 2 : c - 1) * AREA;
 #pragma unroll
-            for (int g = 0; g < GPT; g++) {
-                if (umask & (1u << g)) {
+                for (int g = 0; g < GPT; g++) {
+                    if (!(rw_upd & (1u << g))) continue;
                     const int p = tid + g * NT;
-                    const int gi = r0 + p / EC;
-                    M::update(c, gi, lds, AREA, p, EC);
+                    // one complete path per e (its offsets are constants)
+                    auto upd = [&](auto e_c) __attribute__((always_inline)) {
+                        constexpr int E = decltype(e_c)::value;
+                        const uint32_t up = (uint32_t)sc[p - EC], mid = (uint32_t)sc[p], dn = (uint32_t)sc[p + EC];
+                        const uint32_t y1 = E == 0 ? (uint32_t)sc[p + 1] : mid;
+                        const uint32_t z1 = E == 1 ? (uint32_t)sc[p + EC - 1] : dn;
+                        const uint32_t x2 = E == 2 ? (uint32_t)sc[p - 1] : mid;
+                        const uint32_t y2 = E == 1 ? (uint32_t)sc[p - EC + 1] : up;
+                        const uint32_t u = (y1 & 0xf00f00u) | (z1 & ~0xf00f00u);
+                        const uint32_t v = (x2 & 0xf00fu) | (y2 & ~0xf00fu);
+                        if (rw_col) {
+                            d1p[p] = (T)__builtin_amdgcn_perm(up, u, 0x0c040201u);
+                            d2p[p] = (T)__builtin_amdgcn_perm(dn, v, 0x0c010006u);
+                        }
+                    };
+                    const int e = (rw_res[g] - c + 3) % 3;
+                    if (e == 0) upd(std::integral_constant<int, 0>{});
+                    else if (e == 1) upd(std::integral_constant<int, 1>{});
+                    else upd(std::integral_constant<int, 2>{});
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < GPT; g++) {
+                    if (umask & (1u << g)) {
+                        const int p = tid + g * NT;
+                        const int gi = r0 + p / EC;
+                        M::update(c, gi, lds, AREA, p, EC);
+                    }
                 }
             }
             __syncthreads();
@@ -345,7 +442,27 @@ __global__ __launch_bounds__(NT) void k_persist_batch(PersistArgs a_in) {
     persist_body<M, TR, TC, H, NT, UNI>(a, rules, tile, ntiles);
 }
 
-// tile configurations per model: {TR, TC, H, NT, packed}; packed selects the quad kernel
+// The rectangle-triangle row-wave kernels with the parameters in kernel arguments, built for 8
+// waves per SIMD (at most 64 VGPRs and 96 SGPRs): with the default ~112 SGPRs only 7 fit,
+// which leaves a CU one 1024-thread workgroup, idle while that workgroup loads and stores its
+// tile; with two, one computes while the other streams.  (Per-generator parameters do not fit
+// 64 VGPRs without scratch; those kernels keep the plain wrappers.)
+template <int TR, int TC, int H, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_persist_rw(PersistArgs a) {
+    persist_body<RectTriModel, TR, TC, H, NT, true>(a, a.rp, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+}
+
+template <int TR, int TC, int H, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_persist_rw_batch(PersistArgs a_in) {
+    PersistArgs a = a_in;
+    int tile, ntiles;
+    const RuleParams& rules = resolve_batch(a, tile, ntiles);
+    persist_body<RectTriModel, TR, TC, H, NT, true>(a, rules, tile, ntiles);
+}
+
+// tile configurations per model: {TR, TC, H, NT, packed}; packed selects the specialised
+// sweep: the quad kernel for lozenges and dominoes, the row-wave body (extended rows of 64
+// positions) for rectangle-triangles
 struct Cfg {
     int tr, tc, h, nt, packed;
 };
@@ -384,8 +501,12 @@ struct Cfg {
     X(8, 20, 8, 256, 1)
 #define RT_RECTTRI_CFGS(X)  \
     X(32, 32, 4, 256, 0)    \
-    X(48, 48, 8, 512, 0)    \
-    X(16, 32, 4, 256, 0)
+    X(48, 48, 8, 1024, 1)   \
+    X(16, 32, 4, 256, 0)    \
+    X(56, 56, 4, 1024, 1)   \
+    X(24, 56, 4, 512, 1)    \
+    X(16, 56, 4, 384, 1)    \
+    X(8, 56, 4, 256, 1)
 
 #define RT_CFG_ENTRY(a, b, c, d, e) Cfg{a, b, c, d, e},
 static const Cfg kDominoCfgs[] = {RT_DOMINO_CFGS(RT_CFG_ENTRY)};
@@ -441,6 +562,14 @@ static const void* kernel_ptr(bool uni, bool batch) {
     } else if constexpr (PACKED != 0 && std::is_same<M, DominoPairModel>::value) {
         if (batch) return nullptr;  // coupled pairs run alone (coupling from the past)
         return uni ? (const void*)k_persist_pair2<TR, TC, H, NT, true> : (const void*)k_persist_pair2<TR, TC, H, NT, false>;
+    } else if constexpr (PACKED != 0 && std::is_same<M, RectTriModel>::value) {
+        // persist_body takes the row-wave path for this geometry by itself
+        static_assert(TC + 2 * H == 64 && NT % 64 == 0 && (TR + 2 * H) % (NT / 64) == 0,
+                      "a row-wave rectangle-triangle tile has extended rows of one wavefront");
+        if (batch)
+            return uni ? (const void*)k_persist_rw_batch<TR, TC, H, NT>
+                       : (const void*)k_persist_batch<M, TR, TC, H, NT, false>;
+        return uni ? (const void*)k_persist_rw<TR, TC, H, NT> : (const void*)k_persist<M, TR, TC, H, NT, false>;
     } else if constexpr (PACKED != 0) {
         return nullptr;  // no packed kernel for this model
     } else {

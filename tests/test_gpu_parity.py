@@ -250,6 +250,29 @@ def test_recttri_weighted_parity(gctx, golden):
     assert np.array_equal(g, o)
 
 
+@pytest.mark.parametrize("cfg", [i for i in range(N_CFGS[rt.RT_RECTTRI]) if rt.tile_config_packed(rt.RT_RECTTRI, i)])
+def test_recttri_rowwave_multitile(gctx, sctx, cfg):
+    """The row-wave sweep (extended tile rows of 64 positions) on maxHex(60) (129 x 129: several
+    tiles in both directions), a step count that is no multiple of the block length, persistent
+    and blocked paths, with the default rule, random weights, and a per-generator parameter
+    table."""
+    from randomtilings_amd import statics as St
+    t0 = St.call("recttri.maxHex", args=[60])
+    w = np.random.default_rng(5).uniform(0.3, 1.7, 16).astype(np.float32)
+    w[0] = 0
+    cases = [(gctx, None, default_table(BIG)), (gctx, w, default_table(BIG)), (sctx, w, scrambled_table(BIG, 11))]
+    for ctx, wts, table in cases:
+        o = O.recttri_walk(t0, 203, 17, table, weights16=wts)
+        for path in (rt.RT_PATH_PERSISTENT, rt.RT_PATH_BLOCKED):
+            say("recttri row-wave", cfg, path, wts is not None, ctx is sctx)
+            ctx.set_path(path, cfg)
+            try:
+                g = ctx.recttri_walk(t0, 203, 17, weights16=wts)
+            finally:
+                ctx.set_path(rt.RT_PATH_AUTO, -1)
+            assert np.array_equal(g, o), (path, wts is not None, ctx is sctx)
+
+
 def test_chain_runs_compose(gctx, golden):
     """run(a) then run(b) on a device chain equals one Walk of a+b steps."""
     t0 = golden["lozenge_hex20_0_min"]
