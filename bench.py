@@ -63,6 +63,9 @@ def main() -> int:
     ap.add_argument("--sides", type=int, nargs=2, default=[64, 512], help="c3: domain side range")
     ap.add_argument("--domain-steps", type=int, default=1000, help="c3: chain steps per domain")
     ap.add_argument("--autotune", action="store_true", help="c3: measure the batch's tile configurations first")
+    ap.add_argument("--batch-configs", type=int, nargs=4, default=None, metavar=("DOM", "LOZ", "RT", "TD"),
+                    help="c3: tile configuration per model (domino, lozenge, rect-triangle, triangle-dimer; -1 "
+                         "automatic), e.g. what an earlier --autotune chose")
     ap.add_argument("--no-solo", action="store_true",
                     help="c3: skip the one-run-per-domain pass (and the comparison with it): only the batched run")
     ap.add_argument("--kinds", nargs="+", default=["aztec", "notched", "tridimer", "recttri"],
@@ -549,6 +552,9 @@ def bench_c3(args) -> int:
         w0 = time.perf_counter()
         tuned = b.autotune(64)
         print(f"# c3: batch autotune {tuned} in {time.perf_counter() - w0:.1f} s", file=sys.stderr, flush=True)
+    elif args.batch_configs:
+        for m, cfg in enumerate(args.batch_configs):
+            b.set_config(m, cfg)
     ctx.sync()
     w0 = time.perf_counter()
     b.run(steps, True)
@@ -574,7 +580,7 @@ def bench_c3(args) -> int:
              "batch_seconds": batch_s, "one_run_per_domain_seconds": solo_s,
              "one_run_per_domain_flips_per_s": att / solo_s if solo_s else None, "batch_launches": b.last_launches,
              "batch_graphs": b.last_graphs, "bit_identical_batch_vs_individual": same,
-             "batch_configs_autotuned": tuned}
+             "batch_configs_autotuned": tuned, "batch_configs_set": args.batch_configs}
     line = _line("aggregate flip attempts/s (batched domains)", att / batch_s, "flip attempts/s", args,
                  batch_s * 1e3, "u8/i32/u16",
                  f"{args.domains} domains, sides U[{lo},{hi}], kinds {'/'.join(args.kinds)} (Aztec diamonds, "

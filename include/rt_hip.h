@@ -351,6 +351,11 @@ int rt_batch_sync(rt_batch* b);
  * in the batch).  Every chain is put back exactly as it was (state, generators and host
  * schedule RNGs), so the trajectories do not depend on the probes.  No reference counterpart. */
 int rt_batch_autotune(rt_batch* b, int64_t probe_steps, int32_t* chosen);
+/* Set the tile configuration later runs of the batch use for `model` (RT_DOMINO .. RT_TRIDIMER;
+ * idx -1 returns to the automatic choice), e.g. one an earlier rt_batch_autotune chose for the
+ * same chains.  RT_ERR_ARG for an index out of range; a run fails with RT_ERR_ARG when the
+ * configuration is a packed one and a chain's rule cannot use it.  No reference counterpart. */
+int rt_batch_set_config(rt_batch* b, int model, int32_t idx);
 /* GPU milliseconds of the most recent rt_batch_run (HIP events on the context stream). */
 int rt_batch_elapsed_ms(rt_batch* b, double* ms);
 /* Kernel launches (those of graph replays included) and graphs instantiated by the last run

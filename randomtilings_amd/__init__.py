@@ -107,6 +107,7 @@ SIGNATURES = {
     "rt_batch_run": (C.c_int, [_P, C.c_int64, C.c_int]),
     "rt_batch_sync": (C.c_int, [_P]),
     "rt_batch_autotune": (C.c_int, [_P, C.c_int64, _I32P]),
+    "rt_batch_set_config": (C.c_int, [_P, C.c_int, C.c_int32]),
     "rt_batch_elapsed_ms": (C.c_int, [_P, _DP]),
     "rt_batch_last_launches": (C.c_int64, [_P]),
     "rt_batch_last_graphs": (C.c_int32, [_P]),
@@ -520,6 +521,11 @@ class Batch:
         out = (C.c_int32 * 4)()
         _check(self._lib.rt_batch_autotune(self.h, probe_steps, out))
         return {m: out[m] for m in range(4) if out[m] >= 0}
+
+    def set_config(self, model: int, idx: int) -> None:
+        """Tile configuration of `model` for later runs (-1: the automatic choice), e.g. one an
+        earlier autotune() returned for the same chains."""
+        _check(self._lib.rt_batch_set_config(self.h, model, idx))
 
     def elapsed_ms(self) -> float:
         v = C.c_double()

@@ -1411,7 +1411,12 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         for (rt_chain* ch : planar)
             if (ch->model == m && !packed_rule_ok(ch)) all_packed_ok = false;
         g.cfg = choose_persist_cfg(largest, &nb, &tcn, &res, !all_packed_ok, true);
-        if (b->cfg_override[m] >= 0) g.cfg = b->cfg_override[m];
+        if (b->cfg_override[m] >= 0) {
+            g.cfg = b->cfg_override[m];
+            if (persist_config_packed(m, g.cfg) && !all_packed_ok)
+                return fail(RT_ERR_ARG, "rt_batch_run: configuration %d of model %d is packed and a chain's rule "
+                            "cannot use it", g.cfg, m);
+        }
         if (g.cfg < 0) return fail(RT_ERR_ARG, "rt_batch_run: no tile configuration for model %d", m);
         int tr, tc, h, nt;
         persist_config(m, g.cfg, &tr, &tc, &h, &nt);
@@ -1760,6 +1765,17 @@ int rt_batch_autotune(rt_batch* b, int64_t probe_steps, int32_t* chosen) {
     if (rc != RT_OK) return rc;
     if (chosen)
         for (int m = 0; m < 4; m++) chosen[m] = b->cfg_override[m];
+    return RT_OK;
+}
+
+int rt_batch_set_config(rt_batch* b, int model, int32_t idx) {
+    if (!b) return fail(RT_ERR_ARG, "rt_batch_set_config: null batch");
+    if (model != RT_DOMINO && model != RT_LOZENGE && model != RT_RECTTRI && model != RT_TRIDIMER)
+        return fail(RT_ERR_ARG, "rt_batch_set_config: unknown model %d", model);
+    const int n = model == RT_TRIDIMER ? td_config_count() : persist_config_count(model);
+    if (idx < -1 || idx >= n)
+        return fail(RT_ERR_ARG, "rt_batch_set_config: configuration %d of model %d (%d compiled)", idx, model, n);
+    b->cfg_override[model] = idx;
     return RT_OK;
 }
 

@@ -19,6 +19,7 @@
 #include "rt_packed.hpp"
 
 #include <type_traits>
+#include <utility>
 #include "rt_models.hpp"
 
 #include <algorithm>
@@ -159,6 +160,16 @@ hipError_t launch_stepwise(int model, const StepwiseArgs& a, hipStream_t st) {
 // agent-coherent 32-bit accesses (xstore/xload: relaxed atomics at agent scope, coherent
 // across XCDs without writing back or invalidating whole L2s), rows being 16-byte aligned.
 
+// f(std::integral_constant<int, i>) for i = 0 .. N-1, in order
+template <class F, int... I>
+__device__ __forceinline__ void static_for_seq(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
 template <class M, int TR, int TC, int H, int NT, bool UNI>
 __device__ __forceinline__ void persist_body(const PersistArgs& a, const RuleParams& rules, const int tile,
                                              const int ntiles) {
@@ -277,17 +288,18 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
         if (rr >= 0 && rr < tiles_r && cc >= 0 && cc < a.tiles_c) nbr_tile = rr * a.tiles_c + cc;
     }
 
-    // Row-wave layout: per g, the row's residue mod 3 and whether the row is updated (both the
+    // Row-wave layout: the residue mod 3 of the wave's first row (row g lies NW rows further
+    // per g, so its residue is rw_base + g * NW), and per g whether the row is updated (both the
     // same on every lane); per lane, whether its column is updated.
-    int rw_res[kRowWave ? GPT : 1];
+    int rw_base = 0;
     uint32_t rw_upd = 0;
     bool rw_col = false;
     if constexpr (kRowWave) {
         const int wave = uniform(tid >> 6);
+        rw_base = (((r0 + wave) % 3) + 3) % 3;
 #pragma unroll
         for (int g = 0; g < GPT; g++) {
             const int li = wave + g * NW, gi = r0 + li;
-            rw_res[g] = ((gi % 3) + 3) % 3;
             if (li >= 1 && li <= ER - 2 && gi >= 1 && gi <= N - 2) rw_upd |= 1u << g;
         }
         const int lane = tid & 63, gj = c0 + lane;
@@ -338,16 +350,17 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
                 // selects and one byte permutation:
                 //   c+1: bytes (u.1, u.2, x.0, 0), u = y & 0xf00f00 | z & ~0xf00f00
                 //   c+2: bytes (z.2, v.0, v.1, 0), v = x & 0xf00f | y & ~0xf00f
+                // One branch per step on e0 = e of row g = 0; row g then has e = (e0 + g * NW)
+                // mod 3, a constant, so every row's offsets are immediates.
                 const T* sc = lds + c * AREA;
                 T* d1p = lds + (c == 2 ? 0 : c + 1) * AREA;
                 T* d2p = lds + (c == 0 ? 2 : c - 1) * AREA;
-#pragma unroll
-                for (int g = 0; g < GPT; g++) {
-                    if (!(rw_upd & (1u << g))) continue;
-                    const int p = tid + g * NT;
-                    // one complete path per e (its offsets are constants)
-                    auto upd = [&](auto e_c) __attribute__((always_inline)) {
-                        constexpr int E = decltype(e_c)::value;
+                auto upd_rows = [&](auto e0_c) __attribute__((always_inline)) {
+                    static_for<GPT>([&](auto g_c) __attribute__((always_inline)) {
+                        constexpr int G = decltype(g_c)::value;
+                        constexpr int E = (decltype(e0_c)::value + G * NW) % 3;
+                        if (!(rw_upd & (1u << G))) return;
+                        const int p = tid + G * NT;
                         const uint32_t up = (uint32_t)sc[p - EC], mid = (uint32_t)sc[p], dn = (uint32_t)sc[p + EC];
                         const uint32_t y1 = E == 0 ? (uint32_t)sc[p + 1] : mid;
                         const uint32_t z1 = E == 1 ? (uint32_t)sc[p + EC - 1] : dn;
@@ -359,12 +372,12 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
                             d1p[p] = (T)__builtin_amdgcn_perm(up, u, 0x0c040201u);
                             d2p[p] = (T)__builtin_amdgcn_perm(dn, v, 0x0c010006u);
                         }
-                    };
-                    const int e = (rw_res[g] - c + 3) % 3;
-                    if (e == 0) upd(std::integral_constant<int, 0>{});
-                    else if (e == 1) upd(std::integral_constant<int, 1>{});
-                    else upd(std::integral_constant<int, 2>{});
-                }
+                    });
+                };
+                const int e0 = (rw_base - c + 3) % 3;
+                if (e0 == 0) upd_rows(std::integral_constant<int, 0>{});
+                else if (e0 == 1) upd_rows(std::integral_constant<int, 1>{});
+                else upd_rows(std::integral_constant<int, 2>{});
             } else {
 #pragma unroll
                 for (int g = 0; g < GPT; g++) {

@@ -176,3 +176,63 @@ def test_batch_autotune_leaves_chains_untouched(bctx, golden):
         b.close()
         for ch in solo + batched:
             ch.close()
+
+
+def test_batch_set_config_every_configuration(bctx, golden):
+    """rt_batch_set_config: the mixed batch runs on configuration i of every model at once (i
+    cycling through each model's compiled configurations), run after run, and still ends exactly
+    where individual runs of the same chains end; out-of-range indices and a packed
+    configuration for a rule that cannot use it are refused."""
+    specs = _specs(golden)
+    ncfg = {m: rt.tile_config_count(m) for m in (rt.RT_DOMINO, rt.RT_LOZENGE, rt.RT_RECTTRI, rt.RT_TRIDIMER)}
+    solo, batched = [], []
+    b = rt.Batch(bctx)
+    try:
+        for k, (m, rule, t0, rp) in enumerate(specs):
+            a, c = bctx.chain(m, rule, t0, rp), bctx.chain(m, rule, t0, rp)
+            a.seed(500 + k)
+            c.seed(500 + k)
+            solo.append(a)
+            batched.append(c)
+            b.add(c)
+        total = 0
+        for i in range(max(ncfg.values())):
+            for m, n in ncfg.items():
+                b.set_config(m, i % n)
+            steps = 19 + 3 * i
+            say("batch set_config", i, steps)
+            b.run(steps, i % 2 == 0)
+            total += steps
+            for m, n in ncfg.items():
+                assert all(c.last_config == i % n for c, s in zip(batched, specs) if s[0] == m), (i, m)
+        b.sync()
+        for a in solo:
+            a.run(total)
+        for k, (a, c) in enumerate(zip(solo, batched)):
+            assert np.array_equal(a.read(), c.read()), k
+        for m, n in ncfg.items():
+            with pytest.raises(rt.RTError):
+                b.set_config(m, n)
+        with pytest.raises(rt.RTError):
+            b.set_config(rt.RT_DOMINO_PAIR, 0)
+    finally:
+        b.close()
+        for ch in solo + batched:
+            ch.close()
+    # a weighted domino rule with P(12) = 1 in one class does not split k inside (0, 2^23): the
+    # packed sweep cannot run it, so forcing a packed configuration fails the run
+    packed = [i for i in range(ncfg[rt.RT_DOMINO]) if rt.tile_config_packed(rt.RT_DOMINO, i)]
+    ch = bctx.chain(rt.RT_DOMINO, rt.RT_DOMINO_WEIGHTED, golden["domino_aztec16_min"], [1.0, 0.5, 0.5, 0.5])
+    b = rt.Batch(bctx)
+    try:
+        ch.seed(3)
+        b.add(ch)
+        b.set_config(rt.RT_DOMINO, packed[0])
+        with pytest.raises(rt.RTError):
+            b.run(8)
+        b.set_config(rt.RT_DOMINO, -1)
+        b.run(8)
+        b.sync()
+    finally:
+        b.close()
+        ch.close()
