@@ -69,7 +69,8 @@ def main() -> int:
     ap.add_argument("--no-solo", action="store_true",
                     help="c3: skip the one-run-per-domain pass (and the comparison with it): only the batched run")
     ap.add_argument("--kinds", nargs="+", default=["aztec", "notched", "tridimer", "recttri"],
-                    help="c3: domain kinds of the mix (aztec, notched, tridimer, recttri)")
+                    help="c3: domain kinds of the mix (aztec, notched, tridimer, recttri; also lozenge: "
+                         "almost-hexagons under lozengekernel.cl's rule)")
     ap.add_argument("--canvas", type=int, default=64, help="c5: largest vertex grid side (training canvas)")
     ap.add_argument("--train-domains", type=int, default=96, help="c5: training domains")
     ap.add_argument("--samples-per-domain", type=int, default=16, help="c5: chain samples averaged per domain")
@@ -503,8 +504,8 @@ def bench_tridimer(args) -> int:
 def bench_c3(args) -> int:
     """Batched plain chains: --domains domains drawn from --seed with sides uniform in --sides,
     cycling Aztec diamonds, domino rectangles with random staircase notches (tileability
-    checked by the device max/min construction), triangle-dimer cones and rectangle-triangle
-    hexagons; every chain runs --domain-steps steps.  One batched run (blocked launches per
+    checked by the device max/min construction), triangle-dimer hexagons and rectangle-triangle
+    hexagons (or the --kinds given); every chain runs --domain-steps steps.  One batched run (blocked launches per
     model, HIP-graph replay) against one rt_chain_run per domain, on identical chains."""
     import randomtilings_amd as rt
     from randomtilings_amd import statics
@@ -581,17 +582,20 @@ def bench_c3(args) -> int:
              "one_run_per_domain_flips_per_s": att / solo_s if solo_s else None, "batch_launches": b.last_launches,
              "batch_graphs": b.last_graphs, "bit_identical_batch_vs_individual": same,
              "batch_configs_autotuned": tuned, "batch_configs_set": args.batch_configs}
+    what = {"aztec": "Aztec diamonds", "notched": "notched domino rectangles", "tridimer": "triangle-dimer hexagons",
+            "recttri": "rect-triangle hexagons", "lozenge": "lozenge almost-hexagons"}
     line = _line("aggregate flip attempts/s (batched domains)", att / batch_s, "flip attempts/s", args,
                  batch_s * 1e3, "u8/i32/u16",
-                 f"{args.domains} domains, sides U[{lo},{hi}], kinds {'/'.join(args.kinds)} (Aztec diamonds, "
-                 f"notched domino rectangles, triangle-dimer hexagons, rect-triangle hexagons), {steps} steps each",
+                 f"{args.domains} domains, sides U[{lo},{hi}], kinds {'/'.join(args.kinds)} "
+                 f"({', '.join(what[k] for k in args.kinds)}), {steps} steps each",
                  extra)
     if not args.no_cpu:
         # the oracle on one domain of each kind (the largest), a bounded number of steps each
         import oracle
         walk = {rt.RT_DOMINO: lambda t, n, sd, p: oracle.domino_walk(t, n, sd, p, rule=1),
                 rt.RT_TRIDIMER: lambda t, n, sd, p: oracle.tridimer_walk(t, n, sd, p),
-                rt.RT_RECTTRI: lambda t, n, sd, p: oracle.recttri_walk(t, n, sd, p)}
+                rt.RT_RECTTRI: lambda t, n, sd, p: oracle.recttri_walk(t, n, sd, p),
+                rt.RT_LOZENGE: lambda t, n, sd, p: oracle.lozenge_walk(t, n, sd, p, rule=0)}
         cpu_att, cpu_s, picked = 0, 0.0, []
         for kind in sorted({sp[3] for sp in specs}):
             k = max((i for i in range(len(specs)) if specs[i][3] == kind), key=lambda i: specs[i][2].size)

@@ -513,7 +513,9 @@ struct Cfg {
     X(80, 56, 4, 768, 1)    \
     X(68, 68, 4, 512, 1)    \
     X(16, 28, 8, 256, 1)    \
-    X(8, 20, 8, 256, 1)
+    X(8, 20, 8, 256, 1)     \
+    X(32, 44, 8, 768, 1)    \
+    X(40, 36, 8, 768, 1)
 #define RT_RECTTRI_CFGS(X)  \
     X(32, 32, 4, 256, 0)    \
     X(48, 48, 8, 1024, 1)   \

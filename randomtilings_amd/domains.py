@@ -1,7 +1,7 @@
 """Domain mixes for batched runs: Aztec diamonds, domino rectangles with staircase notches
 (tileability decided by the device maximal/minimal construction), triangle-dimer hexagons
 (covered by TriangleDimerTiler::Hexagon, sides chosen so a dimer cover exists) and
-rectangle-triangle hexagons, drawn from a seed.
+rectangle-triangle hexagons, drawn from a seed (lozenge hexagons on request).
 
 The reference builds one domain per tiler and loops over them in the caller
 (src/Domino/DominoTiler.cpp:118-193, src/TriangleDimer/TriangleDimerTiler.cpp:485-581,
@@ -72,6 +72,9 @@ def domain(ctx, kind: str, side: int, rng: np.random.Generator):
         return rt.RT_TRIDIMER, 0, statics.call("tridimer.Hexagon", args=list(tridimer_hexagon_sides(side)))
     if kind == "recttri":
         return rt.RT_RECTTRI, 0, statics.call("recttri.maxHex", args=[max(1, (side - 7) // 2)])
+    if kind == "lozenge":  # not in the default mix: lozenge batches on their own
+        d = statics.call("lozenge.AlmostHexDomain", args=[max(1, (side - 7) // 2), 0])
+        return rt.RT_LOZENGE, rt.RT_LOZENGE_UNIFORM, statics.call("lozenge.MinTiling", d)
     raise ValueError(kind)
 
 
