@@ -311,10 +311,18 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
     int64_t s = 0;
     while (s < a.nsteps) {
         const int64_t blk = (a.nsteps - s < H) ? (a.nsteps - s) : H;
+        // A flip reads and writes only the thread's own positions of class c, whose words the
+        // thread's own update wrote in the previous step; the updates read class c' (the
+        // previous step's class) of the neighbours.  So the flip must wait for the previous
+        // update only when c == c', which would overwrite what a neighbour may still read; the
+        // block starts after a barrier.
+        int c_prev = -1;
         for (int64_t q = 0; q < blk; q++, s++) {
             const int c = uniform((int)gload(a.sched + s));
             T* lc = lds + c * AREA;
             const auto& sr = M::step_rule(rp, c);
+            if (c == c_prev) __syncthreads();
+            c_prev = c;
             if constexpr (kRowWave) {
                 // Every lane draws: a position without a generator holds state 0, which no
                 // table entry matches, and its draws are discarded.
@@ -388,8 +396,8 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
                     }
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();  // every update of the block is in LDS before it is published
         // publish owned positions into the other copy
         {
             uint32_t* dst = reinterpret_cast<uint32_t*>((buf ? a.cls[0] : a.cls[1]));
