@@ -321,10 +321,10 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
     };
     // the targets of the first DA steps of the next full block, drawn during the exchange
     // (more would not fit the register budget; none with per-generator parameters, whose
-    // registers leave no room).  Tiles of one word per thread on 256 threads (small domains,
+    // registers leave no room).  Tiles of one word per thread on up to 512 threads (small domains,
     // one wave per SIMD, where every draw made inside a step lies on the step's critical path)
     // draw the whole block ahead: H more words of registers.
-    constexpr int DA = !UNI ? 0 : (QPT == 1 && NT <= 256) ? H : H / 2;
+    constexpr int DA = !UNI ? 0 : (QPT == 1 && NT <= 512) ? H : H / 2;
     uint32_t XP[DA > 0 ? DA : 1][QPT];
     auto draw_ahead = [&](auto q0_c, auto q1_c, auto split_c, uint64_t cols) __attribute__((always_inline)) {
 #pragma unroll
