@@ -320,7 +320,6 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
         for (int64_t q = 0; q < blk; q++, s++) {
             const int c = uniform((int)gload(a.sched + s));
             T* lc = lds + c * AREA;
-            const auto& sr = M::step_rule(rp, c);
             if (c == c_prev) __syncthreads();
             c_prev = c;
             if constexpr (kRowWave) {
@@ -338,6 +337,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
                     lc[p] = (T)(((e.x == st) & (t0 < e.y)) ? e.z : st);
                 }
             } else {
+                const auto& sr = M::step_rule(rp, c);
 #pragma unroll
                 for (int g = 0; g < GPT; g++) {
                     if (gmask & (1u << g)) {
