@@ -81,3 +81,28 @@ def test_a_changed_selector_fails():
     for bad in (s1 ^ 0x01, s1 ^ 0x0100, (s1 & ~0xFF0000) | 0x050000):
         assert not np.array_equal(kernel_update((mu, mv, bad, s2), x, y, z, 1).astype(np.int64), ref)
     assert not np.array_equal(kernel_update((mu ^ 0x100, mv, s1, s2), x, y, z, 1).astype(np.int64), ref)
+
+
+@pytest.mark.parametrize("seed", [None, 1, 2, 3])
+def test_raw_output_thresholds_exact(seed):
+    """The row-wave flip compares the raw tempered output t0 with t << 9 (k = t0 >> 9): exact
+    when every band limit and acceptance bound is below 2^23 (k < t <=> t0 < t << 9, and t << 9
+    does not wrap).  They lie at or below the gate, which is below 2^23, for unit and random
+    weights (including weights far from 1, where acceptance saturates)."""
+    import randomtilings_amd as rt
+
+    if seed is None:
+        th = rt.rule_thresholds(rt.RT_RECTTRI, 0)
+    else:
+        w = np.random.default_rng(seed).uniform(0.05, 20.0, 16).astype(np.float32)
+        w[0] = 0
+        th = rt.rule_thresholds(rt.RT_RECTTRI, 0, w.astype(np.float64))
+    r = th[24:]
+    gate, band, cyc_acc, tog_acc = int(r[0]), r[1:4], r[4:24], r[24:36]
+    assert 0 < gate < (1 << 23)
+    for t in list(band) + list(cyc_acc) + list(tog_acc):
+        assert int(t) <= gate
+        k = np.array([0, max(int(t) - 1, 0), int(t), (1 << 23) - 1], dtype=np.uint64)
+        for base in k << np.uint64(9):  # the smallest and largest raw outputs of each k
+            for t0 in (int(base), int(base) | 511):
+                assert (t0 < (int(t) << 9)) == ((t0 >> 9) < int(t))
