@@ -4,7 +4,8 @@
   steps from the minimal tiling, on the autotuned persistent configuration.
 * Lozenge hexagon a = b = c = 1000 (AlmostHexDomain(1000, 0), N = 2007), the q-rule with
   q = 0.999, from the empty room, on the autotuned packed configuration with tile skipping:
-  two runs totalling 407 steps (the generator and state hand-off between runs).
+  two runs totalling 407 steps (the generator and state hand-off between runs); and the same
+  workload at its stated length (10^6 sweeps), checked through the tiling's validity.
 * One run crossing the persistent kernel's launch boundary (2 * 2048 * h + 3 steps in one
   rt_chain_run) on a forced packed configuration with skipping on.
 * A tile grid too large to be resident (Aztec diamond of order 700) on the AUTO path.
@@ -77,6 +78,30 @@ def test_c2_hexagon1000_tuned(fctx):
     o = O.lozenge_walk(t0, 407, 20261203, default_table(t0.size), rule=rt.RT_LOZENGE_QREF, q=0.999)
     assert np.array_equal(g, o)
     assert lozenge_legal(g)
+
+
+def test_c2_hexagon1000_full_length(fctx):
+    """The hexagon 1000 workload at its stated length, 10^6 sweeps of the three colour classes
+    (3 * 10^6 chain steps, about 370 persistent launches with tile skipping), beyond what the
+    oracle can follow: the run must end without the device error flag (no change reached a
+    skipped tile) in a lozenge tiling of exactly the hexagon's faces, with cubes stacked."""
+    from tests.tilings import lozenge_cubes
+    d = S.call("lozenge.AlmostHexDomain", args=[1000, 0])
+    t0 = S.call("lozenge.MinTiling", d)
+    say("c2: hexagon 1000, 3e6 chain steps")
+    ch = fctx.chain(rt.RT_LOZENGE, rt.RT_LOZENGE_QREF, t0, [0.999])
+    try:
+        ch.autotune(16384)
+        ch.seed(20261204)
+        ch.run(3_000_000)
+        g = ch.read()
+        launches = ch.last_launches
+    finally:
+        ch.close()
+    say("c2 full length: launches", launches, "cubes", lozenge_cubes(g))
+    assert launches >= 3_000_000 // (2048 * 8)
+    assert lozenge_legal(g, d)
+    assert lozenge_cubes(g) > lozenge_cubes(t0)
 
 
 @pytest.mark.parametrize("h", [4, 8])

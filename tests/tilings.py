@@ -64,7 +64,22 @@ _LOZ_BIT = {(-1, 0): 0, (-1, 1): 1, (0, -1): 2, (0, 1): 3, (1, -1): 4, (1, 0): 5
 
 
 def lozenge_face_cover(t) -> np.ndarray:
-    """Crossed edges per triangle face (M x 2M, M = N-1)."""
+    """Crossed edges per triangle face (M x 2M, M = N-1), whole arrays at a time: face (i, 2k)
+    has corners (i, k), (i+1, k), (i, k+1), face (i, 2k+1) corners (i+1, k+1), (i+1, k),
+    (i, k+1); an edge is crossed when the bit of its direction is set at its first corner
+    (lozenge_face_cover_loop, the face-by-face form, is the reference for it)."""
+    t = _sq(t).astype(np.int64)
+    N = t.shape[0]
+    M = N - 1
+    cov = np.zeros((M, 2 * M), dtype=np.int32)
+    a, b, c = t[:M, :M], t[1:, :M], t[1:, 1:]
+    cov[:, 0::2] = ((a >> 5) & 1) + ((b >> 1) & 1) + ((a >> 3) & 1)  # (1,0), (-1,1), (0,1)
+    cov[:, 1::2] = ((c >> 2) & 1) + ((b >> 1) & 1) + (c & 1)          # (0,-1), (-1,1), (-1,0)
+    return cov
+
+
+def lozenge_face_cover_loop(t) -> np.ndarray:
+    """lozenge_face_cover one face at a time."""
     t = _sq(t)
     N = t.shape[0]
     M = N - 1
