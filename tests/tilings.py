@@ -129,8 +129,42 @@ def lozenge_cubes(t) -> int:
 # TriangleDimerTiler.cpp:711-735, inspects).
 
 def tridimer_edges(t):
-    """Decode (h, l, r) from a tiling and check that all four copies of every edge agree.
-    Returns (h, l, r, consistent)."""
+    """Decode (h, l, r) from a tiling and check that all four copies of every edge agree,
+    whole arrays at a time (tridimer_edges_loop, the state-by-state form, is the reference for
+    it).  Returns (h, l, r, consistent)."""
+    t = _sq(t).astype(np.int64)
+    N = t.shape[0]
+    E = N // 2
+    ok = not t[0::2, 0::2].any()
+    layers = {"h": [], "l": [], "r": []}
+
+    def put(name, x, y, v):
+        nonlocal ok
+        inside = (x >= 0) & (x <= E) & (y >= 0) & (y <= E)
+        ok &= not v[~inside].any()  # an edge outside the lattice must be absent
+        lay = np.full((E + 1, E + 1), -1, np.int64)
+        lay[x[inside], y[inside]] = v[inside]
+        layers[name].append(lay)
+
+    for (i0, j0), cells in (
+            ((0, 1), [("r", -1, 0), ("l", -1, 1), ("r", 0, 0), ("l", 0, 0)]),
+            ((1, 0), [("h", 0, 0), ("r", 0, 0), ("h", 1, -1), ("r", 0, -1)]),
+            ((1, 1), [("h", 0, 0), ("l", 0, 1), ("h", 1, 0), ("l", 0, 0)])):
+        s = t[i0::2, j0::2]
+        a, b = np.meshgrid(np.arange(s.shape[0]), np.arange(s.shape[1]), indexing="ij")
+        for k, (name, da, db) in enumerate(cells):
+            put(name, (a + da).ravel(), (b + db).ravel(), ((s >> k) & 1).ravel())
+    out = []
+    for name in ("h", "l", "r"):
+        st = np.stack(layers[name])
+        one, zero = (st == 1).any(0), (st == 0).any(0)
+        ok &= not (one & zero).any()
+        out.append(one.astype(np.int64))
+    return out[0], out[1], out[2], bool(ok)
+
+
+def tridimer_edges_loop(t):
+    """tridimer_edges one state at a time."""
     t = _sq(t).astype(np.int64)
     N = t.shape[0]
     E = N // 2
