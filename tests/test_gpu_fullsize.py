@@ -6,6 +6,8 @@
   q = 0.999, from the empty room, on the autotuned packed configuration with tile skipping:
   two runs totalling 407 steps (the generator and state hand-off between runs); and the same
   workload at its stated length (10^6 sweeps), checked through the tiling's validity.
+* Triangle dimers on the cone of side 1002 and a hexagon near side 2048 for long runs, checked
+  through the validity of the final dimer covers.
 * One run crossing the persistent kernel's launch boundary (2 * 2048 * h + 3 steps in one
   rt_chain_run) on a forced packed configuration with skipping on.
 * A tile grid too large to be resident (Aztec diamond of order 700) on the AUTO path.
@@ -102,6 +104,21 @@ def test_c2_hexagon1000_full_length(fctx):
     assert launches >= 3_000_000 // (2048 * 8)
     assert lozenge_legal(g, d)
     assert lozenge_cubes(g) > lozenge_cubes(t0)
+
+
+def test_tridimer_long_runs_stay_valid(fctx):
+    """Triangle dimers at full size for long runs, beyond the oracle's reach: the cone of side
+    1002 (the triangle-dimer workload) for 10^5 steps and a hexagon near tiling side 2048 (the
+    batched-domains mix) for 2 * 10^4 steps end in dimer covers of the same vertex sets."""
+    from randomtilings_amd import domains
+    from tests.tilings import tridimer_covered, tridimer_legal
+    cone = S.call("tridimer.IceCreamCone", args=[1002, 0])
+    hexagon = S.call("tridimer.Hexagon", args=list(domains.tridimer_hexagon_sides(2048)))
+    for name, t0, steps in (("cone 1002", cone, 100_000), ("hexagon", hexagon, 20_000)):
+        say("tridimer long run:", name, int(round(np.sqrt(t0.size))), steps)
+        g = fctx.tridimer_walk(t0, steps, 31337)
+        assert tridimer_legal(g, tridimer_covered(t0)), name
+        assert not np.array_equal(g, t0), name
 
 
 @pytest.mark.parametrize("h", [4, 8])
