@@ -1289,7 +1289,7 @@ struct rt_batch {
     size_t dent_cap = 0;
     TdBatchEntry* dtd = nullptr;  // triangle-dimer chains
     size_t dtd_cap = 0;
-    int64_t* ctr = nullptr;  // [0] step counter of the current group, [1..] scratch
+    int64_t* ctr = nullptr;  // [0] step counter of the current group, [1] the kernels' error word
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_up = nullptr;
     bool up_pending = false;
     std::vector<uint8_t> hsched;
@@ -1313,6 +1313,7 @@ int rt_batch_create(rt_ctx* ctx, rt_batch** out) {
     b->ctx = ctx;
     hipError_t e = hipSetDevice(ctx->device);
     if (e == hipSuccess) e = hipMalloc(&b->ctr, 64);
+    if (e == hipSuccess) e = hipMemset(b->ctr, 0, 64);  // step counter and the kernels' error word
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_up, hipEventDisableTiming);
@@ -1567,7 +1568,9 @@ int rt_batch_run(rt_batch* b, int64_t steps, int use_graph) {
         }
         return hipSuccess;
     };
-    HIPCHK(launch_fill_u32((uint32_t*)b->ctr, 4, 0u, c->stream));  // step counter and error word
+    // the step counter restarts; the error word (b->ctr[1]) keeps what any run raised until
+    // rt_batch_sync reports it
+    HIPCHK(launch_fill_u32((uint32_t*)b->ctr, 2, 0u, c->stream));
     bool round_ok = true;  // every block height divides the round
     for (const Group& g : groups) round_ok = round_ok && kRoundSteps % g.h == 0;
     const int64_t replays = use_graph && round_ok ? steps / kRoundSteps : 0;
@@ -1651,6 +1654,12 @@ int rt_batch_sync(rt_batch* b) {
     if (!b) return fail(RT_ERR_ARG, "rt_batch_sync: null batch");
     HIPCHK(hipSetDevice(b->ctx->device));
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    int64_t err = 0;
+    HIPCHK(hipMemcpy(&err, b->ctr + 1, sizeof(err), hipMemcpyDeviceToHost));
+    if (err != 0) {
+        HIPCHK(hipMemset(b->ctr + 1, 0, sizeof(err)));
+        return fail(RT_ERR_DEVICE, "rt_batch_sync: a batched launch raised the device error flag (%d)", (int)err);
+    }
     return RT_OK;
 }
 

@@ -506,7 +506,8 @@ struct Cfg {
     X(16, 44, 8, 512, 1)     \
     X(24, 28, 8, 512, 1)     \
     X(32, 44, 8, 768, 1)     \
-    X(40, 36, 8, 768, 1)
+    X(40, 36, 8, 768, 1)     \
+    X(48, 48, 8, 1024, 1)
 #define RT_LOZENGE_CFGS(X)  \
     X(64, 84, 8, 1024, 1)   \
     X(64, 84, 4, 1024, 1)   \

@@ -62,9 +62,19 @@ struct PackedDominoPair {
     __device__ __forceinline__ static int rkey(int gi) { return gi & 1; }
 };
 
-template <class PM, int TR, int TC, int H, int NT, bool UNI>
+// ONE: the launch runs a single block of at most H steps (batched launches, which stream every
+// tile through HBM per block), so the neighbour exchange, the skipped-tile check and the
+// drawing ahead (which only hides exchange latency) are compiled out, and their registers with
+// them; a launch of more steps raises the error flag (value 3) and runs nothing.
+template <class PM, int TR, int TC, int H, int NT, bool UNI, bool ONE = false>
 __device__ __forceinline__ void packed_body(const PersistArgs& a, const RuleParams& rules, const int tile,
                                             const int ntiles) {
+    if constexpr (ONE) {
+        if (a.nsteps > H) {
+            if (threadIdx.x == 0) __hip_atomic_store(a.err, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+    }
     using M = typename PM::Model;
     constexpr int NC = PM::NC;
     constexpr int ER = TR + 2 * H, EC = TC + 2 * H;
@@ -155,7 +165,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
     // this thread reloads (into registers), land() writes them to LDS once they arrive, so
     // work placed between the two overlaps the load latency.
     constexpr int QL = (AREAW + NT - 1) / NT;
-    uint32_t hv[QL][NC];
+    uint32_t hv[ONE ? 1 : QL][NC];
     auto halo_issue = [&](int bufi) __attribute__((always_inline)) {
         const uint32_t* src = reinterpret_cast<const uint32_t*>((bufi ? a.cls[1] : a.cls[0]));
 #pragma unroll
@@ -200,7 +210,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
     // skipped tile shows here: they are checked after every block and a change raises the
     // error flag (value 2) instead of going unnoticed.
     uint32_t wmask = 0;
-    if (a.live_map != nullptr) {
+    if (!ONE && a.live_map != nullptr) {
 #pragma unroll
         for (int g = 0; g < QPT; g++) {
             const int w = tid + g * NT;
@@ -324,7 +334,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
     // registers leave no room).  Tiles of one word per thread on up to 512 threads (small domains,
     // one wave per SIMD, where every draw made inside a step lies on the step's critical path)
     // draw the whole block ahead: H more words of registers.
-    constexpr int DA = !UNI ? 0 : (QPT == 1 && NT <= 512) ? H : H / 2;
+    constexpr int DA = (!UNI || ONE) ? 0 : (QPT == 1 && NT <= 512) ? H : H / 2;
     uint32_t XP[DA > 0 ? DA : 1][QPT];
     auto draw_ahead = [&](auto q0_c, auto q1_c, auto split_c, uint64_t cols) __attribute__((always_inline)) {
 #pragma unroll
@@ -495,7 +505,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
             }
         }
         buf ^= 1;
-        if (!last) {
+        if (!ONE && !last) {
             nblk++;
             const uint32_t epoch = a.epoch0 + nblk;
             // the next full block's targets are drawn while the published words drain and
@@ -558,7 +568,7 @@ __global__ __launch_bounds__(NT) void k_persist_loz4_batch(PersistArgs a_in) {
     PersistArgs a = a_in;
     int tile, ntiles;
     const RuleParams& rules = resolve_batch(a, tile, ntiles);
-    packed_body<PackedLozenge, TR, TC, H, NT, UNI>(a, rules, tile, ntiles);
+    packed_body<PackedLozenge, TR, TC, H, NT, UNI, true>(a, rules, tile, ntiles);
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>
@@ -571,7 +581,7 @@ __global__ __launch_bounds__(NT) void k_persist_dom4_batch(PersistArgs a_in) {
     PersistArgs a = a_in;
     int tile, ntiles;
     const RuleParams& rules = resolve_batch(a, tile, ntiles);
-    packed_body<PackedDomino, TR, TC, H, NT, UNI>(a, rules, tile, ntiles);
+    packed_body<PackedDomino, TR, TC, H, NT, UNI, true>(a, rules, tile, ntiles);
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>
