@@ -902,7 +902,7 @@ static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* re
         if (res && !blocked) {
             cost = ext * (1.0 + 1.0 / h) * (persist_config_packed(ch->model, i) ? 0.5 : 1.0);
         } else {
-            const int waves = persist_waves_per_cu(ch->model, i, uni, blocked);
+            const int waves = persist_waves_per_cu(ch->model, i, uni, blocked, true);
             const double occ = std::max(std::min(1.0, waves / 16.0), 1.0 / 64);
             cost = 1e9 + (redund + 4.0 / h) / occ;
         }
@@ -1742,7 +1742,7 @@ int rt_batch_autotune(rt_batch* b, int64_t probe_steps, int32_t* chosen) {
         for (int i = 0; i < ncfg && rc == RT_OK; i++) {
             if (m != RT_TRIDIMER) {
                 int tr, tc, h, nt;
-                if (!persist_config(m, i, &tr, &tc, &h, &nt) || persist_waves_per_cu(m, i, c->uniform, true) <= 0) continue;
+                if (!persist_config(m, i, &tr, &tc, &h, &nt) || persist_waves_per_cu(m, i, c->uniform, true, true) <= 0) continue;
                 if (persist_config_packed(m, i) && !all_packed_ok) continue;
             }
             // the group alone, on configuration i, in a batch of its own

@@ -135,7 +135,7 @@ bool persist_config(int model, int idx, int* tr, int* tc, int* h, int* nt);
 bool persist_config_packed(int model, int idx);
 int persist_capacity(int model, int idx, bool uni, int device);
 bool persist_uses_scratch(int model, int idx, bool uni);
-int persist_waves_per_cu(int model, int idx, bool uni, bool batch);
+int persist_waves_per_cu(int model, int idx, bool uni, bool batch, bool one);
 // coop: launch cooperatively, so that the runtime refuses a grid whose workgroups cannot all
 // be resident at once (hipErrorCooperativeLaunchTooLarge) instead of queuing tiles that would
 // wait on tiles that never start.

@@ -573,8 +573,14 @@ bool persist_config_packed(int model, int idx) {
     return t && idx >= 0 && idx < n && t[idx].packed != 0;
 }
 
+// `one`: the launch runs one block of at most h steps (a grid that is not resident).  The packed
+// domino and coupled-pair sweeps have kernels built for that, without the neighbour exchange and
+// with fewer registers (the 48 x 48 pair tile on 1024 threads fits two workgroups per CU: Aztec
+// diamond 2048 coupled steps 24.9 -> 17.5 us; single domino chains 2 % faster).  A lozenge chain
+// keeps the full kernel: the one-block build of the same tile measured slower (hexagon 3000 on
+// 32 x 44 halo-8 tiles, 4.38e11 against 4.58e11 flip attempts/s, at equal occupancy).
 template <class M, int TR, int TC, int H, int NT, int PACKED>
-static const void* kernel_ptr(bool uni, bool batch) {
+static const void* kernel_ptr(bool uni, bool batch, bool one) {
     if constexpr (PACKED != 0 && std::is_same<M, LozengeModel>::value) {
         if (batch)
             return uni ? (const void*)k_persist_loz4_batch<TR, TC, H, NT, true>
@@ -584,9 +590,15 @@ static const void* kernel_ptr(bool uni, bool batch) {
         if (batch)
             return uni ? (const void*)k_persist_dom4_batch<TR, TC, H, NT, true>
                        : (const void*)k_persist_dom4_batch<TR, TC, H, NT, false>;
+        if (one)
+            return uni ? (const void*)k_persist_dom4_blk<TR, TC, H, NT, true>
+                       : (const void*)k_persist_dom4_blk<TR, TC, H, NT, false>;
         return uni ? (const void*)k_persist_dom4<TR, TC, H, NT, true> : (const void*)k_persist_dom4<TR, TC, H, NT, false>;
     } else if constexpr (PACKED != 0 && std::is_same<M, DominoPairModel>::value) {
         if (batch) return nullptr;  // coupled pairs run alone (coupling from the past)
+        if (one)
+            return uni ? (const void*)k_persist_pair2_blk<TR, TC, H, NT, true>
+                       : (const void*)k_persist_pair2_blk<TR, TC, H, NT, false>;
         return uni ? (const void*)k_persist_pair2<TR, TC, H, NT, true> : (const void*)k_persist_pair2<TR, TC, H, NT, false>;
     } else if constexpr (PACKED != 0 && std::is_same<M, RectTriModel>::value) {
         // persist_body takes the row-wave path for this geometry by itself
@@ -606,10 +618,10 @@ static const void* kernel_ptr(bool uni, bool batch) {
     }
 }
 
-static const void* persist_kernel(int model, int idx, bool uni, bool batch = false) {
+static const void* persist_kernel(int model, int idx, bool uni, bool batch = false, bool one = false) {
     int i = 0;
 #define RT_PICK(M_, a, b, c, d, e)                              \
-    if (i++ == idx) return kernel_ptr<M_, a, b, c, d, e>(uni, batch);
+    if (i++ == idx) return kernel_ptr<M_, a, b, c, d, e>(uni, batch, one);
     switch (model) {
         case RT_DOMINO: {
 #define X(a, b, c, d, e) RT_PICK(DominoModel, a, b, c, d, e)
@@ -653,11 +665,11 @@ int persist_capacity(int model, int idx, bool uni, int device) {
     return std::min(per_cu, 1) * prop.multiProcessorCount;
 }
 
-// Waves per CU that launches of the configuration's kernel (the batched one with `batch`) keep
-// in flight, from the occupancy query: in a blocked run, where workgroups stream their tiles
-// through HBM, this is what hides the load latency.
-int persist_waves_per_cu(int model, int idx, bool uni, bool batch) {
-    const void* k = persist_kernel(model, idx, uni, batch);
+// Waves per CU that launches of the configuration's kernel (the batched one with `batch`, the
+// one-block one with `one`) keep in flight, from the occupancy query: in a blocked run, where
+// workgroups stream their tiles through HBM, this is what hides the load latency.
+int persist_waves_per_cu(int model, int idx, bool uni, bool batch, bool one) {
+    const void* k = persist_kernel(model, idx, uni, batch, one);
     int tr, tc, h, nt;
     if (!k || !persist_config(model, idx, &tr, &tc, &h, &nt)) return 0;
     int per_cu = 0;
@@ -678,9 +690,11 @@ bool persist_uses_scratch(int model, int idx, bool uni) {
 
 hipError_t launch_persist(int model, int idx, bool uni, const PersistArgs& a, int nblocks, hipStream_t st,
                           bool coop) {
-    const void* k = persist_kernel(model, idx, uni, a.batch != nullptr);
     int tr, tc, h, nt;
-    if (!k || !persist_config(model, idx, &tr, &tc, &h, &nt)) return hipErrorInvalidValue;
+    if (!persist_config(model, idx, &tr, &tc, &h, &nt)) return hipErrorInvalidValue;
+    // a plain launch of at most one block needs no exchange: the one-block kernels
+    const void* k = persist_kernel(model, idx, uni, a.batch != nullptr, !coop && a.nsteps <= h);
+    if (!k) return hipErrorInvalidValue;
     void* args[] = {(void*)&a};
     if (coop) return hipLaunchCooperativeKernel(k, dim3(nblocks), dim3(nt), args, 0, st);
     return hipLaunchKernel(k, dim3(nblocks), dim3(nt), args, 0, st);

@@ -62,11 +62,12 @@ struct PackedDominoPair {
     __device__ __forceinline__ static int rkey(int gi) { return gi & 1; }
 };
 
-// ONE: the launch runs a single block of at most H steps (batched launches, which stream every
-// tile through HBM per block), so the neighbour exchange, the skipped-tile check and the
-// drawing ahead (which only hides exchange latency) are compiled out, and their registers with
-// them; a launch of more steps raises the error flag (value 3) and runs nothing.
-template <class PM, int TR, int TC, int H, int NT, bool UNI, bool ONE = false>
+// ONE: the launch runs a single block of at most H steps (blocked launches, which stream every
+// tile through HBM per block), so the neighbour exchange and the drawing ahead (which only hides
+// exchange latency) are compiled out, and their registers with them; a launch of more steps
+// raises the error flag (value 3) and runs nothing.  SKIPCHK: the check of halo words inside
+// skipped neighbours is compiled in (batched launches never skip tiles and leave it out).
+template <class PM, int TR, int TC, int H, int NT, bool UNI, bool ONE = false, bool SKIPCHK = true>
 __device__ __forceinline__ void packed_body(const PersistArgs& a, const RuleParams& rules, const int tile,
                                             const int ntiles) {
     if constexpr (ONE) {
@@ -210,7 +211,7 @@ __device__ __forceinline__ void packed_body(const PersistArgs& a, const RulePara
     // skipped tile shows here: they are checked after every block and a change raises the
     // error flag (value 2) instead of going unnoticed.
     uint32_t wmask = 0;
-    if (!ONE && a.live_map != nullptr) {
+    if (SKIPCHK && a.live_map != nullptr) {
 #pragma unroll
         for (int g = 0; g < QPT; g++) {
             const int w = tid + g * NT;
@@ -568,7 +569,7 @@ __global__ __launch_bounds__(NT) void k_persist_loz4_batch(PersistArgs a_in) {
     PersistArgs a = a_in;
     int tile, ntiles;
     const RuleParams& rules = resolve_batch(a, tile, ntiles);
-    packed_body<PackedLozenge, TR, TC, H, NT, UNI, true>(a, rules, tile, ntiles);
+    packed_body<PackedLozenge, TR, TC, H, NT, UNI, true, false>(a, rules, tile, ntiles);
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>
@@ -576,17 +577,28 @@ __global__ __launch_bounds__(NT) void k_persist_dom4(PersistArgs a) {
     packed_body<PackedDomino, TR, TC, H, NT, UNI>(a, a.rp, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
 }
 
+// One block per launch (a tile grid that is not resident): the same sweep without the exchange.
+template <int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist_dom4_blk(PersistArgs a) {
+    packed_body<PackedDomino, TR, TC, H, NT, UNI, true>(a, a.rp, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+}
+
 template <int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist_dom4_batch(PersistArgs a_in) {
     PersistArgs a = a_in;
     int tile, ntiles;
     const RuleParams& rules = resolve_batch(a, tile, ntiles);
-    packed_body<PackedDomino, TR, TC, H, NT, UNI, true>(a, rules, tile, ntiles);
+    packed_body<PackedDomino, TR, TC, H, NT, UNI, true, false>(a, rules, tile, ntiles);
 }
 
 template <int TR, int TC, int H, int NT, bool UNI>
 __global__ __launch_bounds__(NT) void k_persist_pair2(PersistArgs a) {
     packed_body<PackedDominoPair, TR, TC, H, NT, UNI>(a, a.rp, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
+}
+
+template <int TR, int TC, int H, int NT, bool UNI>
+__global__ __launch_bounds__(NT) void k_persist_pair2_blk(PersistArgs a) {
+    packed_body<PackedDominoPair, TR, TC, H, NT, UNI, true>(a, a.rp, xcd_tile(blockIdx.x, gridDim.x), gridDim.x);
 }
 
 }  // namespace rt
