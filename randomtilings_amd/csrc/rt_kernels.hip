@@ -277,6 +277,12 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
             for (int c = 0; c < NC; c++) lds32[c * AREAW + q] = in ? xload(src + c * CSW + (size_t)gi * PW + gw) : 0u;
         }
     };
+    // The colours of a block (H <= 8 steps) come as one 8-byte word fetched ahead (the first
+    // before the tile loads, each next one before the block is published), so no step waits on a
+    // global load; every schedule buffer is padded past its last step.
+    static_assert(H <= 8, "a block's colours are fetched as one 8-byte word");
+    auto fetch_block = [&](int64_t at) { return gload(reinterpret_cast<const uint2*>(a.sched + at)); };
+    uint2 colours = fetch_block(0);
     load_ext(a.buf, false);
     __syncthreads();
 
@@ -317,8 +323,9 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
         // update only when c == c', which would overwrite what a neighbour may still read; the
         // block starts after a barrier.
         int c_prev = -1;
+        const uint32_t clo = (uint32_t)uniform((int)colours.x), chi = (uint32_t)uniform((int)colours.y);
         for (int64_t q = 0; q < blk; q++, s++) {
-            const int c = uniform((int)gload(a.sched + s));
+            const int c = (int)(((q < 4 ? clo : chi) >> (8 * (q & 3))) & 0xffu);
             T* lc = lds + c * AREA;
             if (c == c_prev) __syncthreads();
             c_prev = c;
@@ -397,6 +404,7 @@ __device__ __forceinline__ void persist_body(const PersistArgs& a, const RulePar
                 }
             }
         }
+        if (s < a.nsteps) colours = fetch_block(s);  // in flight while the block is published
         __syncthreads();  // every update of the block is in LDS before it is published
         // publish owned positions into the other copy
         {
