@@ -344,6 +344,8 @@ int rt_batch_create(rt_ctx* ctx, rt_batch** out);
 int rt_batch_destroy(rt_batch* b);
 int rt_batch_add(rt_batch* b, rt_chain* ch);
 int rt_batch_run(rt_batch* b, int64_t steps, int use_graph);
+/* Waits for the batch's runs; RT_ERR_DEVICE if a batched launch raised the device error flag
+ * since the last rt_batch_sync (the flag is then cleared). */
 int rt_batch_sync(rt_batch* b);
 /* Measure every tile configuration that applies to each model of the batch (probe_steps steps
  * of that model's chains alone, after a warm-up) and keep the fastest for later runs of the
