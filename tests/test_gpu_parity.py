@@ -446,18 +446,20 @@ def test_domino_cftp_parity(gctx, golden, mode, rule, p12):
 @pytest.mark.parametrize("rule,p12", [(rt.RT_DOMINO_CFTP, None), (rt.RT_DOMINO_WEIGHTED, [0.6, 0.4, 0.45, 0.55])])
 def test_domino_cftp_packed_configs(gctx, golden, cfg, rule, p12):
     """Coupled pairs on the packed sweep (two 16-bit positions per word) in every packed
-    configuration: the oracle's sample, replay length and restart count."""
+    configuration, persistent and blocked (the one-block pair kernel): the oracle's sample,
+    replay length and restart count."""
     for n, seed in [(16, 11), (32, 7)]:
         tmax, tmin = golden[f"domino_aztec{n}_max"], golden[f"domino_aztec{n}_min"]
-        say("cftp packed", cfg, rule, n)
-        gctx.set_path(rt.RT_PATH_PERSISTENT, cfg)
-        try:
-            g = gctx.domino_cftp(tmax, tmin, seed, rule=rule, p12=p12, mode=rt.RT_CFTP_DOUBLING, max_restarts=9)
-        finally:
-            gctx.set_path(rt.RT_PATH_AUTO, -1)
         o = O.domino_cftp(tmax, tmin, seed, default_table(BIG), rule=rule, p12=p12, mode=rt.RT_CFTP_DOUBLING,
                           max_restarts=9)
-        assert np.array_equal(g[0], o[0]) and g[1:] == o[1:], (n, g[1:], o[1:])
+        for path in (rt.RT_PATH_PERSISTENT, rt.RT_PATH_BLOCKED):
+            say("cftp packed", cfg, rule, n, path)
+            gctx.set_path(path, cfg)
+            try:
+                g = gctx.domino_cftp(tmax, tmin, seed, rule=rule, p12=p12, mode=rt.RT_CFTP_DOUBLING, max_restarts=9)
+            finally:
+                gctx.set_path(rt.RT_PATH_AUTO, -1)
+            assert np.array_equal(g[0], o[0]) and g[1:] == o[1:], (n, path, g[1:], o[1:])
 
 
 def test_domino_cftp_progress(gctx, golden):
