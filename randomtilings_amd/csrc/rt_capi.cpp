@@ -856,6 +856,18 @@ static bool packed_rule_ok(const rt_chain* ch) {
 // fits on the device at once (one launch runs many blocks of h steps, tiles exchanging
 // borders through flags); otherwise the chain runs "blocked": one launch per block of h
 // steps, the launch boundary being the exchange.
+// Resident grids are launched cooperatively unless RT_COOP_LAUNCH=0 is set in the environment
+// (read once): then as plain launches, for tools that do not survive cooperative launches
+// (rocprofv3 ends before writing its results).  Co-residency then rests on the one-workgroup-
+// per-CU capacity alone, and every neighbour wait stays bounded.
+static bool coop_launches() {
+    static const bool on = [] {
+        const char* v = std::getenv("RT_COOP_LAUNCH");
+        return !(v && std::strcmp(v, "0") == 0);
+    }();
+    return on;
+}
+
 static int choose_persist_cfg(rt_chain* ch, int* nblocks, int* tiles_c, bool* resident, bool no_packed = false,
                               bool blocked = false) {
     const int n = persist_config_count(ch->model);
@@ -1114,7 +1126,8 @@ int rt_chain_run(rt_chain* ch, int64_t steps) {
             a.live = runs ? runs->d_list : nullptr;
             a.live_map = runs ? runs->d_map : nullptr;
             a.ntiles = ch->last_tiles;
-            const hipError_t el = launch_persist(ch->model, cfg, c->uniform, a, nblocks, c->stream, resident);
+            const hipError_t el =
+                launch_persist(ch->model, cfg, c->uniform, a, nblocks, c->stream, resident && coop_launches());
             if (el != hipSuccess && resident) {
                 // Refused before anything ran: the grid cannot be resident
                 // (hipErrorCooperativeLaunchTooLarge), or the runtime cannot give this process
